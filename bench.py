@@ -1,0 +1,167 @@
+"""Benchmark: H_AC applies/s on the D = 1024 Hubbard U(1)xU(1) workload, mixed precision vs FP64
+on the same GPU.
+
+Workload: Fermi-Hubbard chain (t=1, U=4, half filling), (N, M) representation, two-site unit cell,
+d = 4, D = 1024 over 121 charge sectors of sizes 1..60.  One step = one grouped H_AC application
+(VUMPS_helpers.hpp:38-88) to a block of `nvec` Krylov vectors resident in HBM (one kernel launch).
+Operator data are seeded synthetic (isometry-scale normal entries, reduced operators computed on
+the GPU by ReducedOp).
+
+Prints ONE JSON line on stdout.  The value is vector-applies/s of the mixed-precision path
+(split-BF16 on MFMA, FP32 accumulation); FP64 and FP32 rates, single-vector rates, the roofline of
+the grouped kernel and the CPU oracle timed on the host (cpu_baseline) are reported alongside.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 157.3}  # bf16x3 is priced against FP32 MFMA
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nvec", type=int, default=16, help="Krylov vectors per batched apply")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU sharding is described, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import imps_cpp_tools_amd as im
+    from imps_cpp_tools_amd import workloads as wl
+
+    cfg = wl.HUBBARD_1024
+    L = wl.hac_workload_layouts(cfg)
+    rng = np.random.default_rng(1003)
+    H = im.model_ham(cfg["model"], cfg["params"])
+    # one default-priority stream shared by torch (uploads, events) and the library's launches
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx = im.Context(0, stream)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+    # operands: AL, AR at isometry scale, HL/HR by the GPU ReducedOp, EHL/EHR symmetric
+    AL = dev(wl.random_data(L["AL"], rng, scale_by_rows=True))
+    AR = dev(wl.random_data(L["AR"], rng, scale_by_rows=True))
+    rl = im.RedOpPlan(ctx, H, 4, L["AL"], im.LEFT)
+    rr = im.RedOpPlan(ctx, H, 4, L["AR"], im.RIGHT)
+    HLl, HRl = rl.out_layout, rr.out_layout
+    HL = torch.empty(HLl.nelem, dtype=torch.float64, device="cuda")
+    HR = torch.empty(HRl.nelem, dtype=torch.float64, device="cuda")
+    rl.apply(AL, HL)
+    rr.apply(AR, HR)
+    EHL = dev(wl.random_data(L["EHL"], rng, symmetric=True))
+    EHR = dev(wl.random_data(L["EHR"], rng, symmetric=True))
+    n = L["AC"].nelem
+    NV = args.nvec
+    X = dev(rng.standard_normal(n * NV))
+    torch.cuda.synchronize()
+
+    results = {}
+    for name, prec in [("fp64", im.FP64), ("fp32", im.FP32), ("bf16x3", im.BF16X3)]:
+        p = im.HACPlan(ctx, L["AC"], L["EHL"], L["EHR"], HLl, HRl, prec)
+        p.set_operator(EHL, EHR, HL, HR)
+        flops, bytes_ = p.cost()
+        Y = torch.empty(n * NV, dtype=torch.float64, device="cuda")
+        r = {}
+        for label, nv in [("batched", NV), ("single", 1)]:
+            for _ in range(args.warmup):
+                p.apply(X, Y, nv)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record(stream)
+            for _ in range(args.steps):
+                p.apply(X, Y, nv)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            ms = e0.elapsed_time(e1) / args.steps
+            r[label] = dict(nvec=nv, ms_per_launch=ms, applies_per_s=nv / (ms * 1e-3), wall_s=wall,
+                            tflops=nv * flops / (ms * 1e-3) / 1e12)
+        r["flops_per_apply"] = flops
+        r["bytes_per_apply"] = bytes_
+        r["y"] = Y[:n].cpu().numpy()
+        results[name] = r
+        log(f"{name}: batched {r['batched']['applies_per_s']:.4g} applies/s ({r['batched']['tflops']:.3g} TF/s), "
+            f"single {r['single']['applies_per_s']:.4g} applies/s, {r['batched']['ms_per_launch'] * 1e3:.1f} us/launch")
+
+    mixed, fp64 = results["bf16x3"], results["fp64"]
+    flops = mixed["flops_per_apply"]
+    achieved_tf = mixed["batched"]["tflops"]
+    out = {
+        "metric": "H_AC applies/s & VUMPS iter time, D=1024 Hubbard: mixed vs FP64, same GPU",
+        "value": mixed["batched"]["applies_per_s"],
+        "unit": "H_AC vector-applies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["batched"]["ms_per_launch"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3 (split-BF16 MFMA, FP32 accumulate; FP64 in/out)",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "H_AC apply, Fermi-Hubbard U(1)xU(1) (N,M), d=4, D=1024, 121 sectors of size 1..60",
+                   "nvec_per_step": NV, "flops_per_apply": flops, "ac_elements": n},
+        "mixed_vs_fp64": mixed["batched"]["applies_per_s"] / fp64["batched"]["applies_per_s"],
+        "fp64_applies_per_s": fp64["batched"]["applies_per_s"],
+        "fp32_applies_per_s": results["fp32"]["batched"]["applies_per_s"],
+        "single_vector_applies_per_s": {k: results[k]["single"]["applies_per_s"] for k in results},
+        "vumps_iter_s": None,
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_TFLOPS["bf16x3"], "unit": "TFLOP/s",
+                     "frac": achieved_tf / PEAK_TFLOPS["bf16x3"], "traffic": None,
+                     "kernel": "grouped_gemm_kernel<BF16X3,double>",
+                     "per_launch": {"flops": NV * flops, "us": mixed["batched"]["ms_per_launch"] * 1e3},
+                     "fp64_frac_of_fp64_peak": fp64["batched"]["tflops"] / PEAK_TFLOPS["fp64"]},
+    }
+
+    # CPU baseline: the oracle (clean-room FP64 restatement of ApplyHA) on the host, bounded sample
+    if not args.no_cpu:
+        from oracle import pyoracle as orc
+        lay_ob = lambda l, d: orc.OB.like(l, d)  # noqa: E731
+        ob = dict(EHL=lay_ob(L["EHL"], EHL.cpu().numpy()), EHR=lay_ob(L["EHR"], EHR.cpu().numpy()),
+                  HL=lay_ob(HLl, HL.cpu().numpy()), HR=lay_ob(HRl, HR.cpu().numpy()), AC=lay_ob(L["AC"], X[:n].cpu().numpy()))
+        hs = {k: v.handle() for k, v in ob.items()}
+        lib = orc.lib()
+        reps, t0 = 0, time.perf_counter()
+        yref = None
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            h = lib.orc_hac(hs["EHL"].h, hs["EHR"].h, hs["HL"].h, hs["HR"].h, hs["AC"].h)
+            if yref is None:
+                yref = orc._H(h).get().data
+            else:
+                lib.orc_free(h)
+            reps += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": reps / dt, "unit": "H_AC vector-applies/s", "cores": 1, "kind": "port",
+                               "sample": f"{reps} single-vector H_AC applies of the same workload ({dt:.1f} s)"}
+        nr = np.linalg.norm(yref)
+        out["err_vs_oracle"] = {k: float(np.linalg.norm(results[k]["y"] - yref) / nr) for k in results}
+        log(f"cpu oracle: {reps / dt:.3g} applies/s; errors {out['err_vs_oracle']}")
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,423 @@
+"""MI355X block-sparse iMPS operator path.
+
+Python view of the C ABI in include/imps_capi.h (libimps_hip.so, built in-tree by
+``imps_cpp_tools_amd.build``).  The operator applications run as grouped HIP kernels on gfx950;
+there is no CPU fallback: loading fails loudly when the library is missing, and every apply
+needs a GPU context.
+
+Layouts follow the reference's Vectorize order (BlockObj.hpp:764-780, MPSBlockMat.hpp:446-465).
+Device buffers are torch float64 tensors on ``cuda:N``; torch is used only for device memory.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libimps_hip.so")
+
+FP64, FP32, BF16X3 = 0, 1, 2
+LEFT, RIGHT = 0, 1
+PREC_NAMES = {FP64: "fp64", FP32: "fp32", BF16X3: "bf16x3"}
+XXZ, FHUB = 0, 1
+
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "NOT_CONVERGED", 3: "NUMERICAL", 4: "HIP_ERROR", 5: "NO_DEVICE"}
+
+
+class ImpsError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _CLayout(ctypes.Structure):
+    _fields_ = [("nsym", ctypes.c_int), ("nlist", ctypes.c_int), ("nblk", ctypes.POINTER(ctypes.c_int)),
+                ("qin", ctypes.POINTER(ctypes.c_int)), ("qout", ctypes.POINTER(ctypes.c_int)),
+                ("rows", ctypes.POINTER(ctypes.c_int)), ("cols", ctypes.POINTER(ctypes.c_int))]
+
+
+_lib = None
+
+
+def lib():
+    """Load libimps_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m imps_cpp_tools_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, D, V = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        LP = ctypes.POINTER(_CLayout)
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        sig = {
+            "imps_last_error": (ctypes.c_char_p, []),
+            "imps_version": (ctypes.c_char_p, []),
+            "imps_create": (I, [ctypes.POINTER(P), I]),
+            "imps_destroy": (I, [P]),
+            "imps_set_stream": (I, [P, P]),
+            "imps_synchronize": (I, [P]),
+            "imps_plan_destroy": (I, [P]),
+            "imps_plan_cost": (I, [P, dp, dp]),
+            "imps_plan_shape": (I, [P, ip, ip]),
+            "imps_plan_out_layout": (I, [P, ip, ip, ctypes.POINTER(ctypes.c_long), ip, ip, ip, ip, ip]),
+            "imps_tm_plan": (I, [P, ctypes.POINTER(P), I, LP, LP, LP, LP, I]),
+            "imps_tm_set_mps": (I, [P, V, V]),
+            "imps_tm_apply": (I, [P, V, V, I]),
+            "imps_hac_plan": (I, [P, ctypes.POINTER(P), LP, LP, LP, LP, LP, I]),
+            "imps_hac_set_operator": (I, [P, V, V, V, V]),
+            "imps_hac_apply": (I, [P, V, V, I]),
+            "imps_hc_plan": (I, [P, ctypes.POINTER(P), LP, LP, LP, LP, LP, I]),
+            "imps_hc_set_operator": (I, [P, V, V, V, V]),
+            "imps_hc_apply": (I, [P, V, V, I]),
+            "imps_redop_plan": (I, [P, ctypes.POINTER(P), dp, I, LP, I, I]),
+            "imps_redop_apply": (I, [P, V, V]),
+            "imps_model_nsym": (I, [I]),
+            "imps_model_d": (I, [I]),
+            "imps_model_i2k": (I, [I, I, ip, ip]),
+            "imps_model_validpath": (I, [I, I, ip, ip]),
+            "imps_model_dimmap": (I, [I, I, ip, ip, I, ip, ip]),
+            "imps_model_make_dims": (I, [I, I, ip, I, ip, ip, ip, ip]),
+            "imps_model_ham": (I, [I, dp, dp]),
+            "imps_mps_structure": (I, [I, I, ip, I, ip, ip, I, ip, ip, ip, ip, ip, ip, ip]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status: int):
+    if status != 0:
+        raise ImpsError(status, lib().imps_last_error().decode())
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ----------------------------------------------------------------------------------------------
+@dataclass
+class BlockLayout:
+    """Structure of a block-sparse object: nlist entries (1 = BlockMat/BlockDiagMat, d = MPS,
+    d*d = RedOp), each a list of blocks in ascending ingoing charge."""
+    nsym: int
+    nblk: np.ndarray
+    qin: np.ndarray
+    qout: np.ndarray
+    rows: np.ndarray
+    cols: np.ndarray
+
+    def __post_init__(self):
+        self.nblk = np.ascontiguousarray(self.nblk, dtype=np.int32).reshape(-1)
+        nb = int(self.nblk.sum())
+        self.qin = np.ascontiguousarray(self.qin, dtype=np.int32).reshape(nb, self.nsym)
+        self.qout = np.ascontiguousarray(self.qout, dtype=np.int32).reshape(nb, self.nsym)
+        self.rows = np.ascontiguousarray(self.rows, dtype=np.int32).reshape(nb)
+        self.cols = np.ascontiguousarray(self.cols, dtype=np.int32).reshape(nb)
+
+    @property
+    def nlist(self) -> int:
+        return len(self.nblk)
+
+    @property
+    def sizes(self) -> np.ndarray:
+        return self.rows.astype(np.int64) * self.cols.astype(np.int64)
+
+    @property
+    def nelem(self) -> int:
+        return int(self.sizes.sum())
+
+    @property
+    def offsets(self) -> np.ndarray:
+        return np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(np.int64)
+
+    def entry(self, e: int) -> "BlockLayout":
+        s = int(self.nblk[:e].sum())
+        n = int(self.nblk[e])
+        return BlockLayout(self.nsym, [n], self.qin[s:s + n], self.qout[s:s + n], self.rows[s:s + n], self.cols[s:s + n])
+
+    def entry_range(self, e: int):
+        """Element range [a, b) of list entry e in the flat buffer."""
+        s = int(self.nblk[:e].sum())
+        n = int(self.nblk[e])
+        a = int(self.sizes[:s].sum())
+        return a, a + int(self.sizes[s:s + n].sum())
+
+    def c(self) -> _CLayout:
+        self._keep = (self.nblk, self.qin, self.qout, self.rows, self.cols)
+        return _CLayout(self.nsym, self.nlist, _iptr(self.nblk), _iptr(self.qin), _iptr(self.qout), _iptr(self.rows), _iptr(self.cols))
+
+    def __eq__(self, o):
+        return (self.nsym == o.nsym and np.array_equal(self.nblk, o.nblk) and np.array_equal(self.qin, o.qin)
+                and np.array_equal(self.qout, o.qout) and np.array_equal(self.rows, o.rows) and np.array_equal(self.cols, o.cols))
+
+    @staticmethod
+    def concat(parts: Sequence["BlockLayout"]) -> "BlockLayout":
+        return BlockLayout(parts[0].nsym, np.concatenate([p.nblk for p in parts]), np.concatenate([p.qin for p in parts]),
+                           np.concatenate([p.qout for p in parts]), np.concatenate([p.rows for p in parts]),
+                           np.concatenate([p.cols for p in parts]))
+
+    @staticmethod
+    def diag(nsym: int, keys: np.ndarray, sizes: np.ndarray) -> "BlockLayout":
+        keys = np.asarray(keys, dtype=np.int32).reshape(-1, nsym)
+        return BlockLayout(nsym, [len(keys)], keys, keys, sizes, sizes)
+
+
+# ----------------------------------------------------------------------------------------------
+# bookkeeping (host only)
+def model_nsym(model: int) -> int:
+    return lib().imps_model_nsym(model)
+
+
+def model_d(model: int) -> int:
+    return lib().imps_model_d(model)
+
+
+def i2k(model: int, N: int, QN: Sequence[int]) -> np.ndarray:
+    ns, d = model_nsym(model), model_d(model)
+    out = np.zeros(d * ns, np.int32)
+    q = np.asarray(QN, np.int32)
+    r = lib().imps_model_i2k(model, N, _iptr(q), _iptr(out))
+    if r < 0:
+        raise ImpsError(1, "imps_model_i2k failed")
+    return out.reshape(d, ns)
+
+
+def validpath(model: int, N: int, QN: Sequence[int]) -> np.ndarray:
+    out = np.zeros(N, np.int32)
+    q = np.asarray(QN, np.int32)
+    if lib().imps_model_validpath(model, N, _iptr(q), _iptr(out)) < 0:
+        raise ImpsError(1, "imps_model_validpath failed")
+    return out
+
+
+def dimmap(model: int, N: int, QN: Sequence[int], dims: Sequence[int]):
+    ns = model_nsym(model)
+    nmax = len(dims) ** (1 if model == XXZ else 2)
+    keys = np.zeros(nmax * ns, np.int32)
+    sizes = np.zeros(nmax, np.int32)
+    q = np.asarray(QN, np.int32)
+    dd = np.asarray(dims, np.int32)
+    n = lib().imps_model_dimmap(model, N, _iptr(q), _iptr(dd), len(dd), _iptr(keys), _iptr(sizes))
+    if n < 0:
+        raise ImpsError(1, "imps_model_dimmap failed")
+    return keys[:n * ns].reshape(n, ns), sizes[:n]
+
+
+def make_dims(model: int, N: int, QN: Sequence[int], keys0: np.ndarray, sizes0: np.ndarray):
+    """MakeDims from a bond-0 dim map; returns a list of N (keys, sizes) bonds."""
+    ns = model_nsym(model)
+    k0 = np.ascontiguousarray(keys0, np.int32).reshape(-1)
+    s0 = np.ascontiguousarray(sizes0, np.int32)
+    nsec = len(s0)
+    keys = np.zeros(N * nsec * ns, np.int32)
+    sizes = np.zeros(N * nsec, np.int32)
+    q = np.asarray(QN, np.int32)
+    if lib().imps_model_make_dims(model, N, _iptr(q), nsec, _iptr(k0), _iptr(s0), _iptr(keys), _iptr(sizes)) < 0:
+        raise ImpsError(1, "imps_model_make_dims failed")
+    keys = keys.reshape(N, nsec, ns)
+    sizes = sizes.reshape(N, nsec)
+    return [(keys[k], sizes[k]) for k in range(N)]
+
+
+def model_ham(model: int, params: Sequence[float]) -> np.ndarray:
+    d = model_d(model)
+    p = np.asarray(params, np.float64)
+    H = np.zeros(d * d * d * d, np.float64)
+    if lib().imps_model_ham(model, _dptr(p), _dptr(H)) < 0:
+        raise ImpsError(1, "imps_model_ham failed")
+    return H.reshape(d * d, d * d, order="F")
+
+
+def mps_structure(model: int, N: int, QN: Sequence[int], left, right) -> BlockLayout:
+    ns, d = model_nsym(model), model_d(model)
+    kl, sl = (np.ascontiguousarray(a, np.int32) for a in left)
+    kr, sr = (np.ascontiguousarray(a, np.int32) for a in right)
+    cap = d * len(sl)
+    nblk = np.zeros(d, np.int32)
+    qin = np.zeros(cap * ns, np.int32)
+    qout = np.zeros(cap * ns, np.int32)
+    rows = np.zeros(cap, np.int32)
+    cols = np.zeros(cap, np.int32)
+    q = np.asarray(QN, np.int32)
+    n = lib().imps_mps_structure(model, N, _iptr(q), len(sl), _iptr(kl.reshape(-1)), _iptr(sl), len(sr), _iptr(kr.reshape(-1)),
+                                 _iptr(sr), _iptr(nblk), _iptr(qin), _iptr(qout), _iptr(rows), _iptr(cols))
+    if n < 0:
+        raise ImpsError(1, "imps_mps_structure failed")
+    return BlockLayout(ns, nblk, qin[:n * ns], qout[:n * ns], rows[:n], cols[:n])
+
+
+# ----------------------------------------------------------------------------------------------
+# device context and plans
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8 and t.is_contiguous()):
+        raise ValueError("device buffers must be contiguous float64 CUDA tensors")
+    return t.data_ptr()
+
+
+class Context:
+    """A device context (one GPU, one default-priority stream)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        """`stream`: a torch.cuda.Stream (or raw hipStream_t int); default: torch's current stream,
+        so that library launches are ordered with torch's own copies and kernels."""
+        h = ctypes.c_void_p()
+        _check(lib().imps_create(ctypes.byref(h), device))
+        self.h = h
+        self.device = device
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream(device)
+        self.set_stream(stream)
+
+    def set_stream(self, stream):
+        raw = stream if isinstance(stream, int) else stream.cuda_stream
+        _check(lib().imps_set_stream(self.h, ctypes.c_void_p(raw)))
+
+    def synchronize(self):
+        _check(lib().imps_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().imps_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    kind = ""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def cost(self):
+        f, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().imps_plan_cost(self.h, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
+
+    def shape(self):
+        a, t = ctypes.c_int(), ctypes.c_int()
+        _check(lib().imps_plan_shape(self.h, ctypes.byref(a), ctypes.byref(t)))
+        return a.value, t.value
+
+    def close(self):
+        if self.h:
+            lib().imps_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ctx_handle(ctx: Optional[Context]):
+    return ctx.h if ctx is not None else None
+
+
+class TMPlan(Plan):
+    """Mixed / regular transfer-matrix apply (TMBlockFunctions.hpp:352-390, :494-533)."""
+    kind = "tm"
+
+    def __init__(self, ctx: Optional[Context], direction: int, A: BlockLayout, B: Optional[BlockLayout], x: BlockLayout,
+                 y: BlockLayout, prec: int = FP64):
+        h = ctypes.c_void_p()
+        cA, cx, cy = A.c(), x.c(), y.c()
+        cB = B.c() if B is not None else None
+        _check(lib().imps_tm_plan(_ctx_handle(ctx), ctypes.byref(h), direction, ctypes.byref(cA),
+                                  ctypes.byref(cB) if cB is not None else None, ctypes.byref(cx), ctypes.byref(cy), prec))
+        super().__init__(h)
+        self.mixed = B is not None
+        self.xlen, self.ylen = x.nelem, y.nelem
+
+    def set_mps(self, A, B=None):
+        _check(lib().imps_tm_set_mps(self.h, _ptr(A), _ptr(B)))
+
+    def apply(self, x, y, nvec: int = 1):
+        _check(lib().imps_tm_apply(self.h, _ptr(x), _ptr(y), nvec))
+
+
+class HACPlan(Plan):
+    """Two-site H_AC apply (VUMPS_helpers.hpp:38-88)."""
+    kind = "hac"
+
+    def __init__(self, ctx: Optional[Context], AC: BlockLayout, EHL: BlockLayout, EHR: BlockLayout, HL: BlockLayout,
+                 HR: BlockLayout, prec: int = FP64):
+        h = ctypes.c_void_p()
+        c = [l.c() for l in (AC, EHL, EHR, HL, HR)]
+        _check(lib().imps_hac_plan(_ctx_handle(ctx), ctypes.byref(h), *[ctypes.byref(x) for x in c], prec))
+        super().__init__(h)
+        self.n = AC.nelem
+
+    def set_operator(self, EHL, EHR, HL, HR):
+        _check(lib().imps_hac_set_operator(self.h, _ptr(EHL), _ptr(EHR), _ptr(HL), _ptr(HR)))
+
+    def apply(self, x, y, nvec: int = 1):
+        _check(lib().imps_hac_apply(self.h, _ptr(x), _ptr(y), nvec))
+
+
+class HCPlan(Plan):
+    """Bond H_C apply (VUMPS_helpers.hpp:91-133) in reduced form with HL = ReducedOp(H, AL, l)."""
+    kind = "hc"
+
+    def __init__(self, ctx: Optional[Context], C: BlockLayout, EHL: BlockLayout, EHR: BlockLayout, HL: BlockLayout,
+                 AR: BlockLayout, prec: int = FP64):
+        h = ctypes.c_void_p()
+        c = [l.c() for l in (C, EHL, EHR, HL, AR)]
+        _check(lib().imps_hc_plan(_ctx_handle(ctx), ctypes.byref(h), *[ctypes.byref(x) for x in c], prec))
+        super().__init__(h)
+        self.n = C.nelem
+
+    def set_operator(self, EHL, EHR, HL, AR):
+        _check(lib().imps_hc_set_operator(self.h, _ptr(EHL), _ptr(EHR), _ptr(HL), _ptr(AR)))
+
+    def apply(self, x, y, nvec: int = 1):
+        _check(lib().imps_hc_apply(self.h, _ptr(x), _ptr(y), nvec))
+
+
+class RedOpPlan(Plan):
+    """ReducedOp(H, A, dir) (MPSBlockUtilities.hpp:39-74); output layout derived from A."""
+    kind = "redop"
+
+    def __init__(self, ctx: Optional[Context], H: np.ndarray, d: int, A: BlockLayout, direction: int, prec: int = FP64):
+        h = ctypes.c_void_p()
+        Hf = np.asfortranarray(H, dtype=np.float64).reshape(-1, order="F").copy()
+        cA = A.c()
+        _check(lib().imps_redop_plan(_ctx_handle(ctx), ctypes.byref(h), _dptr(Hf), d, ctypes.byref(cA), direction, prec))
+        super().__init__(h)
+        self.out_layout = self._out_layout(A.nsym)
+
+    def _out_layout(self, nsym: int) -> BlockLayout:
+        nl, nb, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        z = ctypes.POINTER(ctypes.c_int)()
+        _check(lib().imps_plan_out_layout(self.h, ctypes.byref(nl), ctypes.byref(nb), ctypes.byref(ne), z, z, z, z, z))
+        nblk = np.zeros(nl.value, np.int32)
+        qin = np.zeros(max(nb.value, 1) * nsym, np.int32)
+        qout = np.zeros(max(nb.value, 1) * nsym, np.int32)
+        rows = np.zeros(max(nb.value, 1), np.int32)
+        cols = np.zeros(max(nb.value, 1), np.int32)
+        _check(lib().imps_plan_out_layout(self.h, None, None, None, _iptr(nblk), _iptr(qin), _iptr(qout), _iptr(rows), _iptr(cols)))
+        n = nb.value
+        return BlockLayout(nsym, nblk, qin[:n * nsym], qout[:n * nsym], rows[:n], cols[:n])
+
+    def apply(self, A, y):
+        _check(lib().imps_redop_apply(self.h, _ptr(A), _ptr(y)))

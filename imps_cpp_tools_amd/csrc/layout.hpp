@@ -1,0 +1,432 @@
+// Host-side sector bookkeeping and grouped block-GEMM programs.
+//
+// A block-sparse object (BlockMat, BlockDiagMat, MPSBlockMat, RedOp) lives in one flat buffer in
+// the reference's Vectorize order; its structure is a Layout: per list entry, the blocks sorted by
+// ingoing charge with their element offsets.  Charge lookups replace the reference's
+// std::map::find (BlockObj.hpp:2557-2625, TMBlockFunctions.hpp) by binary search over the sorted
+// block table, done once per plan on the host.
+//
+// Every operator application is compiled, once per sector structure, into a Program: a short
+// sequence of Stages, each one grouped launch.  A Stage is a list of Tasks, each owning one output
+// tile (<= kTile x kTile) of one output block, and each Task sums a list of Segments
+// C_tile += alpha * op(A) * op(B) taken from the operand slots (input vector, operator store,
+// intermediate workspace).  Because a task owns its tile and sums its segments in a fixed order,
+// results are deterministic and need no atomics.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/imps_capi.h"
+
+namespace imps {
+
+constexpr int kMaxSym = 4;
+constexpr int kTile = 64;  // output tile edge of the grouped GEMM kernel
+
+struct InvalidArg : std::runtime_error { using std::runtime_error::runtime_error; };
+
+// Charge key: nsym U(1) charges compared lexicographically (std::vector<int> order, KeyTypes.hpp:19-58).
+struct Key {
+    int n = 0;
+    int q[kMaxSym] = {0, 0, 0, 0};
+    Key() {}
+    Key(const int* p, int nsym) : n(nsym) { for (int i = 0; i < nsym; ++i) q[i] = p[i]; }
+    bool operator<(const Key& o) const { for (int i = 0; i < n; ++i) if (q[i] != o.q[i]) return q[i] < o.q[i]; return false; }
+    bool operator==(const Key& o) const { for (int i = 0; i < n; ++i) if (q[i] != o.q[i]) return false; return true; }
+    bool operator!=(const Key& o) const { return !(*this == o); }
+    Key operator+(const Key& o) const { Key r(*this); for (int i = 0; i < n; ++i) r.q[i] += o.q[i]; return r; }
+    Key operator-(const Key& o) const { Key r(*this); for (int i = 0; i < n; ++i) r.q[i] -= o.q[i]; return r; }
+};
+
+struct Blk {
+    Key qin, qout;
+    int rows = 0, cols = 0;
+    long off = 0;  // element offset in the object's flat buffer
+    long size() const { return (long)rows * cols; }
+};
+
+// One BlockMat: blocks in ascending qin.
+struct BList {
+    std::vector<Blk> b;
+    const Blk* find(const Key& k) const {
+        auto it = std::lower_bound(b.begin(), b.end(), k, [](const Blk& x, const Key& y) { return x.qin < y; });
+        return (it != b.end() && it->qin == k) ? &*it : nullptr;
+    }
+    // dK = Qout - Qin of the slice (BlockMat::dK); false for an empty slice
+    bool dK(Key& k) const { if (b.empty()) return false; k = b.front().qout - b.front().qin; return true; }
+};
+
+struct Layout {
+    int nsym = 1;
+    std::vector<BList> l;
+    long nelem = 0;
+    size_t nblocks() const { size_t n = 0; for (auto& x : l) n += x.b.size(); return n; }
+};
+
+inline Layout layout_from_c(const imps_layout* c, const char* what) {
+    if (!c) throw InvalidArg(std::string(what) + ": layout is NULL");
+    if (c->nsym < 1 || c->nsym > kMaxSym) throw InvalidArg(std::string(what) + ": nsym must be 1..4");
+    if (c->nlist < 1) throw InvalidArg(std::string(what) + ": nlist must be >= 1");
+    if (!c->nblk) throw InvalidArg(std::string(what) + ": nblk is NULL");
+    Layout L;
+    L.nsym = c->nsym;
+    L.l.resize(c->nlist);
+    long pos = 0;
+    size_t k = 0;
+    for (int e = 0; e < c->nlist; ++e) {
+        if (c->nblk[e] < 0) throw InvalidArg(std::string(what) + ": negative block count");
+        if (c->nblk[e] > 0 && (!c->qin || !c->qout || !c->rows || !c->cols)) throw InvalidArg(std::string(what) + ": NULL block arrays");
+        for (int i = 0; i < c->nblk[e]; ++i, ++k) {
+            Blk b;
+            b.qin = Key(c->qin + k * c->nsym, c->nsym);
+            b.qout = Key(c->qout + k * c->nsym, c->nsym);
+            b.rows = c->rows[k];
+            b.cols = c->cols[k];
+            if (b.rows < 0 || b.cols < 0) throw InvalidArg(std::string(what) + ": negative block size");
+            if (!L.l[e].b.empty() && !(L.l[e].b.back().qin < b.qin))
+                throw InvalidArg(std::string(what) + ": blocks must be in strictly ascending ingoing-charge order");
+            b.off = pos;
+            pos += b.size();
+            L.l[e].b.push_back(b);
+        }
+    }
+    L.nelem = pos;
+    return L;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Grouped GEMM programs
+enum Slot : int { SLOT_X = 0, SLOT_OP = 1, SLOT_W = 2, SLOT_OUT = 3 };
+
+// C_tile += alpha * op(A)[m0:, :K] * op(B)[:K, n0:]; op(A)(i,k) = tr ? S[k + i*ld] : S[i + k*ld]
+struct Seg {
+    long a_off, b_off;
+    int a_slot, a_ld, a_tr;
+    int b_slot, b_ld, b_tr;
+    int K;
+    double alpha;
+};
+struct Task {
+    long c_off;
+    int c_ld, m0, n0, M, N;
+    int seg0, nseg;
+};
+struct Stage {
+    int out_slot = SLOT_OUT;   // SLOT_OUT: final output; SLOT_W: intermediate workspace
+    std::vector<Task> tasks;
+    std::vector<Seg> segs;
+};
+struct Program {
+    std::vector<Stage> stages;
+    long w_elems = 0;    // per-vector workspace elements
+    double flops = 0;    // algorithmic flops per single-vector apply
+    long x_elems = 0, y_elems = 0, op_elems = 0;
+    long operand_elems_read = 0;  // algorithmic operand reads (each operand block counted once)
+};
+
+// An output block being assembled from segments.
+struct OutBlock {
+    long c_off;
+    int M, N;
+    std::vector<Seg> segs;
+};
+
+struct Operand {
+    int slot;
+    long off;
+    int rows, cols;  // stored shape
+    bool tr;         // use the transpose of the stored block
+    int orows() const { return tr ? cols : rows; }
+    int ocols() const { return tr ? rows : cols; }
+};
+inline Operand opnd(int slot, const Blk& b, long base, bool tr = false) { return Operand{slot, base + b.off, b.rows, b.cols, tr}; }
+
+// Append alpha*op(A)*op(B) to an output block.
+inline void add_seg(OutBlock& ob, const Operand& A, const Operand& B, double alpha, double& flops) {
+    if (A.orows() != ob.M || B.ocols() != ob.N || A.ocols() != B.orows())
+        throw InvalidArg("block sizes of matched sectors disagree");
+    Seg s;
+    s.a_off = A.off; s.a_slot = A.slot; s.a_ld = std::max(A.rows, 1); s.a_tr = A.tr ? 1 : 0;
+    s.b_off = B.off; s.b_slot = B.slot; s.b_ld = std::max(B.rows, 1); s.b_tr = B.tr ? 1 : 0;
+    s.K = A.ocols();
+    s.alpha = alpha;
+    if (s.K > 0 && ob.M > 0 && ob.N > 0) {
+        ob.segs.push_back(s);
+        flops += 2.0 * ob.M * ob.N * s.K;
+    }
+}
+
+// Tile the assembled output blocks into tasks of a stage.
+inline void emit_stage(Program& P, std::vector<OutBlock>& obs, int out_slot) {
+    Stage st;
+    st.out_slot = out_slot;
+    for (auto& ob : obs) {
+        if (ob.M == 0 || ob.N == 0) continue;
+        int seg0 = (int)st.segs.size();
+        st.segs.insert(st.segs.end(), ob.segs.begin(), ob.segs.end());
+        for (int m0 = 0; m0 < ob.M; m0 += kTile)
+            for (int n0 = 0; n0 < ob.N; n0 += kTile)
+                st.tasks.push_back(Task{ob.c_off, ob.M, m0, n0, ob.M, ob.N, seg0, (int)ob.segs.size()});
+    }
+    P.stages.push_back(std::move(st));
+}
+
+inline void require_same_nsym(const Layout& a, const Layout& b, const char* what) {
+    if (a.nsym != b.nsym) throw InvalidArg(std::string(what) + ": layouts disagree on nsym");
+}
+
+// ---------------------------------------------------------------------------------------------
+// H_AC (VUMPS_helpers.hpp:38-53): out[i] = EHL*in[i] + in[i]*EHR + sum_j HL[i][j]*in[j] + in[j]*HR[i][j]
+// Operator store layout: [EHL | EHR | HL | HR] in the caller's Vectorize order.
+inline Program build_hac(const Layout& AC, const Layout& EHL, const Layout& EHR, const Layout& HL, const Layout& HR) {
+    const int d = (int)AC.l.size();
+    require_same_nsym(AC, EHL, "hac"); require_same_nsym(AC, EHR, "hac");
+    require_same_nsym(AC, HL, "hac"); require_same_nsym(AC, HR, "hac");
+    if (EHL.l.size() != 1 || EHR.l.size() != 1) throw InvalidArg("hac: EHL/EHR must have nlist = 1");
+    if ((int)HL.l.size() != d * d || (int)HR.l.size() != d * d) throw InvalidArg("hac: HL/HR must have nlist = d*d");
+    Program P;
+    const long ehl0 = 0, ehr0 = EHL.nelem, hl0 = ehr0 + EHR.nelem, hr0 = hl0 + HL.nelem;
+    P.op_elems = hr0 + HR.nelem;
+    P.x_elems = P.y_elems = AC.nelem;
+    std::vector<OutBlock> obs;
+    for (int i = 0; i < d; ++i)
+        for (const Blk& ob : AC.l[i].b) {
+            OutBlock o{ob.off, ob.rows, ob.cols, {}};
+            // EHL*in + in*EHR (BlockDiagMat x BlockMat products, BlockObj.hpp:2583-2606)
+            if (const Blk* e = EHL.l[0].find(ob.qin)) add_seg(o, opnd(SLOT_OP, *e, ehl0), opnd(SLOT_X, ob, 0), 1.0, P.flops);
+            if (const Blk* e = EHR.l[0].find(ob.qout)) add_seg(o, opnd(SLOT_X, ob, 0), opnd(SLOT_OP, *e, ehr0), 1.0, P.flops);
+            for (int j = 0; j < d; ++j) {
+                // HL[i][j]*in[j]: block of HL with Qin = qin, matched with in[j] at its Qout
+                if (const Blk* h = HL.l[i * d + j].find(ob.qin))
+                    if (const Blk* x = AC.l[j].find(h->qout))
+                        if (x->qout == ob.qout) add_seg(o, opnd(SLOT_OP, *h, hl0), opnd(SLOT_X, *x, 0), 1.0, P.flops);
+                // in[j]*HR[i][j]: block of in[j] with Qin = qin, matched with HR at its Qout
+                if (const Blk* x = AC.l[j].find(ob.qin))
+                    if (const Blk* h = HR.l[i * d + j].find(x->qout))
+                        if (h->qout == ob.qout) add_seg(o, opnd(SLOT_X, *x, 0), opnd(SLOT_OP, *h, hr0), 1.0, P.flops);
+            }
+            obs.push_back(std::move(o));
+        }
+    emit_stage(P, obs, SLOT_OUT);
+    P.operand_elems_read = P.op_elems + AC.nelem;
+    return P;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Mixed transfer matrix (TMBlockFunctions.hpp:352-390 left, :494-533 right), two stages:
+//   left : W = x(qi->qo) * A_s(qo) ;  y[Qout B_s(qi)] += B_s(qi)^T * W
+//   right: W = x(qi->qo) * B_s(qo-dB)^T ;  y[Qin A_s(qi-dA)] += A_s(qi-dA) * W
+// Operator store: [A | B] (B omitted when B is A).
+inline Program build_tm(int dir, const Layout& A, const Layout* Bp, const Layout& x, const Layout& y) {
+    const Layout& B = Bp ? *Bp : A;
+    require_same_nsym(A, B, "tm"); require_same_nsym(A, x, "tm"); require_same_nsym(A, y, "tm");
+    if (A.l.size() != B.l.size()) throw InvalidArg("tm: A and B need the same physical dimension");
+    if (x.l.size() != 1 || y.l.size() != 1) throw InvalidArg("tm: x and y must have nlist = 1");
+    Program P;
+    const long a0 = 0, b0 = Bp ? A.nelem : 0;
+    P.op_elems = A.nelem + (Bp ? B.nelem : 0);
+    P.x_elems = x.nelem;
+    P.y_elems = y.nelem;
+    std::vector<OutBlock> st1;
+    std::map<long, OutBlock> st2;  // keyed by output offset, filled in (s, x-block) order
+    for (const Blk& yb : y.l[0].b) st2.emplace(yb.off, OutBlock{yb.off, yb.rows, yb.cols, {}});
+    long w = 0;
+    const int d = (int)A.l.size();
+    for (int s = 0; s < d; ++s) {
+        if (A.l[s].b.empty() || B.l[s].b.empty()) continue;
+        Key dKA, dKB;
+        A.l[s].dK(dKA);
+        B.l[s].dK(dKB);
+        for (const Blk& xb : x.l[0].b) {
+            const Blk *lit, *rit, *yb;
+            if (dir == IMPS_LEFT) {
+                lit = B.l[s].find(xb.qin);
+                rit = A.l[s].find(xb.qout);
+                if (!lit || !rit) continue;
+                yb = y.l[0].find(lit->qout);
+                if (!yb) continue;
+                if (yb->qout != rit->qout) throw InvalidArg("tm: output block has a different outgoing charge");
+                OutBlock wb{w, xb.rows, rit->cols, {}};
+                add_seg(wb, opnd(SLOT_X, xb, 0), opnd(SLOT_OP, *rit, a0), 1.0, P.flops);
+                Operand W{SLOT_W, w, xb.rows, rit->cols, false};
+                add_seg(st2.at(yb->off), opnd(SLOT_OP, *lit, b0, true), W, 1.0, P.flops);
+                w += (long)xb.rows * rit->cols;
+                st1.push_back(std::move(wb));
+            } else {
+                lit = A.l[s].find(xb.qin - dKA);
+                rit = B.l[s].find(xb.qout - dKB);
+                if (!lit || !rit) continue;
+                yb = y.l[0].find(lit->qin);
+                if (!yb) continue;
+                if (yb->qout != rit->qin) throw InvalidArg("tm: output block has a different outgoing charge");
+                OutBlock wb{w, xb.rows, rit->rows, {}};
+                add_seg(wb, opnd(SLOT_X, xb, 0), opnd(SLOT_OP, *rit, b0, true), 1.0, P.flops);
+                Operand W{SLOT_W, w, xb.rows, rit->rows, false};
+                add_seg(st2.at(yb->off), opnd(SLOT_OP, *lit, a0), W, 1.0, P.flops);
+                w += (long)xb.rows * rit->rows;
+                st1.push_back(std::move(wb));
+            }
+        }
+    }
+    P.w_elems = w;
+    emit_stage(P, st1, SLOT_W);
+    std::vector<OutBlock> v2;
+    for (auto& kv : st2) v2.push_back(std::move(kv.second));
+    emit_stage(P, v2, SLOT_OUT);
+    P.operand_elems_read = P.op_elems + x.nelem;
+    return P;
+}
+
+// ---------------------------------------------------------------------------------------------
+// H_C in reduced form (VUMPS_helpers.hpp:91-111 restated with HL = ReducedOp(H,AL,l)):
+//   stage 1: W1_j = C * AR[j]                         (per AR[j] block p->p', C block p)
+//   stage 2: W2_i = sum_j HL[i][j] * W1_j             (blocks q -> q + dI2K[i])
+//   stage 3: out_q = EHL_q C_q + C_q EHR_q + sum_i W2_i(q -> q') * AR[i](q -> q')^T
+// Operator store: [EHL | EHR | HL | AR].
+inline Program build_hc(const Layout& C, const Layout& EHL, const Layout& EHR, const Layout& HL, const Layout& AR) {
+    const int d = (int)AR.l.size();
+    require_same_nsym(C, EHL, "hc"); require_same_nsym(C, EHR, "hc"); require_same_nsym(C, HL, "hc"); require_same_nsym(C, AR, "hc");
+    if (C.l.size() != 1 || EHL.l.size() != 1 || EHR.l.size() != 1) throw InvalidArg("hc: C/EHL/EHR must have nlist = 1");
+    if ((int)HL.l.size() != d * d) throw InvalidArg("hc: HL must have nlist = d*d");
+    Program P;
+    const long ehl0 = 0, ehr0 = EHL.nelem, hl0 = ehr0 + EHR.nelem, ar0 = hl0 + HL.nelem;
+    P.op_elems = ar0 + AR.nelem;
+    P.x_elems = P.y_elems = C.nelem;
+    long w = 0;
+    // stage 1
+    std::vector<OutBlock> s1;
+    std::vector<std::map<Key, Blk>> W1(d);  // per j: qin -> workspace block
+    for (int j = 0; j < d; ++j)
+        for (const Blk& a : AR.l[j].b) {
+            const Blk* c = C.l[0].find(a.qin);
+            if (!c) continue;
+            Blk wb{a.qin, a.qout, c->rows, a.cols, w};
+            OutBlock o{w, wb.rows, wb.cols, {}};
+            add_seg(o, opnd(SLOT_X, *c, 0), opnd(SLOT_OP, a, ar0), 1.0, P.flops);
+            s1.push_back(std::move(o));
+            W1[j].emplace(a.qin, wb);
+            w += wb.size();
+        }
+    emit_stage(P, s1, SLOT_W);
+    // stage 2: W2_i[q] = sum_j HL[i][j](q -> p) * W1_j(p -> p')
+    std::vector<OutBlock> s2;
+    std::vector<std::map<Key, Blk>> W2(d);
+    for (int i = 0; i < d; ++i) {
+        std::map<Key, OutBlock> acc;
+        std::map<Key, Blk> shape;
+        for (int j = 0; j < d; ++j)
+            for (const Blk& h : HL.l[i * d + j].b) {
+                auto it = W1[j].find(h.qout);
+                if (it == W1[j].end()) continue;
+                const Blk& y1 = it->second;
+                auto sh = shape.find(h.qin);
+                if (sh == shape.end()) {
+                    Blk wb{h.qin, y1.qout, h.rows, y1.cols, 0};
+                    shape.emplace(h.qin, wb);
+                    acc.emplace(h.qin, OutBlock{0, h.rows, y1.cols, {}});
+                } else if (sh->second.qout != y1.qout) {
+                    throw InvalidArg("hc: inconsistent charge flow in HL");
+                }
+                Operand Y1{SLOT_W, y1.off, y1.rows, y1.cols, false};
+                add_seg(acc.at(h.qin), opnd(SLOT_OP, h, hl0), Y1, 1.0, P.flops);
+            }
+        for (auto& kv : shape) {
+            kv.second.off = w;
+            OutBlock ob = std::move(acc.at(kv.first));
+            ob.c_off = w;
+            w += kv.second.size();
+            s2.push_back(std::move(ob));
+        }
+        W2[i] = std::move(shape);
+    }
+    emit_stage(P, s2, SLOT_W);
+    // stage 3
+    std::vector<OutBlock> s3;
+    for (const Blk& cb : C.l[0].b) {
+        OutBlock o{cb.off, cb.rows, cb.cols, {}};
+        if (const Blk* e = EHL.l[0].find(cb.qin)) add_seg(o, opnd(SLOT_OP, *e, ehl0), opnd(SLOT_X, cb, 0), 1.0, P.flops);
+        if (const Blk* e = EHR.l[0].find(cb.qout)) add_seg(o, opnd(SLOT_X, cb, 0), opnd(SLOT_OP, *e, ehr0), 1.0, P.flops);
+        for (int i = 0; i < d; ++i) {
+            auto it = W2[i].find(cb.qin);
+            if (it == W2[i].end()) continue;
+            const Blk& z = it->second;
+            const Blk* a = AR.l[i].find(cb.qin);  // AR[i] block q -> q'; its transpose maps q' -> q
+            if (!a || a->qout != z.qout) continue;
+            Operand Z{SLOT_W, z.off, z.rows, z.cols, false};
+            add_seg(o, Z, opnd(SLOT_OP, *a, ar0, true), 1.0, P.flops);
+        }
+        s3.push_back(std::move(o));
+    }
+    emit_stage(P, s3, SLOT_OUT);
+    P.w_elems = w;
+    P.operand_elems_read = P.op_elems + C.nelem;
+    return P;
+}
+
+// ---------------------------------------------------------------------------------------------
+// ReducedOp (MPSBlockUtilities.hpp:39-74).  Single stage; the A tensor is the "x" operand.
+// l: R[i1][j1] = sum h * A[i0]^T A[j0] ; r: R[i0][j0] = sum h * A[j1] A[i1]^T.
+// The nonzeros of H are visited column by column (the order of an Armadillo SpMat iterator).
+inline Program build_redop(const double* H, int d, const Layout& A, int dir, Layout& out) {
+    if ((int)A.l.size() != d) throw InvalidArg("redop: A must have nlist = d");
+    Program P;
+    P.x_elems = A.nelem;
+    const int D = d * d;
+    struct Acc { Key qout; int rows, cols; OutBlock ob; };
+    std::vector<std::map<Key, Acc>> R((size_t)D);
+    for (int c = 0; c < D; ++c)
+        for (int r = 0; r < D; ++r) {
+            const double h = H[r + (size_t)c * D];
+            if (h == 0.0) continue;
+            const int i0 = r / d, i1 = r % d, j0 = c / d, j1 = c % d;
+            if (dir == IMPS_LEFT) {
+                auto& Rm = R[(size_t)i1 * d + j1];
+                for (const Blk& a : A.l[i0].b) {          // A[i0]^T block: qout(a) -> qin(a)
+                    const Blk* b = A.l[j0].find(a.qin);    // A[j0] block qin(a) -> ...
+                    if (!b) continue;
+                    auto it = Rm.find(a.qout);
+                    if (it == Rm.end()) it = Rm.emplace(a.qout, Acc{b->qout, a.cols, b->cols, OutBlock{0, a.cols, b->cols, {}}}).first;
+                    else if (it->second.qout != b->qout) throw InvalidArg("redop: inconsistent charge flow");
+                    add_seg(it->second.ob, opnd(SLOT_X, a, 0, true), opnd(SLOT_X, *b, 0), h, P.flops);
+                }
+            } else {
+                auto& Rm = R[(size_t)i0 * d + j0];
+                Key dK1;
+                if (!A.l[i1].dK(dK1)) continue;
+                for (const Blk& a : A.l[j1].b) {          // A[j1] block p -> p'
+                    const Blk* b = A.l[i1].find(a.qout - dK1);  // A[i1] block with qout = p'
+                    if (!b) continue;
+                    auto it = Rm.find(a.qin);
+                    if (it == Rm.end()) it = Rm.emplace(a.qin, Acc{b->qin, a.rows, b->rows, OutBlock{0, a.rows, b->rows, {}}}).first;
+                    else if (it->second.qout != b->qin) throw InvalidArg("redop: inconsistent charge flow");
+                    add_seg(it->second.ob, opnd(SLOT_X, a, 0), opnd(SLOT_X, *b, 0, true), h, P.flops);
+                }
+            }
+        }
+    out = Layout();
+    out.nsym = A.nsym;
+    out.l.resize(D);
+    long pos = 0;
+    std::vector<OutBlock> obs;
+    for (int e = 0; e < D; ++e)
+        for (auto& kv : R[e]) {
+            Blk b{kv.first, kv.second.qout, kv.second.rows, kv.second.cols, pos};
+            out.l[e].b.push_back(b);
+            OutBlock ob = std::move(kv.second.ob);
+            ob.c_off = pos;
+            pos += b.size();
+            obs.push_back(std::move(ob));
+        }
+    out.nelem = pos;
+    P.y_elems = pos;
+    emit_stage(P, obs, SLOT_OUT);
+    P.operand_elems_read = A.nelem;
+    return P;
+}
+
+}  // namespace imps

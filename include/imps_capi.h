@@ -1,0 +1,158 @@
+/* imps_capi.h -- C ABI of the MI355X block-sparse iMPS operator library (libimps_hip.so).
+ *
+ * The library replaces the operator callbacks that MPShelpers hands to its Krylov drivers
+ * (ARPACK eigs_rs/eigs_n, MPShelpers/include/eigs.h:43-55, and gmres,
+ * MPShelpers/include/IterativeSolvers.hpp:21) with device-resident, grouped HIP kernels for gfx950.
+ *
+ * Conventions shared by every entry point
+ *  - Plain C types only: ints, doubles, raw pointers.  Vectors and operator data are FP64 device
+ *    pointers (hipMalloc'ed by the caller), laid out exactly as the reference's Vectorize():
+ *    blocks concatenated in ascending ingoing-charge order, each block column-major
+ *    (BlockObj.hpp:764-780, :1709-1726; MPSBlockMat.hpp:446-465).  An MPS tensor is s-major.
+ *  - Block structures are described by imps_layout (below), the counterpart of the dims vectors
+ *    the reference passes to its callbacks (dimkeypair_vec / dimpair_vec, DimMaps.hpp:31-201).
+ *  - `nvec` applies the operator to nvec vectors stored back to back (x[v*len_x + i]); each
+ *    output vector is fully overwritten, as the reference callbacks memset `out` first
+ *    (VUMPS_helpers.hpp:69,118; EigsBlockFunctions.hpp:55,73).
+ *  - Every function returns an imps_status; no C++ exception crosses this boundary.  On error,
+ *    imps_last_error() returns a thread-local message.
+ *  - A plan is not thread-safe.  Work is ordered on the context's HIP stream.  The library never
+ *    retains caller pointers beyond a call, except the operator data copied by *_set_* calls.
+ *  - Precision (imps_prec) selects the arithmetic of the contractions.  Inputs and outputs stay FP64.
+ */
+#ifndef IMPS_CAPI_H
+#define IMPS_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    IMPS_OK = 0,
+    IMPS_INVALID_ARG = 1,   /* malformed layout, mismatched sizes, unknown enum value */
+    IMPS_NOT_CONVERGED = 2, /* iterative solver hit maxit before tol */
+    IMPS_NUMERICAL = 3,     /* breakdown / non-finite values */
+    IMPS_HIP_ERROR = 4,     /* a HIP runtime call failed */
+    IMPS_NO_DEVICE = 5      /* no usable gfx950 device */
+} imps_status;
+
+typedef enum {
+    IMPS_FP64 = 0,   /* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 accumulation */
+    IMPS_FP32 = 1,   /* FP32 MFMA (v_mfma_f32_16x16x4f32), FP32 accumulation */
+    IMPS_BF16X3 = 2  /* split-BF16: a = a_hi + a_lo, a*b ~ hi*hi + hi*lo + lo*hi on bf16 MFMA, FP32 accumulation */
+} imps_prec;
+
+typedef enum { IMPS_LEFT = 0, IMPS_RIGHT = 1 } imps_dir;  /* dirtype l / r (arma_typedefs.h) */
+
+/* Block structure of a BlockMat (nlist = 1), BlockDiagMat (nlist = 1, qin == qout, square blocks),
+ * MPSBlockMat (nlist = d) or RedOp (nlist = d*d, row-major grid, OperatorTypes.hpp:160-187).
+ * Within each list entry blocks are in strictly ascending qin order (lexicographic over the nsym
+ * charges), which is the std::map order of the reference. */
+typedef struct {
+    int nsym;          /* number of U(1) charges per sector key, 1..4 */
+    int nlist;         /* list entries */
+    const int* nblk;   /* [nlist] blocks per entry */
+    const int* qin;    /* [sum(nblk) * nsym] ingoing charges */
+    const int* qout;   /* [sum(nblk) * nsym] outgoing charges */
+    const int* rows;   /* [sum(nblk)] */
+    const int* cols;   /* [sum(nblk)] */
+} imps_layout;
+
+typedef struct imps_context_s* imps_context;
+typedef struct imps_plan_s* imps_plan;
+
+/* ---- runtime ---------------------------------------------------------------------------- */
+const char* imps_last_error(void);
+const char* imps_version(void);
+/* Binds `device`, creates a default-priority stream.  Fails with IMPS_NO_DEVICE if absent. */
+imps_status imps_create(imps_context* ctx, int device);
+imps_status imps_destroy(imps_context ctx);
+/* Order all later work of this context on a caller stream (hipStream_t passed as void*); NULL
+ * selects the legacy default stream.  Until called, the context uses a stream of its own. */
+imps_status imps_set_stream(imps_context ctx, void* stream);
+imps_status imps_synchronize(imps_context ctx);
+imps_status imps_plan_destroy(imps_plan plan);
+/* Algorithmic cost of one single-vector apply: flops (sum of 2*M*N*K over executed block GEMMs)
+ * and bytes (each operand block read once and each output block written once at the working
+ * precision).  Either pointer may be NULL. */
+imps_status imps_plan_cost(imps_plan plan, double* flops, double* bytes);
+/* Number of grouped kernel launches per apply and total work tiles per vector. */
+imps_status imps_plan_shape(imps_plan plan, int* launches, int* tiles);
+/* Output structure of a plan whose output layout the library derives (imps_redop_plan):
+ * first call with qin == NULL to get nlist, total blocks and total elements. */
+imps_status imps_plan_out_layout(imps_plan plan, int* nlist, int* nblk_total, long* nelem,
+                                 int* nblk, int* qin, int* qout, int* rows, int* cols);
+
+/* ---- transfer matrices (A7/A8) ------------------------------------------------------------
+ * Replaces the preallocated ApplyTMmixedLeft / ApplyTMmixedRight (TMBlockFunctions.hpp:352-390,
+ * :494-533) and, with B == NULL, the regular ApplyTMLeft / ApplyTMRight on BlockDiagMat and
+ * BlockMat inputs (:60-84, :97-125, :184-205, :220-250) as called by TMMultDiag / TMMult
+ * (EigsBlockFunctions.hpp:50-85):
+ *   left : y[Qout B_s(qi)] += B_s(qi)^T * x(qi->qo) * A_s(qo)
+ *   right: y[Qin A_s(qi-dA_s)] += A_s(qi-dA_s) * x(qi->qo) * B_s(qo-dB_s)^T
+ * x and y are BlockMat/BlockDiagMat layouts; y must hold every sector that receives a
+ * contribution it should keep (others are dropped, as in the reference). */
+imps_status imps_tm_plan(imps_context ctx, imps_plan* plan, int dir, const imps_layout* A,
+                         const imps_layout* B, const imps_layout* x, const imps_layout* y, int prec);
+/* Copies (and converts to the plan precision) the MPS data; B may be NULL for the regular TM. */
+imps_status imps_tm_set_mps(imps_plan plan, const double* A, const double* B);
+imps_status imps_tm_apply(imps_plan plan, const double* x, double* y, int nvec);
+
+/* ---- two-site effective Hamiltonians (A12/A13) -------------------------------------------
+ * H_AC: ApplyHAvec (VUMPS_helpers.hpp:55-88) / ApplyHA (:38-53):
+ *   out[i] = EHL*in[i] + in[i]*EHR + sum_j (HL[i][j]*in[j] + in[j]*HR[i][j])
+ * AC: MPS layout (d lists); EHL, EHR: BlockDiagMat; HL, HR: RedOp layouts (d*d lists). */
+imps_status imps_hac_plan(imps_context ctx, imps_plan* plan, const imps_layout* AC,
+                          const imps_layout* EHL, const imps_layout* EHR,
+                          const imps_layout* HL, const imps_layout* HR, int prec);
+imps_status imps_hac_set_operator(imps_plan plan, const double* EHL, const double* EHR,
+                                  const double* HL, const double* HR);
+imps_status imps_hac_apply(imps_plan plan, const double* x, double* y, int nvec);
+
+/* H_C: ApplyHCvec (VUMPS_helpers.hpp:113-133) / ApplyHC (:91-111).  The reference evaluates
+ * sum_{nnz H} h*AL[i0]^T*AL[j0]*C*AR[j1]*AR[i1]^T; this plan executes the algebraically identical
+ * reduced form  EHL*C + C*EHR + sum_{i,j} HL[i][j]*(C*AR[j])*AR[i]^T  with HL = ReducedOp(H,AL,l)
+ * (MPSBlockUtilities.hpp:39-74), keeping diagonal blocks only (BlockObj.hpp:1527-1536). */
+imps_status imps_hc_plan(imps_context ctx, imps_plan* plan, const imps_layout* C,
+                         const imps_layout* EHL, const imps_layout* EHR,
+                         const imps_layout* HL, const imps_layout* AR, int prec);
+imps_status imps_hc_set_operator(imps_plan plan, const double* EHL, const double* EHR,
+                                 const double* HL, const double* AR);
+imps_status imps_hc_apply(imps_plan plan, const double* x, double* y, int nvec);
+
+/* ---- reduced operators (A11) ---------------------------------------------------------------
+ * ReducedOp(H, A, dir) (MPSBlockUtilities.hpp:39-74); H is a dense d^2 x d^2 column-major host
+ * array whose nonzeros define the sparse two-site operator.  The output layout (d*d lists) is
+ * derived from A's structure; query it with imps_plan_out_layout.  apply: y = ReducedOp. */
+imps_status imps_redop_plan(imps_context ctx, imps_plan* plan, const double* H, int d,
+                            const imps_layout* A, int dir, int prec);
+imps_status imps_redop_apply(imps_plan plan, const double* A, double* y);
+
+/* ---- bookkeeping (A1/A2) -------------------------------------------------------------------
+ * model: 0 = XXZ spin-1/2 (one U(1)), 1 = Fermi-Hubbard in (N,M) representation (U(1)xU(1)).
+ * imps_model_i2k: GetI2K (Models.cpp:262-295, :699-733); writes d*nsym charges, returns d or -1.
+ * imps_model_validpath: validpath (ItoKey.hpp:126-165); writes N indices.
+ * imps_model_make_dims: MakeDims (Models.cpp:13-32) from an explicit bond-0 dim map
+ * (keys0 ascending); writes N bonds of nsec (keys, sizes). Returns nsec or -1.
+ * imps_model_dimmap: GetDimMap (Models.cpp:248-260, :680-697) from the per-sector dims list.
+ * imps_model_ham: local two-site Hamiltonian (Models.cpp:57-93, :543-572), dense d^2 x d^2
+ * column-major; params XXZ {J, Delta, hz}, FHUB {t, U, V, mu, hz}. */
+int imps_model_nsym(int model);
+int imps_model_d(int model);
+int imps_model_i2k(int model, int N, const int* QN, int* i2k);
+int imps_model_validpath(int model, int N, const int* QN, int* path);
+int imps_model_dimmap(int model, int N, const int* QN, const int* dims, int ndims, int* keys, int* sizes);
+int imps_model_make_dims(int model, int N, const int* QN, int nsec, const int* keys0, const int* sizes0,
+                         int* keys, int* sizes);
+int imps_model_ham(int model, const double* params, double* H);
+/* MPSBlockMat(I2K, ml, mr) structure (MPSBlockMat.hpp:221-236): blocks q -> q + I2K[s] for every
+ * left sector q whose shifted charge exists on the right.  Writes nblk[d] and the block arrays
+ * (capacity d*nl blocks); returns the total block count or -1. */
+int imps_mps_structure(int model, int N, const int* QN, int nl, const int* kl, const int* sl,
+                       int nr, const int* kr, const int* sr, int* nblk, int* qin, int* qout,
+                       int* rows, int* cols);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMPS_CAPI_H */

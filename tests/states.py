@@ -1,0 +1,116 @@
+"""Synthetic, seeded block-sparse states for the tests, built with the oracle (the checker).
+
+Sector profiles follow the measurement configurations: XXZ U(1) with charges spaced by 2 and
+Gaussian-in-charge sizes; Fermi-Hubbard U(1)xU(1) (N, M) sectors on a checkerboard.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from oracle import pyoracle as orc  # noqa: E402
+from imps_cpp_tools_amd.workloads import fhub_profile, xxz_profile  # noqa: E402
+from oracle.pyoracle import OB  # noqa: E402
+
+XXZ, FHUB = 0, 1
+
+
+def xxz_bond0(nsec: int, peak: int, lo: int = 1, total: int = None):
+    return xxz_profile(nsec, peak, lo, total)
+
+
+def fhub_bond0(nside: int, peak: int, lo: int = 1, total: int = None):
+    return fhub_profile(nside, peak, lo, total)
+
+
+def diag_layout(nsym, keys, sizes, data=None):
+    keys = np.asarray(keys, np.int32).reshape(-1, nsym)
+    sizes = np.asarray(sizes, np.int32)
+    lay = OB(nsym, np.array([len(sizes)], np.int32), keys, keys, sizes, sizes, np.zeros(int((sizes.astype(np.int64) ** 2).sum())))
+    if data is not None:
+        lay.data = np.ascontiguousarray(data, np.float64)
+    return lay
+
+
+def random_like(ob: OB, rng, sym=False, shift=0.0) -> OB:
+    out = OB(ob.nsym, ob.nblk, ob.qin, ob.qout, ob.rows, ob.cols, rng.standard_normal(ob.nelem))
+    if sym:
+        for _, qi, qo, m in out.blocks():
+            m[...] = 0.5 * (m + m.T) + shift * np.eye(m.shape[0])
+    return out
+
+
+class Cell:
+    """A unit cell of N sites: bonds, left/right isometries AL, AR (random, QR-orthonormalised)."""
+
+    def __init__(self, model, N, QN, keys0, sizes0, seed):
+        self.model, self.N, self.QN = model, N, list(QN)
+        self.nsym = 1 if model == XXZ else 2
+        self.d = 2 if model == XXZ else 4
+        self.bonds = orc.make_dims(model, N, QN, keys0, sizes0)
+        rng = np.random.default_rng(seed)
+        self.rng = rng
+        self.struct = [orc.mps_structure(model, N, QN, self.bonds[n], self.bonds[(n + 1) % N]) for n in range(N)]
+        self.AL, self.AR = [], []
+        for n in range(N):
+            A = random_like(self.struct[n], rng)
+            Q, _ = orc.mps_qr(A, 0)
+            assert _same_structure(Q, self.struct[n]), "QR changed the sector structure (sizes too uneven)"
+            self.AL.append(Q)
+            A = random_like(self.struct[n], rng)
+            Q, _ = orc.mps_qr(A, 1)
+            assert _same_structure(Q, self.struct[n]), "QR changed the sector structure (sizes too uneven)"
+            self.AR.append(Q)
+
+    def diag(self, bond, data=None):
+        k, s = self.bonds[bond % self.N]
+        return diag_layout(self.nsym, k, s, data)
+
+    def bond_offsets(self, bond):
+        k, s = self.bonds[bond % self.N]
+        off = np.concatenate([[0], np.cumsum(s)[:-1]])
+        return {tuple(int(v) for v in kk): (int(o), int(ss)) for kk, o, ss in zip(k, off, s)}, int(s.sum())
+
+
+def _same_structure(a: OB, b: OB) -> bool:
+    return (np.array_equal(a.nblk, b.nblk) and np.array_equal(a.qin, b.qin) and np.array_equal(a.qout, b.qout)
+            and np.array_equal(a.rows, b.rows) and np.array_equal(a.cols, b.cols))
+
+
+def dense_mps(ob: OB, loff, Dl, roff, Dr):
+    """Embed an MPS tensor (nlist entries) into a dense (d, Dl, Dr) array."""
+    d = len(ob.nblk)
+    out = np.zeros((d, Dl, Dr))
+    for e, qi, qo, m in ob.blocks():
+        a, _ = loff[tuple(int(v) for v in qi)]
+        b, _ = roff[tuple(int(v) for v in qo)]
+        out[e, a:a + m.shape[0], b:b + m.shape[1]] = m
+    return out
+
+
+def dense_mat(ob: OB, loff, Dl, roff, Dr, entry=0):
+    out = np.zeros((Dl, Dr))
+    for e, qi, qo, m in ob.blocks():
+        if e != entry:
+            continue
+        a, _ = loff[tuple(int(v) for v in qi)]
+        b, _ = roff[tuple(int(v) for v in qo)]
+        out[a:a + m.shape[0], b:b + m.shape[1]] = m
+    return out
+
+
+def extract(ob_layout: OB, dense, loff, roff, per_entry=False):
+    """Read the blocks of `ob_layout` out of a dense (D, D) or (d, D, D) array, flat in layout order."""
+    parts = []
+    for e, qi, qo, m in ob_layout.blocks():
+        a, _ = loff[tuple(int(v) for v in qi)]
+        b, _ = roff[tuple(int(v) for v in qo)]
+        src = dense[e] if per_entry else dense
+        parts.append(src[a:a + m.shape[0], b:b + m.shape[1]].reshape(-1, order="F"))
+    return np.concatenate(parts) if parts else np.zeros(0)

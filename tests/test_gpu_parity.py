@@ -1,0 +1,156 @@
+"""GPU parity of the grouped HIP kernels against the FP64 oracle, through the C ABI.
+
+Tolerances (norm-wise relative error ||y - y_oracle|| / ||y_oracle||):
+  FP64   : 1e-12  (the target is 1e-10 relative)
+  FP32   : 1e-5   (FP32 MFMA, FP32 accumulation over K <= ~500)
+  BF16X3 : 2e-4   (split-BF16 hi*hi + hi*lo + lo*hi, each product accurate to ~2^-16 relative)
+Block structures are exact by construction: every output block of the layout is compared.
+"""
+import numpy as np
+import pytest
+
+import imps_cpp_tools_amd as im
+from states import FHUB, XXZ, Cell, fhub_bond0, random_like, xxz_bond0
+from oracle import pyoracle as orc
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+TOL = {im.FP64: 1e-12, im.FP32: 1e-5, im.BF16X3: 2e-4}
+PRECS = [im.FP64, im.FP32, im.BF16X3]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = im.Context(0)
+    yield c
+    c.close()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+
+def lay(ob):
+    return im.BlockLayout(ob.nsym, ob.nblk, ob.qin, ob.qout, ob.rows, ob.cols)
+
+
+def relerr(got, ref):
+    n = np.linalg.norm(ref)
+    return np.linalg.norm(got - ref) / (n if n > 0 else 1.0)
+
+
+CELLS = {}
+
+
+def cell(name):
+    if name not in CELLS:
+        if name == "xxz":
+            k, s = xxz_bond0(9, 12)
+            CELLS[name] = Cell(XXZ, 2, [0], k, s, seed=101)
+        elif name == "fhub":
+            k, s = fhub_bond0(5, 10)
+            CELLS[name] = Cell(FHUB, 2, [2, 0], k, s, seed=102)
+        elif name == "xxz_big":  # TM microbenchmark shapes: 17 sectors, sizes up to 64
+            k, s = xxz_bond0(17, 64, 2)
+            CELLS[name] = Cell(XXZ, 2, [0], k, s, seed=1002)
+    return CELLS[name]
+
+
+def ham(c):
+    return orc.model_ham(c.model, [1.0, -1.0, 0.1] if c.model == XXZ else [1.0, 4.0, 0.0, 0.0, 0.0])
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("name", ["xxz", "fhub", "xxz_big"])
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_tm_parity(ctx, prec, name, direction, mixed):
+    c = cell(name)
+    rng = np.random.default_rng(7)
+    A = random_like(c.struct[0], rng)
+    B = random_like(c.struct[0], rng) if mixed else None
+    xo = random_like(c.diag(0 if direction == 0 else 1), rng)
+    yl = c.diag(1 if direction == 0 else 0)
+    ref = orc.tm_mixed(direction, A, B, xo, yl).data
+    p = im.TMPlan(ctx, direction, lay(A), lay(B) if mixed else None, lay(xo), lay(yl), prec)
+    p.set_mps(dev(A.data), dev(B.data) if mixed else None)
+    nvec = 3
+    X = np.stack([xo.data, 2 * xo.data, -xo.data])
+    y = torch.full((nvec * yl.nelem,), float("nan"), dtype=torch.float64, device="cuda")
+    p.apply(dev(X), y, nvec)
+    got = y.cpu().numpy().reshape(nvec, -1)
+    for v, f in enumerate([1.0, 2.0, -1.0]):
+        assert relerr(got[v], f * ref) < TOL[prec], (v, relerr(got[v], f * ref))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("name", ["xxz", "fhub"])
+def test_hac_parity(ctx, prec, name):
+    c = cell(name)
+    H = ham(c)
+    rng = np.random.default_rng(8)
+    HL = orc.reduced_op(H, c.d, c.AL[1], 0)
+    HR = orc.reduced_op(H, c.d, c.AR[1], 1)
+    EHL = random_like(c.diag(0), rng, sym=True)
+    EHR = random_like(c.diag(1), rng, sym=True)
+    xs = [random_like(c.struct[0], rng) for _ in range(4)]
+    refs = [orc.hac(EHL, EHR, HL, HR, x).data for x in xs]
+    p = im.HACPlan(ctx, lay(xs[0]), lay(EHL), lay(EHR), lay(HL), lay(HR), prec)
+    p.set_operator(dev(EHL.data), dev(EHR.data), dev(HL.data), dev(HR.data))
+    for nvec in (1, 4):
+        X = np.concatenate([x.data for x in xs[:nvec]])
+        y = torch.full((X.size,), float("nan"), dtype=torch.float64, device="cuda")
+        p.apply(dev(X), y, nvec)
+        got = y.cpu().numpy().reshape(nvec, -1)
+        for v in range(nvec):
+            assert relerr(got[v], refs[v]) < TOL[prec], (nvec, v, relerr(got[v], refs[v]))
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("name", ["xxz", "fhub"])
+def test_hc_parity(ctx, prec, name):
+    """H_C through the reduced form equals the reference's per-nonzero 5-factor chain."""
+    c = cell(name)
+    H = ham(c)
+    rng = np.random.default_rng(9)
+    AL, AR = c.AL[1], c.AR[0]
+    EHL = random_like(c.diag(0), rng, sym=True)
+    EHR = random_like(c.diag(0), rng, sym=True)
+    C = random_like(c.diag(0), rng)
+    ref = orc.hc(EHL, EHR, H, c.d, AL, AR, C).data
+    HL = orc.reduced_op(H, c.d, AL, 0)
+    p = im.HCPlan(ctx, lay(C), lay(EHL), lay(EHR), lay(HL), lay(AR), prec)
+    p.set_operator(dev(EHL.data), dev(EHR.data), dev(HL.data), dev(AR.data))
+    y = torch.full((C.nelem,), float("nan"), dtype=torch.float64, device="cuda")
+    p.apply(dev(C.data), y)
+    assert relerr(y.cpu().numpy(), ref) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", [im.FP64, im.FP32])
+@pytest.mark.parametrize("name", ["xxz", "fhub"])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_redop_parity(ctx, prec, name, direction):
+    c = cell(name)
+    H = ham(c)
+    A = c.AL[1] if direction == 0 else c.AR[1]
+    ref = orc.reduced_op(H, c.d, A, direction)
+    p = im.RedOpPlan(ctx, H, c.d, lay(A), direction, prec)
+    assert p.out_layout == lay(ref)
+    y = torch.full((ref.nelem,), float("nan"), dtype=torch.float64, device="cuda")
+    p.apply(dev(A.data), y)
+    assert relerr(y.cpu().numpy(), ref.data) < TOL[prec]
+
+
+def test_full_size_hubbard_hac(ctx):
+    """D = 1024 Hubbard U(1)xU(1), d = 4: every precision against the oracle at full size."""
+    import synth_full
+    st = synth_full.hubbard_1024(seed=1003)
+    ref = orc.hac(st["EHL"], st["EHR"], st["HL"], st["HR"], st["AC"]).data
+    for prec in PRECS:
+        p = im.HACPlan(ctx, lay(st["AC"]), lay(st["EHL"]), lay(st["EHR"]), lay(st["HL"]), lay(st["HR"]), prec)
+        p.set_operator(dev(st["EHL"].data), dev(st["EHR"].data), dev(st["HL"].data), dev(st["HR"].data))
+        y = torch.full((ref.size,), float("nan"), dtype=torch.float64, device="cuda")
+        p.apply(dev(st["AC"].data), y)
+        assert relerr(y.cpu().numpy(), ref) < TOL[prec], (prec, relerr(y.cpu().numpy(), ref))
