@@ -9,9 +9,11 @@
 //   FP32   : v_mfma_f32_16x16x4f32,  K-chunk 16
 //   BF16X3 : v_mfma_f32_16x16x32_bf16, K-chunk 32; each FP32 operand is split as hi = bf16(a),
 //            lo = bf16(a - hi) and the product is hi*hi + hi*lo + lo*hi (FP32 accumulation).
-// Fragment layouts (16x16 output tiles): A(i,k) from lane i + 16*k' ; B(k,j) from lane j + 16*k';
-// lane l holds D(4*(l/16) + r, l%16), r = 0..3.  For the bf16 form lane l holds 8 consecutive k
-// (k = 8*(l/16) + e).
+// Fragment layouts (16x16 output tiles, measured on gfx950 with one-hot operands):
+//   A(i,k) from lane i + 16*k, B(k,j) from lane j + 16*k (bf16 form: lane l holds the 8 consecutive
+//   k = 8*(l/16) + e of row/column l%16);
+//   result register r of lane l: FP32 and bf16 forms D(4*(l/16) + r, l%16),
+//                                FP64 form D(4*r + l/16, l%16).
 // Quadrants or 16x16 sub-tiles that lie wholly outside the block are skipped (wave-uniform).
 
 #include <hip/hip_runtime.h>
@@ -159,7 +161,8 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const Task* __re
             const int col = t.n0 + wc + b * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = t.m0 + wr + a * 16 + 4 * (lane >> 4) + r;
+                const int rr = (PREC == IMPS_FP64) ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+                const int row = t.m0 + wr + a * 16 + rr;
                 if (row < t.M && col < t.N) C[row + (long)col * t.c_ld] = static_cast<TOut>(acc[a][b][r]);
             }
         }
