@@ -21,7 +21,7 @@ struct DevStage {
     int out_slot = SLOT_OUT;
 };
 
-// One grouped launch: grid (ntasks, nvec).  Compute type is double for IMPS_FP64 and float
+// One grouped launch: grid (ntasks, nvec), four waves per workgroup.  Compute type is double for IMPS_FP64 and float
 // otherwise; the output is double when `out_double`, else the compute type.
 hipError_t launch_stage(int prec, const DevStage& st, const Slots& slots, void* out, long out_stride,
                         bool out_double, int nvec, hipStream_t stream);

@@ -26,7 +26,7 @@
 namespace imps {
 
 constexpr int kMaxSym = 4;
-constexpr int kTile = 64;  // output tile edge of the grouped GEMM kernel
+constexpr int kTile = 32;  // output tile edge of the grouped GEMM kernel (one workgroup per tile)
 
 struct InvalidArg : std::runtime_error { using std::runtime_error::runtime_error; };
 
@@ -118,7 +118,7 @@ struct Task {
 };
 struct Stage {
     int out_slot = SLOT_OUT;   // SLOT_OUT: final output; SLOT_W: intermediate workspace
-    std::vector<Task> tasks;
+    std::vector<Task> tasks;   // tiles of at most kTile x kTile
     std::vector<Seg> segs;
 };
 struct Program {
