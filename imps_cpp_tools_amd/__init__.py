@@ -76,6 +76,10 @@ def lib():
             "imps_hc_apply": (I, [P, V, V, I]),
             "imps_redop_plan": (I, [P, ctypes.POINTER(P), dp, I, LP, I, I]),
             "imps_redop_apply": (I, [P, V, V]),
+            "imps_env_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, I]),
+            "imps_env_set_mps": (I, [P, ctypes.POINTER(V)]),
+            "imps_env_solve": (I, [P, V, V, V, V, V, D, I, I, ip, dp]),
+            "imps_eigsh": (I, [P, P, V, D, I, I, dp, ip, dp]),
             "imps_model_nsym": (I, [I]),
             "imps_model_d": (I, [I]),
             "imps_model_i2k": (I, [I, I, ip, ip]),
@@ -421,3 +425,38 @@ class RedOpPlan(Plan):
 
     def apply(self, A, y):
         _check(lib().imps_redop_apply(self.h, _ptr(A), _ptr(y)))
+
+
+class EnvPlan(Plan):
+    """Environment (geometric-sum) solver over a unit cell (InvertE_proj, InvEBlockFunctions.hpp:205-266)."""
+    kind = "env"
+
+    def __init__(self, ctx: Context, direction: int, A: Sequence[BlockLayout], x: BlockLayout, prec: int = FP64):
+        h = ctypes.c_void_p()
+        arr = (_CLayout * len(A))(*[a.c() for a in A])
+        self._A = list(A)
+        cx = x.c()
+        _check(lib().imps_env_plan(_ctx_handle(ctx), ctypes.byref(h), direction, len(A), arr, ctypes.byref(cx), prec))
+        super().__init__(h)
+        self.n = x.nelem
+
+    def set_mps(self, tensors):
+        arr = (ctypes.c_void_p * len(tensors))(*[_ptr(t) for t in tensors])
+        _check(lib().imps_env_set_mps(self.h, arr))
+
+    def solve(self, x, L, R, y, y0=None, tol: float = 1e-14, maxit: int = 0, krylov: int = 40, check: bool = True):
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        st = lib().imps_env_solve(self.h, _ptr(x), _ptr(L), _ptr(R), _ptr(y0), _ptr(y), tol, maxit, krylov, ctypes.byref(it), ctypes.byref(rr))
+        if check:
+            _check(st)
+        return it.value, rr.value
+
+
+def eigsh(plan: Plan, x, tol: float = 1e-12, maxit: int = 0, krylov: int = 20, plan_fp64: Optional[Plan] = None, check: bool = True):
+    """Lowest eigenpair of an H_AC / H_C plan (eigs_rs "SA", nev = 1); x: start vector in, eigenvector out."""
+    ev, it, rr = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+    st = lib().imps_eigsh(plan.h, plan_fp64.h if plan_fp64 is not None else None, _ptr(x), tol, maxit, krylov,
+                          ctypes.byref(ev), ctypes.byref(it), ctypes.byref(rr))
+    if check:
+        _check(st)
+    return ev.value, it.value, rr.value

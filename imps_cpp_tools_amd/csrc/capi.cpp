@@ -8,26 +8,14 @@
 #include <vector>
 
 #include "../../include/imps_capi.h"
-#include "engine.hpp"
-#include "layout.hpp"
+#include "krylov.hpp"
+#include "plan.hpp"
 
 using namespace imps;
-
-struct imps_context_s {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-};
 
 namespace {
 
 thread_local std::string g_err;
-
-struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
-
-void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
 
 template <class F>
 imps_status guarded(F&& f) {
@@ -39,6 +27,12 @@ imps_status guarded(F&& f) {
     } catch (const HipError& e) {
         g_err = e.what();
         return IMPS_HIP_ERROR;
+    } catch (const NotConverged& e) {
+        g_err = e.what();
+        return IMPS_NOT_CONVERGED;
+    } catch (const Numerical& e) {
+        g_err = e.what();
+        return IMPS_NUMERICAL;
     } catch (const std::bad_alloc&) {
         g_err = "host allocation failed";
         return IMPS_INVALID_ARG;
@@ -46,118 +40,6 @@ imps_status guarded(F&& f) {
         g_err = e.what();
         return IMPS_INVALID_ARG;
     }
-}
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    void ensure(size_t n) {
-        if (n <= bytes) return;
-        if (p) { hip_check(hipFree(p), "hipFree"); p = nullptr; bytes = 0; }
-        if (n == 0) return;
-        hip_check(hipMalloc(&p, n), "hipMalloc");
-        bytes = n;
-    }
-};
-
-enum Kind { K_TM = 0, K_HAC = 1, K_HC = 2, K_REDOP = 3 };
-
-}  // namespace
-
-struct imps_plan_s {
-    imps_context ctx = nullptr;
-    int kind = 0;
-    int prec = IMPS_FP64;
-    Program prog;
-    Layout out_layout;           // derived output structure (redop)
-    std::vector<long> op_parts;  // element counts of the operator-store pieces, in set order
-    bool op_set = false;
-    // device state
-    DevBuf d_prog, d_op, d_w;
-    std::vector<DevStage> dstages;
-    size_t elem_bytes() const { return prec == IMPS_FP64 ? 8 : 4; }
-};
-
-namespace {
-
-void upload_program(imps_plan_s* P) {
-    if (!P->ctx) return;
-    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
-    size_t bytes = 0;
-    for (auto& st : P->prog.stages) bytes += st.tasks.size() * sizeof(Task) + st.segs.size() * sizeof(Seg) + 64;
-    P->d_prog.ensure(bytes + 64);
-    std::vector<char> host(bytes + 64, 0);
-    size_t pos = 0;
-    P->dstages.clear();
-    for (auto& st : P->prog.stages) {
-        DevStage ds;
-        ds.ntasks = (int)st.tasks.size();
-        ds.out_slot = st.out_slot;
-        pos = (pos + 15) & ~size_t(15);
-        std::memcpy(host.data() + pos, st.tasks.data(), st.tasks.size() * sizeof(Task));
-        ds.tasks = (const Task*)((char*)P->d_prog.p + pos);
-        pos += st.tasks.size() * sizeof(Task);
-        pos = (pos + 15) & ~size_t(15);
-        std::memcpy(host.data() + pos, st.segs.data(), st.segs.size() * sizeof(Seg));
-        ds.segs = (const Seg*)((char*)P->d_prog.p + pos);
-        pos += st.segs.size() * sizeof(Seg);
-        P->dstages.push_back(ds);
-    }
-    if (pos) hip_check(hipMemcpyAsync(P->d_prog.p, host.data(), pos, hipMemcpyHostToDevice, P->ctx->stream), "upload program");
-    hip_check(hipStreamSynchronize(P->ctx->stream), "upload program");
-    P->d_op.ensure(std::max<long>(P->prog.op_elems, 1) * P->elem_bytes());
-}
-
-// Copy FP64 operator pieces into the plan's operator store at its working precision.
-void set_operator(imps_plan_s* P, const std::vector<const double*>& parts) {
-    if (!P->ctx) throw InvalidArg("plan was created without a context");
-    if (parts.size() != P->op_parts.size()) throw InvalidArg("operator piece count mismatch");
-    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
-    long off = 0;
-    for (size_t i = 0; i < parts.size(); ++i) {
-        const long n = P->op_parts[i];
-        if (n > 0 && !parts[i]) throw InvalidArg("operator data pointer is NULL");
-        if (n > 0) {
-            if (P->prec == IMPS_FP64)
-                hip_check(hipMemcpyAsync((double*)P->d_op.p + off, parts[i], n * sizeof(double), hipMemcpyDeviceToDevice, P->ctx->stream),
-                          "copy operator");
-            else
-                hip_check(launch_d2f(parts[i], (float*)P->d_op.p + off, n, P->ctx->stream), "convert operator");
-        }
-        off += n;
-    }
-    P->op_set = true;
-}
-
-void run(imps_plan_s* P, const double* x, double* y, int nvec) {
-    if (!P->ctx) throw InvalidArg("plan was created without a context");
-    if (nvec < 1) throw InvalidArg("nvec must be >= 1");
-    if ((P->prog.x_elems > 0 && !x) || (P->prog.y_elems > 0 && !y)) throw InvalidArg("x or y is NULL");
-    if (!P->op_set && P->prog.op_elems > 0) throw InvalidArg("operator data not set");
-    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
-    hipStream_t s = P->ctx->stream;
-    const size_t eb = P->elem_bytes();
-    P->d_w.ensure((size_t)std::max<long>(P->prog.w_elems, 1) * nvec * eb);
-    Slots sl;
-    sl.base[SLOT_X] = x;           sl.stride[SLOT_X] = P->prog.x_elems;  // FP64, converted in-kernel
-    sl.base[SLOT_OP] = P->d_op.p;  sl.stride[SLOT_OP] = 0;
-    sl.base[SLOT_W] = P->d_w.p;    sl.stride[SLOT_W] = P->prog.w_elems;
-    for (auto& ds : P->dstages) {
-        if (ds.out_slot == SLOT_W)
-            hip_check(launch_stage(P->prec, ds, sl, P->d_w.p, P->prog.w_elems, P->prec == IMPS_FP64, nvec, s), "grouped gemm");
-        else
-            hip_check(launch_stage(P->prec, ds, sl, y, P->prog.y_elems, true, nvec, s), "grouped gemm");
-    }
-}
-
-imps_plan_s* new_plan(imps_context ctx, int kind, int prec) {
-    if (prec < IMPS_FP64 || prec > IMPS_BF16X3) throw InvalidArg("unknown precision");
-    auto* P = new imps_plan_s;
-    P->ctx = ctx;
-    P->kind = kind;
-    P->prec = prec;
-    return P;
 }
 
 imps_status finish_plan(imps_plan* out, std::unique_ptr<imps_plan_s>& P) {
@@ -372,6 +254,81 @@ imps_status imps_redop_plan(imps_context ctx, imps_plan* plan, const double* H, 
 }
 imps_status imps_redop_apply(imps_plan P, const double* A, double* y) {
     return guarded([&] { check_kind(P, K_REDOP); run(P, A, y, 1); return IMPS_OK; });
+}
+
+// ---- environments (A14/A15) ----
+imps_status imps_env_plan(imps_context ctx, imps_plan* plan, int dir, int ncell, const imps_layout* A, const imps_layout* x, int prec) {
+    return guarded([&] {
+        if (!plan || !A) throw InvalidArg("plan or A is NULL");
+        if (dir != IMPS_LEFT && dir != IMPS_RIGHT) throw InvalidArg("dir must be IMPS_LEFT or IMPS_RIGHT");
+        if (ncell < 1 || ncell > 64) throw InvalidArg("ncell out of range");
+        std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_ENV, prec));
+        std::vector<Layout> LA;
+        for (int k = 0; k < ncell; ++k) LA.push_back(layout_from_c(&A[k], "A"));
+        Layout lx = layout_from_c(x, "x");
+        P->env = std::make_shared<EnvSolver>();
+        P->env->build(ctx, dir, LA, lx, prec);
+        P->op_set = true;
+        *plan = P.release();
+        return IMPS_OK;
+    });
+}
+imps_status imps_env_set_mps(imps_plan P, const double* const* A) {
+    return guarded([&] {
+        check_kind(P, K_ENV);
+        if (!A) throw InvalidArg("A is NULL");
+        P->env->set_mps(std::vector<const double*>(A, A + P->env->ncell));
+        return IMPS_OK;
+    });
+}
+imps_status imps_env_solve(imps_plan P, const double* x, const double* L, const double* R, const double* y0, double* y, double tol,
+                           int maxit, int krylov, int* iters, double* relres) {
+    return guarded([&] {
+        check_kind(P, K_ENV);
+        if (!(tol > 0)) throw InvalidArg("tol must be > 0");
+        KrylovStats st = P->env->solve(x, L, R, y0, y, tol, maxit, krylov);
+        if (iters) *iters = st.iters;
+        if (relres) *relres = st.relres;
+        if (!st.converged) throw NotConverged("env solve: relative residual " + std::to_string(st.relres) + " after " +
+                                              std::to_string(st.iters) + " applications");
+        return IMPS_OK;
+    });
+}
+
+// ---- lowest eigenpair of H_AC / H_C (A17) ----
+imps_status imps_eigsh(imps_plan P, imps_plan P64, double* x, double tol, int maxit, int krylov, double* eval, int* iters, double* resid) {
+    return guarded([&] {
+        if (!P || !x) throw InvalidArg("plan or x is NULL");
+        if (P->kind != K_HAC && P->kind != K_HC) throw InvalidArg("eigsh needs an H_AC or H_C plan");
+        if (P64 && (P64->kind != P->kind || P64->prec != IMPS_FP64 || P64->prog.x_elems != P->prog.x_elems))
+            throw InvalidArg("refinement plan must be an FP64 plan of the same operator");
+        if (!P->ctx) throw InvalidArg("plan was created without a context");
+        if (!(tol > 0)) throw InvalidArg("tol must be > 0");
+        hipStream_t s = P->ctx->stream;
+        hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+        if (maxit <= 0) maxit = 10000;
+        if (krylov <= 0) krylov = 20;  // ARPACK NCV floor used by eigs_rs (eigs.cpp:400)
+        auto op_of = [](imps_plan q) {
+            LinOp op;
+            op.n = q->prog.x_elems;
+            op.apply = [q](const double* a, double* b) { run(q, a, b, 1); };
+            return op;
+        };
+        KrylovStats st;
+        if (P->prec == IMPS_FP64 || !P64) {
+            st = lanczos_lowest(s, op_of(P), x, tol, maxit, krylov);
+        } else {
+            const double tlo = std::max(tol, P->prec == IMPS_FP32 ? 1e-6 : 1e-4);
+            KrylovStats a = lanczos_lowest(s, op_of(P), x, tlo, maxit, krylov);
+            st = lanczos_lowest(s, op_of(P64), x, tol, std::max(1, maxit - a.iters), krylov);
+            st.iters += a.iters;
+        }
+        if (eval) *eval = st.value;
+        if (iters) *iters = st.iters;
+        if (resid) *resid = st.relres;
+        if (!st.converged) throw NotConverged("eigsh: relative residual " + std::to_string(st.relres));
+        return IMPS_OK;
+    });
 }
 
 }  // extern "C"

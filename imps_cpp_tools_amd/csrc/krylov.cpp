@@ -1,0 +1,414 @@
+// Device Krylov drivers (see krylov.hpp).
+#include "krylov.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+#include "blas.hpp"
+
+namespace imps {
+
+namespace {
+
+void d2h(hipStream_t s, double* h, const double* d, size_t n) {
+    hip_check(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    hip_check(hipStreamSynchronize(s), "sync");
+}
+// synchronous: the host source may be a short-lived temporary
+void h2d(hipStream_t s, double* d, const double* h, size_t n) {
+    hip_check(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+    hip_check(hipStreamSynchronize(s), "sync");
+}
+void d2d(hipStream_t s, double* d, const double* src, size_t n) {
+    if (n) hip_check(hipMemcpyAsync(d, src, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
+}
+
+// Scratch for one Krylov run: basis V (m+1 vectors), w, and reduction buffers.
+struct Work {
+    DevBuf V, w, dh, scratch;
+    long n = 0;
+    int m = 0;
+    void init(long n_, int m_) {
+        n = n_;
+        m = m_;
+        V.ensure(sizeof(double) * (size_t)n * (m + 1));
+        w.ensure(sizeof(double) * (size_t)std::max<long>(n, 1));
+        dh.ensure(sizeof(double) * (size_t)(2 * (m + 2)));
+        scratch.ensure(sizeof(double) * blas_scratch_doubles(m + 2));
+    }
+    double* Vi(int i) const { return V.d() + (size_t)i * n; }
+};
+
+double dev_norm(hipStream_t s, Work& W, const double* v) {
+    double h = 0;
+    hip_check(blas_multidot(v, 0, 1, v, W.n, W.dh.d(), W.scratch.d(), s), "norm");
+    d2h(s, &h, W.dh.d(), 1);
+    return std::sqrt(std::max(h, 0.0));
+}
+
+// w <- w - V[0..k) (V^T w), twice (CGS2); returns the k projection coefficients (summed)
+void cgs2(hipStream_t s, Work& W, int k, double* w, std::vector<double>& h) {
+    h.assign(k, 0.0);
+    std::vector<double> t(k);
+    for (int pass = 0; pass < 2; ++pass) {
+        hip_check(blas_multidot(W.V.d(), W.n, k, w, W.n, W.dh.d(), W.scratch.d(), s), "multidot");
+        hip_check(blas_gemv_sub(W.V.d(), W.n, k, W.dh.d(), w, W.n, s), "gemv");
+        d2h(s, t.data(), W.dh.d(), k);
+        for (int i = 0; i < k; ++i) h[i] += t[i];
+    }
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (column-major, FP64).
+void jacobi_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& Z) {
+    Z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Z[i + (size_t)i * n] = 1.0;
+    auto a = [&](int i, int j) -> double& { return A[i + (size_t)j * n]; };
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0, tot = 0;
+        for (int j = 0; j < n; ++j) for (int i = 0; i < n; ++i) { tot += a(i, j) * a(i, j); if (i != j) off += a(i, j) * a(i, j); }
+        if (off <= 1e-30 * tot || off == 0) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = a(p, q);
+                if (apq == 0) continue;
+                const double theta = (a(q, q) - a(p, p)) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::abs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < n; ++k) {
+                    const double akp = a(k, p), akq = a(k, q);
+                    a(k, p) = c * akp - sn * akq;
+                    a(k, q) = sn * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double apk = a(p, k), aqk = a(q, k);
+                    a(p, k) = c * apk - sn * aqk;
+                    a(q, k) = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double zkp = Z[k + (size_t)p * n], zkq = Z[k + (size_t)q * n];
+                    Z[k + (size_t)p * n] = c * zkp - sn * zkq;
+                    Z[k + (size_t)q * n] = sn * zkp + c * zkq;
+                }
+            }
+    }
+    w.resize(n);
+    for (int i = 0; i < n; ++i) w[i] = a(i, i);
+}
+
+}  // namespace
+
+KrylovStats gmres(hipStream_t s, const LinOp& A, const double* b, double* x, double tol, int maxit, int m) {
+    KrylovStats st;
+    const long n = A.n;
+    if (n == 0) { st.converged = true; return st; }
+    m = (int)std::max<long>(1, std::min<long>(m, n));
+    Work W;
+    W.init(n, m);
+    const double bnorm = dev_norm(s, W, b);
+    if (bnorm == 0) {
+        hip_check(blas_axpby(0, nullptr, false, b, 0, x, n, s), "zero");
+        st.converged = true;
+        return st;
+    }
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h;
+    while (st.iters < maxit) {
+        // r = b - A x  -> V0
+        A.apply(x, W.Vi(0));
+        ++st.iters;
+        hip_check(blas_axpby(1.0, nullptr, false, b, -1.0, W.Vi(0), n, s), "residual");
+        const double beta = dev_norm(s, W, W.Vi(0));
+        st.relres = beta / bnorm;
+        if (beta < tol * bnorm) { st.converged = true; break; }
+        hip_check(blas_axpby(1.0 / beta, nullptr, false, W.Vi(0), 0.0, W.Vi(0), n, s), "scale");
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        for (int j = 0; j < m && st.iters < maxit; ++j) {
+            double* w = W.Vi(j + 1);
+            A.apply(W.Vi(j), w);
+            ++st.iters;
+            cgs2(s, W, j + 1, w, h);
+            const double hn = dev_norm(s, W, w);
+            for (int i = 0; i <= j; ++i) H[i + (size_t)j * (m + 1)] = h[i];
+            H[j + 1 + (size_t)j * (m + 1)] = hn;
+            for (int i = 0; i < j; ++i) {  // previous Givens rotations
+                const double t1 = H[i + (size_t)j * (m + 1)], t2 = H[i + 1 + (size_t)j * (m + 1)];
+                H[i + (size_t)j * (m + 1)] = cs[i] * t1 + sn[i] * t2;
+                H[i + 1 + (size_t)j * (m + 1)] = -sn[i] * t1 + cs[i] * t2;
+            }
+            const double hjj = H[j + (size_t)j * (m + 1)];
+            const double beta2 = std::hypot(hjj, hn);
+            k = j + 1;
+            if (beta2 == 0) break;
+            cs[j] = hjj / beta2;
+            sn[j] = hn / beta2;
+            H[j + (size_t)j * (m + 1)] = beta2;
+            H[j + 1 + (size_t)j * (m + 1)] = 0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] *= cs[j];
+            st.relres = std::abs(g[j + 1]) / bnorm;
+            if (hn > 0) hip_check(blas_axpby(1.0 / hn, nullptr, false, w, 0.0, w, n, s), "normalise");
+            if (st.relres < tol || hn == 0) break;
+        }
+        // x += V[0..k) * R^{-1} g
+        std::vector<double> y(k, 0.0);
+        for (int i = k - 1; i >= 0; --i) {
+            double t = g[i];
+            for (int c = i + 1; c < k; ++c) t -= H[i + (size_t)c * (m + 1)] * y[c];
+            y[i] = t / H[i + (size_t)i * (m + 1)];
+        }
+        for (auto& v : y) v = -v;  // gemv_sub subtracts
+        h2d(s, W.dh.d(), y.data(), k);
+        hip_check(blas_gemv_sub(W.V.d(), n, k, W.dh.d(), x, n, s), "update");
+        hip_check(hipStreamSynchronize(s), "sync");
+        if (st.relres < tol) {
+            // confirm with the true residual on the next pass
+            continue;
+        }
+    }
+    if (!st.converged && st.iters >= maxit) {
+        // final true residual
+        A.apply(x, W.Vi(0));
+        hip_check(blas_axpby(1.0, nullptr, false, b, -1.0, W.Vi(0), n, s), "residual");
+        st.relres = dev_norm(s, W, W.Vi(0)) / bnorm;
+        st.converged = st.relres < tol;
+    }
+    return st;
+}
+
+KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, const double* b, double* x, double tol, int maxit, int m,
+                          double inner_tol) {
+    KrylovStats st;
+    const long n = A64.n;
+    Work W;
+    W.init(n, 1);
+    DevBuf r, dx;
+    r.ensure(sizeof(double) * std::max<long>(n, 1));
+    dx.ensure(sizeof(double) * std::max<long>(n, 1));
+    const double bnorm = dev_norm(s, W, b);
+    if (bnorm == 0) {
+        hip_check(blas_axpby(0, nullptr, false, b, 0, x, n, s), "zero");
+        st.converged = true;
+        return st;
+    }
+    // Each inner solve only needs to reduce the residual by a constant factor; it is capped at
+    // 4 restart cycles so that stagnation in low precision cannot use up the budget.
+    while (st.iters < maxit) {
+        A64.apply(x, r.d());
+        ++st.iters;
+        hip_check(blas_axpby(1.0, nullptr, false, b, -1.0, r.d(), n, s), "residual");
+        const double rn = dev_norm(s, W, r.d());
+        st.relres = rn / bnorm;
+        if (st.relres < tol) { st.converged = true; break; }
+        if (st.iters >= maxit) break;
+        hip_check(blas_axpby(0, nullptr, false, r.d(), 0, dx.d(), n, s), "zero");
+        const int budget = std::min(maxit - st.iters, 4 * (m + 1));
+        KrylovStats in = gmres(s, Alo, r.d(), dx.d(), std::max(inner_tol, 0.5 * tol * bnorm / rn), budget, m);
+        st.iters += in.iters;
+        hip_check(blas_axpby(1.0, nullptr, false, dx.d(), 1.0, x, n, s), "update");
+    }
+    return st;
+}
+
+KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol, int maxit, int m) {
+    KrylovStats st;
+    const long n = A.n;
+    m = (int)std::max<long>(2, std::min<long>(m, n));
+    Work W;
+    W.init(n, m);
+    double nx = dev_norm(s, W, x);
+    if (nx == 0) throw InvalidArg("lanczos: zero start vector");
+    hip_check(blas_axpby(1.0 / nx, nullptr, false, x, 0.0, W.Vi(0), n, s), "scale");
+    std::vector<double> T, h, w, Z;
+    while (true) {
+        // Krylov basis with full reorthogonalisation
+        T.assign((size_t)m * m, 0.0);
+        int k = 0;
+        for (int j = 0; j < m; ++j) {
+            double* wv = W.Vi(j + 1 <= m ? j + 1 : m);
+            A.apply(W.Vi(j), wv);
+            ++st.iters;
+            cgs2(s, W, j + 1, wv, h);
+            const double beta = dev_norm(s, W, wv);
+            for (int i = 0; i <= j; ++i) { T[i + (size_t)j * m] = h[i]; T[j + (size_t)i * m] = h[i]; }
+            k = j + 1;
+            if (j + 1 < m) {
+                T[j + 1 + (size_t)j * m] = beta;
+                T[j + (size_t)(j + 1) * m] = beta;
+            }
+            if (beta <= 1e-14 * std::abs(h[j]) || j + 1 == m || st.iters >= maxit) {
+                // Ritz pair of the k x k projection
+                std::vector<double> Tk((size_t)k * k);
+                for (int c = 0; c < k; ++c) for (int r = 0; r < k; ++r) Tk[r + (size_t)c * k] = T[r + (size_t)c * m];
+                jacobi_eig(k, Tk, w, Z);
+                int imin = (int)(std::min_element(w.begin(), w.end()) - w.begin());
+                const double theta = w[imin];
+                const double res = std::abs(beta * Z[(k - 1) + (size_t)imin * k]);
+                st.value = theta;
+                st.relres = res / std::max(std::abs(theta), 1e-300);
+                // x = V[0..k) z
+                std::vector<double> z(k);
+                for (int i = 0; i < k; ++i) z[i] = -Z[i + (size_t)imin * k];
+                h2d(s, W.dh.d(), z.data(), k);
+                hip_check(blas_axpby(0, nullptr, false, x, 0, x, n, s), "zero");
+                hip_check(blas_gemv_sub(W.V.d(), n, k, W.dh.d(), x, n, s), "ritz");
+                nx = dev_norm(s, W, x);
+                hip_check(blas_axpby(1.0 / nx, nullptr, false, x, 0.0, x, n, s), "scale");
+                break;
+            }
+            hip_check(blas_axpby(1.0 / beta, nullptr, false, wv, 0.0, wv, n, s), "normalise");
+        }
+        if (st.relres <= tol || k < 2) { st.converged = st.relres <= tol; break; }
+        if (st.iters >= maxit) break;
+        d2d(s, W.Vi(0), x, n);
+    }
+    // true residual ||A x - theta x|| / |theta|
+    A.apply(x, W.w.d());
+    hip_check(blas_axpby(-st.value, nullptr, false, x, 1.0, W.w.d(), n, s), "resid");
+    st.relres = dev_norm(s, W, W.w.d()) / std::max(std::abs(st.value), 1e-300);
+    st.converged = st.relres <= std::max(tol, 1e-14) * 10;
+    return st;
+}
+
+// ------------------------------------------------------------------------------------------------
+namespace {
+// BlockDiagMat layout of the left (qin) or right (qout) bond of an MPS tensor
+Layout bond_layout(const Layout& A, bool right) {
+    std::map<Key, int> m;
+    for (auto& bl : A.l)
+        for (auto& b : bl.b) {
+            const Key& k = right ? b.qout : b.qin;
+            const int sz = right ? b.cols : b.rows;
+            auto it = m.find(k);
+            if (it == m.end()) m.emplace(k, sz);
+            else if (it->second != sz) throw InvalidArg("env: inconsistent sector sizes on a bond");
+        }
+    Layout L;
+    L.nsym = A.nsym;
+    L.l.resize(1);
+    long pos = 0;
+    for (auto& kv : m) {
+        Blk b{kv.first, kv.first, kv.second, kv.second, pos};
+        pos += b.size();
+        L.l[0].b.push_back(b);
+    }
+    L.nelem = pos;
+    return L;
+}
+
+std::unique_ptr<imps_plan_s> tm_plan(imps_context ctx, int dir, const Layout& A, const Layout& x, const Layout& y, int prec) {
+    std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_TM, prec));
+    P->prog = build_tm(dir, A, nullptr, x, y);
+    P->op_parts = {A.nelem};
+    upload_program(P.get());
+    return P;
+}
+}  // namespace
+
+void EnvSolver::build(imps_context c, int direction, const std::vector<Layout>& A, const Layout& x, int precision) {
+    if (!c) throw InvalidArg("env: a device context is required");
+    if (A.empty()) throw InvalidArg("env: empty unit cell");
+    if (x.l.size() != 1) throw InvalidArg("env: x must be a BlockDiagMat layout");
+    ctx = c;
+    dir = direction;
+    prec = precision;
+    ncell = (int)A.size();
+    xlay = x;
+    // application order and bond layouts
+    std::vector<int> order(ncell);
+    for (int k = 0; k < ncell; ++k) order[k] = dir == IMPS_LEFT ? k : ncell - 1 - k;
+    Layout in = x;
+    bond_elems.clear();
+    for (int k = 0; k < ncell; ++k) {
+        const Layout& Ak = A[order[k]];
+        Layout out = (k == ncell - 1) ? x : bond_layout(Ak, dir == IMPS_LEFT);
+        tm64.push_back(tm_plan(ctx, dir, Ak, in, out, IMPS_FP64));
+        if (prec != IMPS_FP64) tmlo.push_back(tm_plan(ctx, dir, Ak, in, out, prec));
+        if (k < ncell - 1) bond_elems.push_back(out.nelem);
+        in = out;
+    }
+    long tot = 0;
+    for (long e : bond_elems) tot += e;
+    bonds.ensure(sizeof(double) * std::max<long>(tot, 1));
+    rhoT.ensure(sizeof(double) * std::max<long>(x.nelem, 1));
+    rhs.ensure(sizeof(double) * std::max<long>(x.nelem, 1));
+    tmp.ensure(sizeof(double) * std::max<long>(x.nelem, 1));
+}
+
+void EnvSolver::set_mps(const std::vector<const double*>& A) {
+    if ((int)A.size() != ncell) throw InvalidArg("env: wrong number of MPS tensors");
+    for (int k = 0; k < ncell; ++k) {
+        const int idx = dir == IMPS_LEFT ? k : ncell - 1 - k;
+        set_operator(tm64[k].get(), {A[idx]});
+        if (!tmlo.empty()) set_operator(tmlo[k].get(), {A[idx]});
+    }
+    mps_set = true;
+}
+
+KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, const double* y0, double* y, double tol, int maxit,
+                             int krylov) {
+    if (!mps_set) throw InvalidArg("env: MPS data not set");
+    if (!x || !L || !R || !y) throw InvalidArg("env: NULL vector");
+    hipStream_t s = ctx->stream;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const long n = xlay.nelem;
+    // rho_in / rho_out: l -> (R, L), r -> (L, R).  tr(v rho_in) = <v, rho_in^T> with the blockwise transpose.
+    const double* rho_in = dir == IMPS_LEFT ? R : L;
+    const double* rho_out = dir == IMPS_LEFT ? L : R;
+    {
+        std::vector<double> h(n), t(n);
+        d2h(s, h.data(), rho_in, n);
+        for (auto& b : xlay.l[0].b)
+            for (int j = 0; j < b.cols; ++j)
+                for (int i = 0; i < b.rows; ++i) t[b.off + j + (long)i * b.rows] = h[b.off + i + (long)j * b.rows];
+        h2d(s, rhoT.d(), t.data(), n);
+    }
+    DevBuf dd;
+    dd.ensure(sizeof(double) * (blas_scratch_doubles(1) + 4));
+    auto trace_in = [&](const double* v) {
+        double t = 0;
+        hip_check(blas_multidot(rhoT.d(), 0, 1, v, n, dd.d(), dd.d() + 4, s), "trace");
+        d2h(s, &t, dd.d(), 1);
+        return t;
+    };
+    auto make_op = [&](std::vector<std::unique_ptr<imps_plan_s>>& chain) {
+        LinOp op;
+        op.n = n;
+        op.apply = [&, n](const double* v, double* out) {
+            // out = v - T(v) + tr(v rho_in) rho_out
+            const double* in = v;
+            long off = 0;
+            for (int k = 0; k < ncell; ++k) {
+                double* o = (k == ncell - 1) ? tmp.d() : bonds.d() + off;
+                run(chain[k].get(), in, o, 1);
+                in = o;
+                if (k < ncell - 1) off += bond_elems[k];
+            }
+            const double tr = trace_in(v);
+            d2d(s, out, v, n);
+            hip_check(blas_axpby(-1.0, nullptr, false, tmp.d(), 1.0, out, n, s), "1-T");
+            hip_check(blas_axpby(tr, nullptr, false, rho_out, 1.0, out, n, s), "projector");
+        };
+        return op;
+    };
+    // right-hand side: x - tr(x rho_in) rho_out  (InvEBlockFunctions.hpp:243, :252)
+    const double trx = trace_in(x);
+    d2d(s, rhs.d(), x, n);
+    hip_check(blas_axpby(-trx, nullptr, false, rho_out, 1.0, rhs.d(), n, s), "rhs");
+    if (y0) d2d(s, y, y0, n);
+    else hip_check(blas_axpby(0, nullptr, false, x, 0, y, n, s), "zero y");
+    if (maxit <= 0) maxit = (int)std::min<long>(std::max<long>(20 * n, 1000), 100000);
+    if (krylov <= 0) krylov = 40;
+    LinOp A64 = make_op(tm64);
+    KrylovStats st;
+    if (prec == IMPS_FP64) st = gmres(s, A64, rhs.d(), y, tol, maxit, krylov);
+    else {
+        LinOp Alo = make_op(tmlo);
+        st = gmres_refined(s, A64, Alo, rhs.d(), y, tol, maxit, krylov, prec == IMPS_FP32 ? 1e-4 : 1e-3);
+    }
+    hip_check(hipStreamSynchronize(s), "sync");
+    return st;
+}
+
+}  // namespace imps

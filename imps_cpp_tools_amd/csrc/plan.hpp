@@ -1,0 +1,47 @@
+// Plans: a compiled grouped-GEMM Program, its device copy and the operator store.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../../include/imps_capi.h"
+#include "common.hpp"
+#include "engine.hpp"
+
+struct imps_context_s {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+namespace imps {
+
+enum Kind { K_TM = 0, K_HAC = 1, K_HC = 2, K_REDOP = 3, K_ENV = 4 };
+
+struct EnvSolver;  // krylov.cpp
+
+}  // namespace imps
+
+struct imps_plan_s {
+    imps_context ctx = nullptr;
+    int kind = 0;
+    int prec = IMPS_FP64;
+    imps::Program prog;
+    imps::Layout out_layout;      // derived output structure (redop)
+    std::vector<long> op_parts;   // element counts of the operator-store pieces, in set order
+    bool op_set = false;
+    imps::DevBuf d_prog, d_op, d_w;
+    std::vector<imps::DevStage> dstages;
+    std::shared_ptr<imps::EnvSolver> env;  // K_ENV
+    size_t elem_bytes() const { return prec == IMPS_FP64 ? 8 : 4; }
+};
+
+namespace imps {
+
+imps_plan_s* new_plan(imps_context ctx, int kind, int prec);
+void upload_program(imps_plan_s* P);
+void set_operator(imps_plan_s* P, const std::vector<const double*>& parts);
+// y = op(x) for nvec vectors on the context stream
+void run(imps_plan_s* P, const double* x, double* y, int nvec);
+
+}  // namespace imps

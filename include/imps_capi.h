@@ -128,6 +128,32 @@ imps_status imps_redop_plan(imps_context ctx, imps_plan* plan, const double* H, 
                             const imps_layout* A, int dir, int prec);
 imps_status imps_redop_apply(imps_plan plan, const double* A, double* y);
 
+/* ---- environments (A14/A15) ---------------------------------------------------------------
+ * InvertE_proj (InvEBlockFunctions.hpp:205-266) with the geometric-sum map of InvEfun_proj (:13-47)
+ * over a unit cell of ncell MPS tensors (chained TM, TMBlockFunctions.hpp:565-615):
+ *   dir l: (y|[1 - T + |R)(L|] = (x|[1 - |R)(L|],   T(y) = T_L[A_{N-1}](... T_L[A_0](y))
+ *   dir r: [1 - T + |R)(L|]|y) = [1 - |R)(L|]|x),   T(y) = T_R[A_0](... T_R[A_{N-1}](y))
+ * x, L, R, y: BlockDiagMat vectors on the layout x.  Solved by restarted GMRES(krylov) with FP64
+ * CGS2 orthogonalisation; with prec != IMPS_FP64 the TM applies of the inner solves run in that
+ * precision inside FP64 residual refinement.  Stops at ||b - A y|| < tol ||b|| (the rule of gmres,
+ * IterativeSolvers.hpp:119).  y0 (optional) is the initial guess.  Returns IMPS_NOT_CONVERGED
+ * (with y, iters and relres written) when maxit operator applications do not suffice. */
+imps_status imps_env_plan(imps_context ctx, imps_plan* plan, int dir, int ncell, const imps_layout* A,
+                          const imps_layout* x, int prec);
+imps_status imps_env_set_mps(imps_plan plan, const double* const* A);
+imps_status imps_env_solve(imps_plan plan, const double* x, const double* L, const double* R,
+                           const double* y0, double* y, double tol, int maxit, int krylov,
+                           int* iters, double* relres);
+
+/* ---- lowest eigenpair (A17) ----------------------------------------------------------------
+ * Replaces eigs_rs(MultOPx, N, vals, vecs, 1, "SA", tol, x0) (eigs.h:43-44; eigs.cpp:369-494) for
+ * an H_AC or H_C plan: restarted Lanczos with full FP64 reorthogonalisation.  x: start vector in,
+ * normalised eigenvector out; converged when ||H x - e x|| <= tol |e| (ARPACK's criterion).  With a
+ * low-precision plan and an FP64 plan `plan_fp64` of the same operator, the Lanczos runs first in
+ * low precision and is then refined with FP64 applies from its Ritz vector. */
+imps_status imps_eigsh(imps_plan plan, imps_plan plan_fp64, double* x, double tol, int maxit,
+                       int krylov, double* eval, int* iters, double* resid);
+
 /* ---- bookkeeping (A1/A2) -------------------------------------------------------------------
  * model: 0 = XXZ spin-1/2 (one U(1)), 1 = Fermi-Hubbard in (N,M) representation (U(1)xU(1)).
  * imps_model_i2k: GetI2K (Models.cpp:262-295, :699-733); writes d*nsym charges, returns d or -1.
