@@ -80,6 +80,10 @@ def lib():
             "imps_env_set_mps": (I, [P, ctypes.POINTER(V)]),
             "imps_env_solve": (I, [P, V, V, V, V, V, D, I, I, ip, dp]),
             "imps_eigsh": (I, [P, P, V, D, I, I, dp, ip, dp]),
+            "imps_prod_plan": (I, [P, ctypes.POINTER(P), I, LP, LP]),
+            "imps_prod_apply": (I, [P, V, V, V]),
+            "imps_qr_plan": (I, [P, ctypes.POINTER(P), LP, I]),
+            "imps_qr_apply": (I, [P, V, V, V]),
             "imps_model_nsym": (I, [I]),
             "imps_model_d": (I, [I]),
             "imps_model_i2k": (I, [I, I, ip, ip]),
@@ -312,6 +316,19 @@ class Plan:
     def __init__(self, handle):
         self.h = handle
 
+    def _out_layout(self, nsym: int) -> BlockLayout:
+        nl, nb, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        z = ctypes.POINTER(ctypes.c_int)()
+        _check(lib().imps_plan_out_layout(self.h, ctypes.byref(nl), ctypes.byref(nb), ctypes.byref(ne), z, z, z, z, z))
+        n = nb.value
+        nblk = np.zeros(nl.value, np.int32)
+        qin = np.zeros(max(n, 1) * nsym, np.int32)
+        qout = np.zeros(max(n, 1) * nsym, np.int32)
+        rows = np.zeros(max(n, 1), np.int32)
+        cols = np.zeros(max(n, 1), np.int32)
+        _check(lib().imps_plan_out_layout(self.h, None, None, None, _iptr(nblk), _iptr(qin), _iptr(qout), _iptr(rows), _iptr(cols)))
+        return BlockLayout(nsym, nblk, qin[:n * nsym], qout[:n * nsym], rows[:n], cols[:n])
+
     def cost(self):
         f, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().imps_plan_cost(self.h, ctypes.byref(f), ctypes.byref(b)))
@@ -410,19 +427,6 @@ class RedOpPlan(Plan):
         super().__init__(h)
         self.out_layout = self._out_layout(A.nsym)
 
-    def _out_layout(self, nsym: int) -> BlockLayout:
-        nl, nb, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
-        z = ctypes.POINTER(ctypes.c_int)()
-        _check(lib().imps_plan_out_layout(self.h, ctypes.byref(nl), ctypes.byref(nb), ctypes.byref(ne), z, z, z, z, z))
-        nblk = np.zeros(nl.value, np.int32)
-        qin = np.zeros(max(nb.value, 1) * nsym, np.int32)
-        qout = np.zeros(max(nb.value, 1) * nsym, np.int32)
-        rows = np.zeros(max(nb.value, 1), np.int32)
-        cols = np.zeros(max(nb.value, 1), np.int32)
-        _check(lib().imps_plan_out_layout(self.h, None, None, None, _iptr(nblk), _iptr(qin), _iptr(qout), _iptr(rows), _iptr(cols)))
-        n = nb.value
-        return BlockLayout(nsym, nblk, qin[:n * nsym], qout[:n * nsym], rows[:n], cols[:n])
-
     def apply(self, A, y):
         _check(lib().imps_redop_apply(self.h, _ptr(A), _ptr(y)))
 
@@ -460,3 +464,36 @@ def eigsh(plan: Plan, x, tol: float = 1e-12, maxit: int = 0, krylov: int = 20, p
     if check:
         _check(st)
     return ev.value, it.value, rr.value
+
+
+MAT_MPS, MPS_MAT, MPS_MATT, MATT_MPS, TM_LEFT_UNITY, TM_RIGHT_UNITY, MPS_MPS = range(7)
+
+
+class ProdPlan(Plan):
+    """Two-operand block product (FP64); output layout in .out_layout."""
+    kind = "prod"
+
+    def __init__(self, ctx: Optional[Context], kind: int, X1: BlockLayout, X2: BlockLayout):
+        h = ctypes.c_void_p()
+        c1, c2 = X1.c(), X2.c()
+        _check(lib().imps_prod_plan(_ctx_handle(ctx), ctypes.byref(h), kind, ctypes.byref(c1), ctypes.byref(c2)))
+        super().__init__(h)
+        self.out_layout = self._out_layout(X1.nsym)
+
+    def apply(self, x1, x2, y):
+        _check(lib().imps_prod_apply(self.h, _ptr(x1), _ptr(x2), _ptr(y)))
+
+
+class QRPlan(Plan):
+    """Per-sector QR of an MPS tensor (qr(R, A, dir) with qr_pos), FP64 on the GPU; R layout in .out_layout."""
+    kind = "qr"
+
+    def __init__(self, ctx: Optional[Context], A: BlockLayout, direction: int):
+        h = ctypes.c_void_p()
+        cA = A.c()
+        _check(lib().imps_qr_plan(_ctx_handle(ctx), ctypes.byref(h), ctypes.byref(cA), direction))
+        super().__init__(h)
+        self.out_layout = self._out_layout(A.nsym)
+
+    def apply(self, A, Q, R):
+        _check(lib().imps_qr_apply(self.h, _ptr(A), _ptr(Q), _ptr(R)))

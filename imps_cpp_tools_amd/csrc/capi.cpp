@@ -331,4 +331,63 @@ imps_status imps_eigsh(imps_plan P, imps_plan P64, double* x, double tol, int ma
     });
 }
 
+// ---- block products (A6) ----
+imps_status imps_prod_plan(imps_context ctx, imps_plan* plan, int kind, const imps_layout* X1, const imps_layout* X2) {
+    return guarded([&] {
+        if (!plan) throw InvalidArg("plan is NULL");
+        std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_PROD, IMPS_FP64));
+        Layout a = layout_from_c(X1, "X1"), b = layout_from_c(X2, "X2");
+        P->prog = build_prod(kind, a, b, P->out_layout);
+        P->op_set = true;
+        return finish_plan(plan, P);
+    });
+}
+imps_status imps_prod_apply(imps_plan P, const double* x1, const double* x2, double* y) {
+    return guarded([&] {
+        check_kind(P, K_PROD);
+        if (!x2 && P->prog.op_elems > 0) throw InvalidArg("x2 is NULL");
+        run(P, x1, y, 1, x2);
+        return IMPS_OK;
+    });
+}
+
+// ---- per-sector QR (A18) ----
+imps_status imps_qr_plan(imps_context ctx, imps_plan* plan, const imps_layout* A, int dir) {
+    return guarded([&] {
+        if (!plan) throw InvalidArg("plan is NULL");
+        if (dir != IMPS_LEFT && dir != IMPS_RIGHT) throw InvalidArg("dir must be IMPS_LEFT or IMPS_RIGHT");
+        std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_QR, IMPS_FP64));
+        Layout a = layout_from_c(A, "A");
+        P->qr = build_qr(a, dir);
+        P->qr_dir = dir;
+        P->out_layout = P->qr.R;
+        P->prog.x_elems = a.nelem;
+        P->prog.y_elems = a.nelem;
+        P->op_set = true;
+        if (ctx) {
+            hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+            const size_t b1 = P->qr.secs.size() * sizeof(QrSector), b2 = P->qr.parts.size() * sizeof(QrPart);
+            P->d_prog.ensure(b1 + b2 + 64);
+            if (b1) hip_check(hipMemcpy(P->d_prog.p, P->qr.secs.data(), b1, hipMemcpyHostToDevice), "upload");
+            if (b2) hip_check(hipMemcpy((char*)P->d_prog.p + ((b1 + 15) & ~size_t(15)), P->qr.parts.data(), b2, hipMemcpyHostToDevice), "upload");
+        }
+        *plan = P.release();
+        return IMPS_OK;
+    });
+}
+imps_status imps_qr_apply(imps_plan P, const double* A, double* Q, double* R) {
+    return guarded([&] {
+        check_kind(P, K_QR);
+        if (!P->ctx) throw InvalidArg("plan was created without a context");
+        if (!A || !Q || !R) throw InvalidArg("NULL buffer");
+        hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+        const size_t b1 = P->qr.secs.size() * sizeof(QrSector);
+        const QrSector* secs = (const QrSector*)P->d_prog.p;
+        const QrPart* parts = (const QrPart*)((char*)P->d_prog.p + ((b1 + 15) & ~size_t(15)));
+        hip_check(launch_sector_qr(secs, (int)P->qr.secs.size(), parts, A, Q, R, P->qr_dir == IMPS_RIGHT, P->qr.lds, P->ctx->stream),
+                  "sector qr");
+        return IMPS_OK;
+    });
+}
+
 }  // extern "C"

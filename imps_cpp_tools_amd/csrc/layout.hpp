@@ -369,6 +369,133 @@ inline Program build_hc(const Layout& C, const Layout& EHL, const Layout& EHR, c
 }
 
 // ---------------------------------------------------------------------------------------------
+// Two-operand block products (BlockObj.hpp:2557-2625, MPSBlockMat.hpp:934-1043,
+// TMBlockFunctions.hpp:259-470).  Operand 1 is the X slot, operand 2 the OP slot (FP64 plans bind
+// it to the caller's buffer).  The output layout is derived from the operands' structure.
+enum ProdKind : int {
+    PROD_MAT_MPS = 0,    // y[s] = M * A[s]           (M: BlockMat/BlockDiagMat, A: MPS)
+    PROD_MPS_MAT = 1,    // y[s] = A[s] * M
+    PROD_MPS_MATT = 2,   // y[s] = A[s] * M^T
+    PROD_MATT_MPS = 3,   // y[s] = M^T * A[s]
+    PROD_TM_LEFT = 4,    // y = sum_s B_s^T A_s, keyed by Qout(B_s) (ApplyTMmixedLeft(A,B) unity)
+    PROD_TM_RIGHT = 5,   // y = sum_s A_s B_s^T, keyed by Qin(A_s)  (ApplyTMmixedRight(A,B) unity)
+    PROD_MPS_MPS = 6     // y[i*d2 + j] = A[i] * B[j] (two-site concatenation)
+};
+
+// Per-entry product lhs * rhs of two BlockMats with the BlockMat operator* matching (lhs Qout =
+// rhs Qin); lhs/rhs may be used transposed.  Output blocks keyed by the lhs row charge.
+inline void prod_entry(const BList& L, long l0, int lslot, bool ltr, const BList& R, long r0, int rslot, bool rtr, BList& out,
+                       std::vector<OutBlock>& obs, long& pos, double& flops) {
+    // rows of op(L) are keyed by Qin (no transpose) or Qout (transpose); index op(R) blocks by row key
+    std::map<Key, const Blk*> rrow;
+    for (const Blk& b : R.b) rrow[rtr ? b.qout : b.qin] = &b;
+    std::map<Key, std::pair<Blk, OutBlock>> acc;
+    for (const Blk& a : L.b) {
+        const Key arow = ltr ? a.qout : a.qin, acol = ltr ? a.qin : a.qout;
+        auto it = rrow.find(acol);
+        if (it == rrow.end()) continue;
+        const Blk& b = *it->second;
+        const Key bcol = rtr ? b.qin : b.qout;
+        const int M = ltr ? a.cols : a.rows, N = rtr ? b.rows : b.cols;
+        auto ins = acc.find(arow);
+        if (ins == acc.end()) ins = acc.emplace(arow, std::make_pair(Blk{arow, bcol, M, N, 0}, OutBlock{0, M, N, {}})).first;
+        else if (ins->second.first.qout != bcol) throw InvalidArg("product: inconsistent charge flow");
+        add_seg(ins->second.second, opnd(lslot, a, l0, ltr), opnd(rslot, b, r0, rtr), 1.0, flops);
+    }
+    for (auto& kv : acc) {
+        Blk b = kv.second.first;
+        b.off = pos;
+        OutBlock ob = std::move(kv.second.second);
+        ob.c_off = pos;
+        pos += b.size();
+        out.b.push_back(b);
+        obs.push_back(std::move(ob));
+    }
+}
+
+inline Program build_prod(int kind, const Layout& X1, const Layout& X2, Layout& out) {
+    require_same_nsym(X1, X2, "prod");
+    Program P;
+    P.x_elems = X1.nelem;
+    P.op_elems = X2.nelem;
+    out = Layout();
+    out.nsym = X1.nsym;
+    long pos = 0;
+    std::vector<OutBlock> obs;
+    auto entry_offset = [](const Layout& L, size_t e) { return L.l[e].b.empty() ? 0L : L.l[e].b.front().off; };
+    (void)entry_offset;
+    switch (kind) {
+        case PROD_MAT_MPS:
+        case PROD_MATT_MPS:
+            if (X1.l.size() != 1) throw InvalidArg("prod: M must have nlist = 1");
+            out.l.resize(X2.l.size());
+            for (size_t s = 0; s < X2.l.size(); ++s)
+                prod_entry(X1.l[0], 0, SLOT_X, kind == PROD_MATT_MPS, X2.l[s], 0, SLOT_OP, false, out.l[s], obs, pos, P.flops);
+            break;
+        case PROD_MPS_MAT:
+        case PROD_MPS_MATT:
+            if (X2.l.size() != 1) throw InvalidArg("prod: M must have nlist = 1");
+            out.l.resize(X1.l.size());
+            for (size_t s = 0; s < X1.l.size(); ++s)
+                prod_entry(X1.l[s], 0, SLOT_X, false, X2.l[0], 0, SLOT_OP, kind == PROD_MPS_MATT, out.l[s], obs, pos, P.flops);
+            break;
+        case PROD_MPS_MPS:
+            out.l.resize(X1.l.size() * X2.l.size());
+            for (size_t i = 0; i < X1.l.size(); ++i)
+                for (size_t j = 0; j < X2.l.size(); ++j)
+                    prod_entry(X1.l[i], 0, SLOT_X, false, X2.l[j], 0, SLOT_OP, false, out.l[i * X2.l.size() + j], obs, pos, P.flops);
+            break;
+        case PROD_TM_LEFT:
+        case PROD_TM_RIGHT: {
+            if (X1.l.size() != X2.l.size()) throw InvalidArg("prod: A and B need the same physical dimension");
+            // accumulate over s into blocks keyed by the output row charge
+            std::map<Key, std::pair<Blk, OutBlock>> acc;
+            for (size_t s = 0; s < X1.l.size(); ++s) {
+                if (kind == PROD_TM_LEFT) {  // B_s^T A_s: match Qin(B) with Qin(A)
+                    for (const Blk& a : X1.l[s].b) {
+                        const Blk* b = X2.l[s].find(a.qin);
+                        if (!b) continue;
+                        auto it = acc.find(b->qout);
+                        if (it == acc.end()) it = acc.emplace(b->qout, std::make_pair(Blk{b->qout, a.qout, b->cols, a.cols, 0}, OutBlock{0, b->cols, a.cols, {}})).first;
+                        else if (it->second.first.qout != a.qout) throw InvalidArg("prod: inconsistent charge flow");
+                        add_seg(it->second.second, opnd(SLOT_OP, *b, 0, true), opnd(SLOT_X, a, 0), 1.0, P.flops);
+                    }
+                } else {  // A_s B_s^T: match Qout(A) with Qout(B)
+                    Key dKB;
+                    if (!X2.l[s].dK(dKB)) continue;
+                    for (const Blk& a : X1.l[s].b) {
+                        const Blk* b = X2.l[s].find(a.qout - dKB);
+                        if (!b) continue;
+                        auto it = acc.find(a.qin);
+                        if (it == acc.end()) it = acc.emplace(a.qin, std::make_pair(Blk{a.qin, b->qin, a.rows, b->rows, 0}, OutBlock{0, a.rows, b->rows, {}})).first;
+                        else if (it->second.first.qout != b->qin) throw InvalidArg("prod: inconsistent charge flow");
+                        add_seg(it->second.second, opnd(SLOT_X, a, 0), opnd(SLOT_OP, *b, 0, true), 1.0, P.flops);
+                    }
+                }
+            }
+            out.l.resize(1);
+            for (auto& kv : acc) {
+                Blk b = kv.second.first;
+                b.off = pos;
+                OutBlock ob = std::move(kv.second.second);
+                ob.c_off = pos;
+                pos += b.size();
+                out.l[0].b.push_back(b);
+                obs.push_back(std::move(ob));
+            }
+            break;
+        }
+        default:
+            throw InvalidArg("prod: unknown kind");
+    }
+    out.nelem = pos;
+    P.y_elems = pos;
+    emit_stage(P, obs, SLOT_OUT);
+    P.operand_elems_read = X1.nelem + X2.nelem;
+    return P;
+}
+
+// ---------------------------------------------------------------------------------------------
 // ReducedOp (MPSBlockUtilities.hpp:39-74).  Single stage; the A tensor is the "x" operand.
 // l: R[i1][j1] = sum h * A[i0]^T A[j0] ; r: R[i0][j0] = sum h * A[j1] A[i1]^T.
 // The nonzeros of H are visited column by column (the order of an Armadillo SpMat iterator).

@@ -62,17 +62,18 @@ void set_operator(imps_plan_s* P, const std::vector<const double*>& parts) {
     P->op_set = true;
 }
 
-void run(imps_plan_s* P, const double* x, double* y, int nvec) {
+void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_override) {
     if (!P->ctx) throw InvalidArg("plan was created without a context");
     if (nvec < 1) throw InvalidArg("nvec must be >= 1");
     if ((P->prog.x_elems > 0 && !x) || (P->prog.y_elems > 0 && !y)) throw InvalidArg("x or y is NULL");
-    if (!P->op_set && P->prog.op_elems > 0) throw InvalidArg("operator data not set");
+    if (!P->op_set && !op_override && P->prog.op_elems > 0) throw InvalidArg("operator data not set");
     hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
     hipStream_t s = P->ctx->stream;
     P->d_w.ensure((size_t)std::max<long>(P->prog.w_elems, 1) * nvec * P->elem_bytes());
     Slots sl;
     sl.base[SLOT_X] = x;           sl.stride[SLOT_X] = P->prog.x_elems;  // FP64, converted in-kernel
-    sl.base[SLOT_OP] = P->d_op.p;  sl.stride[SLOT_OP] = 0;
+    sl.base[SLOT_OP] = op_override ? (const void*)op_override : P->d_op.p;
+    sl.stride[SLOT_OP] = 0;
     sl.base[SLOT_W] = P->d_w.p;    sl.stride[SLOT_W] = P->prog.w_elems;
     for (auto& ds : P->dstages) {
         if (ds.out_slot == SLOT_W)

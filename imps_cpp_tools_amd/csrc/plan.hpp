@@ -7,6 +7,7 @@
 #include "../../include/imps_capi.h"
 #include "common.hpp"
 #include "engine.hpp"
+#include "qr.hpp"
 
 struct imps_context_s {
     int device = 0;
@@ -16,7 +17,7 @@ struct imps_context_s {
 
 namespace imps {
 
-enum Kind { K_TM = 0, K_HAC = 1, K_HC = 2, K_REDOP = 3, K_ENV = 4 };
+enum Kind { K_TM = 0, K_HAC = 1, K_HC = 2, K_REDOP = 3, K_ENV = 4, K_PROD = 5, K_QR = 6 };
 
 struct EnvSolver;  // krylov.cpp
 
@@ -33,6 +34,8 @@ struct imps_plan_s {
     imps::DevBuf d_prog, d_op, d_w;
     std::vector<imps::DevStage> dstages;
     std::shared_ptr<imps::EnvSolver> env;  // K_ENV
+    imps::QrPlanData qr;                   // K_QR
+    int qr_dir = 0;
     size_t elem_bytes() const { return prec == IMPS_FP64 ? 8 : 4; }
 };
 
@@ -41,7 +44,8 @@ namespace imps {
 imps_plan_s* new_plan(imps_context ctx, int kind, int prec);
 void upload_program(imps_plan_s* P);
 void set_operator(imps_plan_s* P, const std::vector<const double*>& parts);
-// y = op(x) for nvec vectors on the context stream
-void run(imps_plan_s* P, const double* x, double* y, int nvec);
+// y = op(x) for nvec vectors on the context stream; op_override (FP64 plans) replaces the operator
+// store by a caller buffer (two-operand products)
+void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_override = nullptr);
 
 }  // namespace imps

@@ -154,6 +154,27 @@ imps_status imps_env_solve(imps_plan plan, const double* x, const double* L, con
 imps_status imps_eigsh(imps_plan plan, imps_plan plan_fp64, double* x, double tol, int maxit,
                        int krylov, double* eval, int* iters, double* resid);
 
+/* ---- block products (A6) ------------------------------------------------------------------
+ * Two-operand products of block objects with the reference's matching rules (BlockMat operator*,
+ * BlockObj.hpp:2557-2625; MPS products, MPSBlockMat.hpp:934-1043; unity mixed TMs,
+ * TMBlockFunctions.hpp:310-333 and :444-470).  kind:
+ *   0 y[s] = M A[s]   1 y[s] = A[s] M   2 y[s] = A[s] M^T   3 y[s] = M^T A[s]
+ *   4 y = sum_s B_s^T A_s  (X1 = A, X2 = B)   5 y = sum_s A_s B_s^T  (X1 = A, X2 = B)
+ *   6 y[i*d + j] = A[i] B[j] (two-site tensor; X1 = A, X2 = B)
+ * For kinds 0/3, X1 is M; for 1/2, X2 is M.  FP64.  The output layout is derived (query with
+ * imps_plan_out_layout). */
+imps_status imps_prod_plan(imps_context ctx, imps_plan* plan, int kind, const imps_layout* X1,
+                           const imps_layout* X2);
+imps_status imps_prod_apply(imps_plan plan, const double* x1, const double* x2, double* y);
+
+/* ---- per-sector QR (A18) -------------------------------------------------------------------
+ * qr(R, MPS, dir) (MPSBlockUtilities.hpp:181-244) with qr_pos (helpers.hpp:181-195): the dense
+ * regrouped sector matrices (GetDenseMat, MPSBlockMat.hpp:468-526) are factorised in FP64 on the
+ * GPU, one workgroup per sector; Q has the layout of A, R is a BlockDiagMat (out layout).  Every
+ * sector must have at least as many stacked rows (l) / columns (r) as its size. */
+imps_status imps_qr_plan(imps_context ctx, imps_plan* plan, const imps_layout* A, int dir);
+imps_status imps_qr_apply(imps_plan plan, const double* A, double* Q, double* R);
+
 /* ---- bookkeeping (A1/A2) -------------------------------------------------------------------
  * model: 0 = XXZ spin-1/2 (one U(1)), 1 = Fermi-Hubbard in (N,M) representation (U(1)xU(1)).
  * imps_model_i2k: GetI2K (Models.cpp:262-295, :699-733); writes d*nsym charges, returns d or -1.
