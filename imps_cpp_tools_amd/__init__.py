@@ -80,6 +80,8 @@ def lib():
             "imps_env_set_mps": (I, [P, ctypes.POINTER(V)]),
             "imps_env_solve": (I, [P, V, V, V, V, V, D, I, I, ip, dp]),
             "imps_eigsh": (I, [P, P, V, D, I, I, dp, ip, dp]),
+            "imps_env_tm_apply": (I, [P, V, V]),
+            "imps_env_fixed_point": (I, [P, V, D, I, ip, dp]),
             "imps_prod_plan": (I, [P, ctypes.POINTER(P), I, LP, LP]),
             "imps_prod_apply": (I, [P, V, V, V]),
             "imps_qr_plan": (I, [P, ctypes.POINTER(P), LP, I]),
@@ -448,6 +450,16 @@ class EnvPlan(Plan):
         arr = (ctypes.c_void_p * len(tensors))(*[_ptr(t) for t in tensors])
         _check(lib().imps_env_set_mps(self.h, arr))
 
+    def tm_apply(self, x, y):
+        _check(lib().imps_env_tm_apply(self.h, _ptr(x), _ptr(y)))
+
+    def fixed_point(self, x, tol: float = 1e-14, maxit: int = 0, check: bool = True):
+        it, dl = ctypes.c_int(), ctypes.c_double()
+        st = lib().imps_env_fixed_point(self.h, _ptr(x), tol, maxit, ctypes.byref(it), ctypes.byref(dl))
+        if check:
+            _check(st)
+        return it.value, dl.value
+
     def solve(self, x, L, R, y, y0=None, tol: float = 1e-14, maxit: int = 0, krylov: int = 40, check: bool = True):
         it, rr = ctypes.c_int(), ctypes.c_double()
         st = lib().imps_env_solve(self.h, _ptr(x), _ptr(L), _ptr(R), _ptr(y0), _ptr(y), tol, maxit, krylov, ctypes.byref(it), ctypes.byref(rr))
@@ -466,7 +478,7 @@ def eigsh(plan: Plan, x, tol: float = 1e-12, maxit: int = 0, krylov: int = 20, p
     return ev.value, it.value, rr.value
 
 
-MAT_MPS, MPS_MAT, MPS_MATT, MATT_MPS, TM_LEFT_UNITY, TM_RIGHT_UNITY, MPS_MPS = range(7)
+MAT_MPS, MPS_MAT, MPS_MATT, MATT_MPS, TM_LEFT_UNITY, TM_RIGHT_UNITY, MPS_MPS, REDOP_MPS, MPS_REDOP = range(9)
 
 
 class ProdPlan(Plan):

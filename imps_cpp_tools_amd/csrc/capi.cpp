@@ -390,4 +390,25 @@ imps_status imps_qr_apply(imps_plan P, const double* A, double* Q, double* R) {
     });
 }
 
+imps_status imps_env_tm_apply(imps_plan P, const double* x, double* y) {
+    return guarded([&] {
+        check_kind(P, K_ENV);
+        if (!x || !y) throw InvalidArg("NULL vector");
+        P->env->tm_apply(x, y);
+        return IMPS_OK;
+    });
+}
+imps_status imps_env_fixed_point(imps_plan P, double* x, double tol, int maxit, int* iters, double* delta) {
+    return guarded([&] {
+        check_kind(P, K_ENV);
+        if (!x) throw InvalidArg("NULL vector");
+        double dl = 0;
+        const int it = P->env->fixed_point(x, tol, maxit <= 0 ? 100000 : maxit, &dl);
+        if (iters) *iters = it;
+        if (delta) *delta = dl;
+        if (dl >= tol) throw NotConverged("fixed point: last change " + std::to_string(dl));
+        return IMPS_OK;
+    });
+}
+
 }  // extern "C"

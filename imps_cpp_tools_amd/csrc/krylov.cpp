@@ -399,7 +399,7 @@ KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, 
     if (y0) d2d(s, y, y0, n);
     else hip_check(blas_axpby(0, nullptr, false, x, 0, y, n, s), "zero y");
     if (maxit <= 0) maxit = (int)std::min<long>(std::max<long>(20 * n, 1000), 100000);
-    if (krylov <= 0) krylov = 40;
+    if (krylov <= 0) krylov = 100;
     LinOp A64 = make_op(tm64);
     KrylovStats st;
     if (prec == IMPS_FP64) st = gmres(s, A64, rhs.d(), y, tol, maxit, krylov);
@@ -409,6 +409,64 @@ KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, 
     }
     hip_check(hipStreamSynchronize(s), "sync");
     return st;
+}
+
+void EnvSolver::tm_apply(const double* x, double* y) {
+    if (!mps_set) throw InvalidArg("env: MPS data not set");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const double* in = x;
+    long off = 0;
+    for (int k = 0; k < ncell; ++k) {
+        double* o = (k == ncell - 1) ? y : bonds.d() + off;
+        run(tm64[k].get(), in, o, 1);
+        in = o;
+        if (k < ncell - 1) off += bond_elems[k];
+    }
+}
+
+int EnvSolver::fixed_point(double* x, double tol, int maxit, double* delta) {
+    // The cell TM of an isometric chain preserves the trace (sum_s A_s^T A_s = 1 or sum_s A_s A_s^T = 1),
+    // so for x0 of unit trace the fixed point is x0 + dx with [1 - T + |x0)(1|] dx = T(x0) - x0:
+    // tracing the equation gives tr(dx) = 0 and then (1 - T) dx = T(x0) - x0.  Each round is one
+    // GMRES solve of the environment form; a second round only polishes rounding.
+    hipStream_t s = ctx->stream;
+    const long n = xlay.nelem;
+    DevBuf id, dd, r0, dx;
+    id.ensure(sizeof(double) * std::max<long>(n, 1));
+    r0.ensure(sizeof(double) * std::max<long>(n, 1));
+    dx.ensure(sizeof(double) * std::max<long>(n, 1));
+    dd.ensure(sizeof(double) * (blas_scratch_doubles(1) + 4));
+    {
+        std::vector<double> h(n, 0.0);
+        for (auto& b : xlay.l[0].b) for (int i = 0; i < std::min(b.rows, b.cols); ++i) h[b.off + i + (long)i * b.rows] = 1.0;
+        h2d(s, id.d(), h.data(), n);
+    }
+    auto dotd = [&](const double* a, const double* b) {
+        double t = 0;
+        hip_check(blas_multidot(a, 0, 1, b, n, dd.d(), dd.d() + 4, s), "dot");
+        d2h(s, &t, dd.d(), 1);
+        return t;
+    };
+    int its = 0;
+    double dl = 1;
+    for (int round = 0; round < 6; ++round) {
+        const double tr = dotd(id.d(), x);
+        if (tr == 0) throw Numerical("fixed point: iterate has zero trace");
+        hip_check(blas_axpby(1.0 / tr, nullptr, false, x, 0.0, x, n, s), "normalise");
+        tm_apply(x, r0.d());
+        ++its;
+        hip_check(blas_axpby(-1.0, nullptr, false, x, 1.0, r0.d(), n, s), "T(x) - x");
+        dl = std::sqrt(std::max(dotd(r0.d(), r0.d()), 0.0));
+        if (dl < tol || its >= maxit) break;
+        // l chain: rho_in = R slot, rho_out = L slot; r chain: rho_in = L, rho_out = R
+        const double* Lp = dir == IMPS_LEFT ? x : id.d();
+        const double* Rp = dir == IMPS_LEFT ? id.d() : x;
+        KrylovStats st = solve(r0.d(), Lp, Rp, nullptr, dx.d(), std::max(0.1 * tol / std::max(dl, 1e-300), 1e-13), maxit - its, 100);
+        its += st.iters;
+        hip_check(blas_axpby(1.0, nullptr, false, dx.d(), 1.0, x, n, s), "update");
+    }
+    if (delta) *delta = dl;
+    return its;
 }
 
 }  // namespace imps

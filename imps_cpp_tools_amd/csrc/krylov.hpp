@@ -53,6 +53,12 @@ struct EnvSolver {
     void build(imps_context c, int direction, const std::vector<Layout>& A, const Layout& x, int precision);
     void set_mps(const std::vector<const double*>& A);
     KrylovStats solve(const double* x, const double* L, const double* R, const double* y0, double* y, double tol, int maxit, int krylov);
+    // y = T(x), the unit-cell transfer matrix of the chain (FP64)
+    void tm_apply(const double* x, double* y);
+    // dominant fixed point of the trace-preserving cell TM from the start x (overwritten), trace
+    // normalised (TMDominantEig, EigsBlockFunctions.hpp:91-172), by a correction solve of the
+    // environment form; returns operator applications, *delta = ||T(x) - x|| of the result
+    int fixed_point(double* x, double tol, int maxit, double* delta);
 };
 
 }  // namespace imps

@@ -379,7 +379,9 @@ enum ProdKind : int {
     PROD_MATT_MPS = 3,   // y[s] = M^T * A[s]
     PROD_TM_LEFT = 4,    // y = sum_s B_s^T A_s, keyed by Qout(B_s) (ApplyTMmixedLeft(A,B) unity)
     PROD_TM_RIGHT = 5,   // y = sum_s A_s B_s^T, keyed by Qin(A_s)  (ApplyTMmixedRight(A,B) unity)
-    PROD_MPS_MPS = 6     // y[i*d2 + j] = A[i] * B[j] (two-site concatenation)
+    PROD_MPS_MPS = 6,    // y[i*d2 + j] = A[i] * B[j] (two-site concatenation)
+    PROD_REDOP_MPS = 7,  // y[i] = sum_j R[i][j] * A[j]   (X1 = R: d*d lists, X2 = A)
+    PROD_MPS_REDOP = 8   // y[i] = sum_j A[j] * R[i][j]   (X1 = R, X2 = A)
 };
 
 // Per-entry product lhs * rhs of two BlockMats with the BlockMat operator* matching (lhs Qout =
@@ -482,6 +484,47 @@ inline Program build_prod(int kind, const Layout& X1, const Layout& X2, Layout& 
                 pos += b.size();
                 out.l[0].b.push_back(b);
                 obs.push_back(std::move(ob));
+            }
+            break;
+        }
+        case PROD_REDOP_MPS:
+        case PROD_MPS_REDOP: {
+            const size_t d = X2.l.size();
+            if (X1.l.size() != d * d) throw InvalidArg("prod: R must have nlist = d*d");
+            out.l.resize(d);
+            for (size_t i = 0; i < d; ++i) {
+                std::map<Key, std::pair<Blk, OutBlock>> acc;
+                for (size_t j = 0; j < d; ++j) {
+                    const BList& R = X1.l[i * d + j];
+                    if (kind == PROD_REDOP_MPS) {  // R block (q -> p) times A[j] block (p -> p')
+                        for (const Blk& h : R.b) {
+                            const Blk* a = X2.l[j].find(h.qout);
+                            if (!a) continue;
+                            auto it = acc.find(h.qin);
+                            if (it == acc.end()) it = acc.emplace(h.qin, std::make_pair(Blk{h.qin, a->qout, h.rows, a->cols, 0}, OutBlock{0, h.rows, a->cols, {}})).first;
+                            else if (it->second.first.qout != a->qout) throw InvalidArg("prod: inconsistent charge flow");
+                            add_seg(it->second.second, opnd(SLOT_X, h, 0), opnd(SLOT_OP, *a, 0), 1.0, P.flops);
+                        }
+                    } else {  // A[j] block (q -> p) times R block (p -> p')
+                        for (const Blk& a : X2.l[j].b) {
+                            const Blk* h = R.find(a.qout);
+                            if (!h) continue;
+                            auto it = acc.find(a.qin);
+                            if (it == acc.end()) it = acc.emplace(a.qin, std::make_pair(Blk{a.qin, h->qout, a.rows, h->cols, 0}, OutBlock{0, a.rows, h->cols, {}})).first;
+                            else if (it->second.first.qout != h->qout) throw InvalidArg("prod: inconsistent charge flow");
+                            add_seg(it->second.second, opnd(SLOT_OP, a, 0), opnd(SLOT_X, *h, 0), 1.0, P.flops);
+                        }
+                    }
+                }
+                for (auto& kv : acc) {
+                    Blk b = kv.second.first;
+                    b.off = pos;
+                    OutBlock ob = std::move(kv.second.second);
+                    ob.c_off = pos;
+                    pos += b.size();
+                    out.l[i].b.push_back(b);
+                    obs.push_back(std::move(ob));
+                }
             }
             break;
         }

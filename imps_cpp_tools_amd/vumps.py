@@ -1,0 +1,319 @@
+"""VUMPS ground-state iteration for an N-site unit cell at fixed bond structure.
+
+A thin driver over the library's GPU operations: every contraction, eigensolve, environment solve,
+QR and fixed point runs in the HIP library through the C ABI; this module only sequences them and
+keeps the device buffers.  The update order follows the outer loop of VUMPS_main.cpp:165-354
+(sequential sweep over the unit cell):
+
+  site n (left bond n, right bond n+1; C[n] lives on bond n+1):
+    F(n)  = gradient norm at AC = C[n-1] AR[n]          (GradNormRight, VUMPS_helpers.hpp:147-153)
+    AC    = lowest eigenvector of H_AC                   (eigs_rs "SA", :186)
+    CL,CR = lowest eigenvectors of H_C on bonds n, n+1   (:199-202, sign fixed by the first entry)
+    AL[n] = Q(AC,l) Q(CR,l)^T,  AR[n] = Q(CL,r)^T Q(AC,r) (QR polar scheme, :225-232)
+    environments: LE[n] by the left geometric sum with R = fixed point of the cell TM (trueLR),
+    LE[n+1], LE[n+2] by TM steps, RE[n+2] by the right geometric sum, RE[n+1] by a TM step
+    (:247-305).
+Environments: LE[b] holds the two-site terms left of bond b that do not touch site b
+(InvertE_proj on GetHL, VUMPS_helpers.hpp:3-18); RE[b] the terms right of bond b - 1 on sites >= b
+(GetHR, :20-36).  H_AC at site n uses LE[n], RE[n+1] and the reduced operators of AL[n-1],
+AR[n+1]; H_C on bond b uses LE[b], RE[b], AL[b-1], AR[b].
+
+Energy per site = tr(HL_b R_b) / N with HL_b the one-cell source of the left geometric sum.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import (BF16X3, FP32, FP64, LEFT, RIGHT, MAT_MPS, MATT_MPS, MPS_MAT, MPS_MATT, MPS_REDOP, REDOP_MPS, TM_LEFT_UNITY,
+               TM_RIGHT_UNITY, BlockLayout, Context, EnvPlan, HACPlan, HCPlan, ProdPlan, QRPlan, RedOpPlan, TMPlan, eigsh,
+               make_dims, model_d, model_ham, model_nsym, mps_structure)
+
+
+@dataclass
+class VumpsStats:
+    iteration: int
+    energy: float
+    F: List[float]
+    gradnorm: List[float]
+    seconds: float
+    eig_iters: int = 0
+    env_iters: int = 0
+
+
+class VUMPS:
+    def __init__(self, ctx: Context, model: int, N: int, QN, keys0, sizes0, params, prec: int = FP64, seed: int = 0,
+                 eig_tol_max: float = 1e-6, eig_tol_min: float = 5e-16, env_tol: float = 5e-15, env_fac: float = 2.0,
+                 krylov: int = 20):
+        import torch
+        self.torch = torch
+        self.ctx, self.prec, self.N = ctx, prec, N
+        self.d, self.nsym = model_d(model), model_nsym(model)
+        self.bonds = make_dims(model, N, QN, keys0, sizes0)
+        self.blay = [BlockLayout.diag(self.nsym, k, s) for (k, s) in self.bonds]
+        self.slay = [mps_structure(model, N, QN, self.bonds[n], self.bonds[(n + 1) % N]) for n in range(N)]
+        self.H = model_ham(model, params)
+        self.eig_tol_max, self.eig_tol_min, self.env_tol, self.env_fac, self.krylov = eig_tol_max, eig_tol_min, env_tol, env_fac, krylov
+        self._plans: Dict = {}
+        rng = np.random.default_rng(seed)
+        self.AL, self.AR, self.C = [], [], []
+        for n in range(N):
+            A = self._dev(rng.standard_normal(self.slay[n].nelem))
+            self.AL.append(self._qr(A, self.slay[n], LEFT)[0])
+            A = self._dev(rng.standard_normal(self.slay[n].nelem))
+            self.AR.append(self._qr(A, self.slay[n], RIGHT)[0])
+        for n in range(N):  # C[n] on bond n+1: identity / sqrt(D), so that tr(C C^T) = 1
+            self.C.append(self._identity((n + 1) % N, 1.0 / np.sqrt(self.bonds[(n + 1) % N][1].sum())))
+        self.prec_n = np.ones(N)
+        self.LE: Dict[int, object] = {}
+        self.RE: Dict[int, object] = {}
+        self.eig_iters = 0
+        self.env_iters = 0
+        self._init_environments()
+
+    # ------------------------------------------------------------------ helpers
+    def _dev(self, a):
+        return self.torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+    def _empty(self, n):
+        return self.torch.empty(n, dtype=self.torch.float64, device="cuda")
+
+    def _b(self, b):
+        return b % self.N
+
+    def _identity(self, bond, scale=1.0):
+        lay = self.blay[bond]
+        out = np.zeros(lay.nelem)
+        for o, s in zip(lay.offsets, lay.rows):
+            out[o:o + s * s].reshape(s, s)[np.diag_indices(s)] = scale
+        return self._dev(out)
+
+    def _plan(self, key, make):
+        p = self._plans.get(key)
+        if p is None:
+            p = make()
+            self._plans[key] = p
+        return p
+
+    def _qr(self, A, lay, direction):
+        p = self._plan(("qr", id(lay), direction), lambda: QRPlan(self.ctx, lay, direction))
+        Q = self._empty(lay.nelem)
+        R = self._empty(p.out_layout.nelem)
+        p.apply(A, Q, R)
+        return Q, R
+
+    def _prod(self, kind, l1, x1, l2, x2, key):
+        p = self._plan(("prod", kind, key), lambda: ProdPlan(self.ctx, kind, l1, l2))
+        y = self._empty(p.out_layout.nelem)
+        p.apply(x1, x2, y)
+        return y, p.out_layout
+
+    def _redop(self, A, n, direction):
+        p = self._plan(("redop", n, direction), lambda: RedOpPlan(self.ctx, self.H, self.d, self.slay[n], direction))
+        y = self._empty(p.out_layout.nelem)
+        p.apply(A, y)
+        return y, p.out_layout
+
+    def _tm(self, direction, A, n, x, bin_, bout):
+        p = self._plan(("tm", direction, n), lambda: TMPlan(self.ctx, direction, self.slay[n], None, self.blay[bin_], self.blay[bout]))
+        p.set_mps(A)
+        y = self._empty(self.blay[bout].nelem)
+        p.apply(x, y)
+        return y
+
+    def _optm_left(self, a):
+        """ApplyOpTMLeftDiag(H, AL[a] AL[a+1]) on bond a+2 (TMBlockFunctions.hpp:800-806)."""
+        a = self._b(a)
+        b = self._b(a + 1)
+        HL, HLl = self._redop(self.AL[self._b(a)], self._b(a), LEFT)
+        W, Wl = self._prod(REDOP_MPS, HLl, HL, self.slay[b], self.AL[b], ("optmL", a))
+        y, _ = self._prod(TM_LEFT_UNITY, Wl, W, self.slay[b], self.AL[b], ("optmL2", a))
+        return y
+
+    def _optm_right(self, a):
+        """ApplyOpTMRightDiag(H, AR[a] AR[a+1]) on bond a (TMBlockFunctions.hpp:842-848)."""
+        a = self._b(a)
+        HR, HRl = self._redop(self.AR[self._b(a + 1)], self._b(a + 1), RIGHT)
+        W, Wl = self._prod(MPS_REDOP, HRl, HR, self.slay[a], self.AR[a], ("optmR", a))
+        y, _ = self._prod(TM_RIGHT_UNITY, Wl, W, self.slay[a], self.AR[a], ("optmR2", a))
+        return y
+
+    def _get_hl(self, b):
+        """GetHL for the cell starting at site b (VUMPS_helpers.hpp:3-18), on bond b."""
+        N = self.N
+        h = self._optm_left(b - 1)  # term (b-1, b) on bond b+1
+        for k in range(1, N):
+            s = self._b(b + k)
+            h = self._tm(LEFT, self.AL[s], s, h, s, self._b(s + 1)) + self._optm_left(b + k - 1)
+        return h
+
+    def _get_hr(self, b):
+        """GetHR for the cell starting at site b (VUMPS_helpers.hpp:20-36), on bond b."""
+        N = self.N
+        h = self._optm_right(b + N - 1)  # term (b+N-1, b+N) on bond b+N-1
+        for k in range(N - 2, -1, -1):
+            s = self._b(b + k)
+            h = self._tm(RIGHT, self.AR[s], s, h, self._b(s + 1), s) + self._optm_right(b + k)
+        return h
+
+    def _env_plan(self, direction, b, As_layouts_key):
+        N = self.N
+        sites = [self._b(b + k) for k in range(N)]
+        return self._plan(("env", direction, b), lambda: EnvPlan(self.ctx, direction, [self.slay[s] for s in sites], self.blay[b], self.prec))
+
+    def _left_env(self, b, tol):
+        """LE[b] = InvertE_proj(AL cell from b, GetHL, 1, R, l) with R the cell fixed point (trueLR)."""
+        b = self._b(b)
+        sites = [self._b(b + k) for k in range(self.N)]
+        p = self._env_plan(LEFT, b, None)
+        p.set_mps([self.AL[s] for s in sites])
+        Cb = self.C[self._b(b - 1)]
+        R, _ = self._prod(MPS_MATT, self.blay[b], Cb, self.blay[b], Cb, ("CCt", b))  # C C^T
+        # fixed point of T_R over the cell: the right-direction chain of the same AL tensors
+        pr = self._plan(("envR_AL", b), lambda: EnvPlan(self.ctx, RIGHT, [self.slay[s] for s in sites], self.blay[b], FP64))
+        pr.set_mps([self.AL[s] for s in sites])
+        pr.fixed_point(R, tol=1e-14, maxit=5000, check=False)
+        HAL = self._get_hl(b)
+        one = self._identity(b)
+        y = self._empty(self.blay[b].nelem)
+        y0 = self.LE.get(b)
+        it, rr = p.solve(HAL, one, R, y, y0=y0, tol=tol, maxit=5000, krylov=100, check=False)
+        self.env_iters += it
+        e_cell = float(self.torch.dot(HAL, self._transpose_blocks(R, b)).item())
+        return y, e_cell
+
+    def _right_env(self, b, tol):
+        b = self._b(b)
+        sites = [self._b(b + k) for k in range(self.N)]
+        p = self._env_plan(RIGHT, b, None)
+        p.set_mps([self.AR[s] for s in sites])
+        Cb = self.C[self._b(b - 1)]
+        L, _ = self._prod(MATT_MPS, self.blay[b], Cb, self.blay[b], Cb, ("CtC", b))  # C^T C
+        pl = self._plan(("envL_AR", b), lambda: EnvPlan(self.ctx, LEFT, [self.slay[s] for s in sites], self.blay[b], FP64))
+        pl.set_mps([self.AR[s] for s in sites])
+        pl.fixed_point(L, tol=1e-14, maxit=5000, check=False)
+        HAR = self._get_hr(b)
+        one = self._identity(b)
+        y = self._empty(self.blay[b].nelem)
+        it, rr = p.solve(HAR, L, one, y, y0=self.RE.get(b), tol=tol, maxit=5000, krylov=100, check=False)
+        self.env_iters += it
+        return y
+
+    def _transpose_blocks(self, x, b):
+        lay = self.blay[b]
+        t = self.torch.empty_like(x)
+        for o, s in zip(lay.offsets, lay.rows):
+            t[o:o + s * s] = x[o:o + s * s].view(s, s).t().reshape(-1)
+        return t
+
+    def _init_environments(self):
+        N = self.N
+        for b in range(N):
+            self.LE[b], _ = self._left_env(b, self.env_tol)
+            self.RE[b] = self._right_env(b, self.env_tol)
+
+    # ------------------------------------------------------------------ operators
+    def _hac_plans(self, n):
+        N = self.N
+        left, right = self._b(n - 1), self._b(n + 1)
+        HL, HLl = self._redop(self.AL[left], left, LEFT)
+        HR, HRl = self._redop(self.AR[right], right, RIGHT)
+        mk = lambda prec: HACPlan(self.ctx, self.slay[n], self.blay[n], self.blay[self._b(n + 1)], HLl, HRl, prec)  # noqa: E731
+        p = self._plan(("hac", n, self.prec), lambda: mk(self.prec))
+        ops = (self.LE[n], self.RE[self._b(n + 1)], HL, HR)
+        p.set_operator(*ops)
+        p64 = None
+        if self.prec != FP64:
+            p64 = self._plan(("hac", n, FP64), lambda: mk(FP64))
+            p64.set_operator(*ops)
+        return p, p64
+
+    def _hc_plans(self, b):
+        b = self._b(b)
+        a = self._b(b - 1)
+        HL, HLl = self._redop(self.AL[a], a, LEFT)
+        mk = lambda prec: HCPlan(self.ctx, self.blay[b], self.blay[b], self.blay[b], HLl, self.slay[b], prec)  # noqa: E731
+        p = self._plan(("hc", b, self.prec), lambda: mk(self.prec))
+        ops = (self.LE[b], self.RE[b], HL, self.AR[b])
+        p.set_operator(*ops)
+        p64 = None
+        if self.prec != FP64:
+            p64 = self._plan(("hc", b, FP64), lambda: mk(FP64))
+            p64.set_operator(*ops)
+        return p, p64
+
+    def gradnorm(self, n, AC, hac):
+        """GradNormRight: norm_scaled(ApplyTMmixedRight(H_AC(C AR), NR)) = ||HAC - (sum_s HAC_s AR_s^T) AR|| / sqrt(nsec),
+        nsec = number of left sectors of AR[n] with a null space; returns (F, ||B||)."""
+        lay = self.slay[n]
+        HAC = self._empty(lay.nelem)
+        hac.apply(AC, HAC)
+        X, Xl = self._prod(TM_RIGHT_UNITY, lay, HAC, lay, self.AR[n], ("gX", n))
+        Y, _ = self._prod(MAT_MPS, Xl, X, lay, self.AR[n], ("gY", n))
+        B = float(self.torch.linalg.norm(HAC - Y).item())
+        cols, rows = {}, {}
+        for qi, r, c in zip(map(tuple, lay.qin), lay.rows, lay.cols):
+            cols[qi] = cols.get(qi, 0) + int(c)
+            rows[qi] = int(r)
+        nsec = sum(1 for q in cols if cols[q] > rows[q])
+        return (B / np.sqrt(nsec) if nsec else 0.0), B
+
+    # ------------------------------------------------------------------ one outer iteration
+    def sweep(self) -> VumpsStats:
+        torch = self.torch
+        N = self.N
+        t0 = time.perf_counter()
+        F, G = np.zeros(N), np.zeros(N)
+        e_cell = 0.0
+        it0, env0 = self.eig_iters, self.env_iters
+        for n in range(N):
+            left = self._b(n - 1)
+            tol = min(max(self.prec_n[n] / 100.0, self.eig_tol_min), self.eig_tol_max)
+            hac, hac64 = self._hac_plans(n)
+            AC, _ = self._prod(MAT_MPS, self.blay[n], self.C[left], self.slay[n], self.AR[n], ("CAR", n))
+            F[n], G[n] = self.gradnorm(n, AC, hac64 or hac)
+            _, it, _ = eigsh(hac, AC, tol=tol, krylov=self.krylov, plan_fp64=hac64, check=False)
+            self.eig_iters += it
+            hcl, hcl64 = self._hc_plans(n)
+            CL = self.C[left].clone()
+            _, it, _ = eigsh(hcl, CL, tol=tol, krylov=self.krylov, plan_fp64=hcl64, check=False)
+            self.eig_iters += it
+            hcr, hcr64 = self._hc_plans(n + 1)
+            CR = self.C[n].clone()
+            _, it, _ = eigsh(hcr, CR, tol=tol, krylov=self.krylov, plan_fp64=hcr64, check=False)
+            self.eig_iters += it
+            CL = CL if float(CL[0].item()) >= 0 else -CL  # sign convention of VUMPS_main.cpp:201-202
+            CR = CR if float(CR[0].item()) >= 0 else -CR
+            QLa, _ = self._qr(AC, self.slay[n], LEFT)
+            QLc, _ = self._qr(CR, self.blay[self._b(n + 1)], LEFT)
+            QRa, _ = self._qr(AC, self.slay[n], RIGHT)
+            QRc, _ = self._qr(CL, self.blay[n], RIGHT)
+            ALn, _ = self._prod(MPS_MATT, self.slay[n], QLa, self.blay[self._b(n + 1)], QLc, ("ALnew", n))
+            ARn, _ = self._prod(MATT_MPS, self.blay[n], QRc, self.slay[n], QRa, ("ARnew", n))
+            # precision of the gauge: max(||AC - AL CR||_inf, ||AC - CL AR||_inf)
+            ALCR, _ = self._prod(MPS_MAT, self.slay[n], ALn, self.blay[self._b(n + 1)], CR, ("ALCR", n))
+            CLAR, _ = self._prod(MAT_MPS, self.blay[n], CL, self.slay[n], ARn, ("CLAR", n))
+            self.prec_n[n] = max(float((AC - ALCR).abs().max().item()), float((AC - CLAR).abs().max().item()))
+            self.AL[n], self.AR[n] = ALn, ARn
+            self.C[left], self.C[n] = CL, CR
+            # environments (VUMPS_main.cpp:247-305)
+            etol = max(tol / self.env_fac, self.env_tol)
+            self.LE[n], e_cell = self._left_env(n, etol)
+            self.LE[self._b(n + 1)] = self._tm(LEFT, self.AL[n], n, self.LE[n], n, self._b(n + 1)) + self._optm_left(n - 1)
+            self.LE[self._b(n + 2)] = (self._tm(LEFT, self.AL[self._b(n + 1)], self._b(n + 1), self.LE[self._b(n + 1)], self._b(n + 1),
+                                                self._b(n + 2)) + self._optm_left(n))
+            self.RE[self._b(n + 2)] = self._right_env(n + 2, etol)
+            self.RE[self._b(n + 1)] = (self._tm(RIGHT, self.AR[self._b(n + 1)], self._b(n + 1), self.RE[self._b(n + 2)], self._b(n + 2),
+                                                self._b(n + 1)) + self._optm_right(n + 1))
+        torch.cuda.synchronize()
+        self.iteration = getattr(self, "iteration", 0) + 1
+        return VumpsStats(self.iteration, e_cell / N, list(F), list(G), time.perf_counter() - t0, self.eig_iters - it0,
+                          self.env_iters - env0)
+
+    def energy(self) -> float:
+        b = 0
+        _, e_cell = self._left_env(b, self.env_tol)
+        return e_cell / self.N

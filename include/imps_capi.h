@@ -144,6 +144,14 @@ imps_status imps_env_set_mps(imps_plan plan, const double* const* A);
 imps_status imps_env_solve(imps_plan plan, const double* x, const double* L, const double* R,
                            const double* y0, double* y, double tol, int maxit, int krylov,
                            int* iters, double* relres);
+/* y = T(x) with the plan's unit-cell chain (ApplyTMLeft/Right on an MPSBlockMatArray,
+ * TMBlockFunctions.hpp:555-615), FP64. */
+imps_status imps_env_tm_apply(imps_plan plan, const double* x, double* y);
+/* Dominant fixed point of the unit-cell TM of isometric tensors (TMDominantEig,
+ * EigsBlockFunctions.hpp:91-172, trace-normalised as at :162), from the start x: the TM preserves
+ * the trace, so with x0 of unit trace the fixed point is x0 + dx, [1 - T + |x0)(1|] dx = T(x0) - x0,
+ * solved by the environment GMRES; until ||T(x) - x|| < tol (IMPS_NOT_CONVERGED otherwise). */
+imps_status imps_env_fixed_point(imps_plan plan, double* x, double tol, int maxit, int* iters, double* delta);
 
 /* ---- lowest eigenpair (A17) ----------------------------------------------------------------
  * Replaces eigs_rs(MultOPx, N, vals, vecs, 1, "SA", tol, x0) (eigs.h:43-44; eigs.cpp:369-494) for
@@ -161,6 +169,7 @@ imps_status imps_eigsh(imps_plan plan, imps_plan plan_fp64, double* x, double to
  *   0 y[s] = M A[s]   1 y[s] = A[s] M   2 y[s] = A[s] M^T   3 y[s] = M^T A[s]
  *   4 y = sum_s B_s^T A_s  (X1 = A, X2 = B)   5 y = sum_s A_s B_s^T  (X1 = A, X2 = B)
  *   6 y[i*d + j] = A[i] B[j] (two-site tensor; X1 = A, X2 = B)
+ *   7 y[i] = sum_j R[i][j] A[j]   8 y[i] = sum_j A[j] R[i][j]   (X1 = R, a RedOp; X2 = A)
  * For kinds 0/3, X1 is M; for 1/2, X2 is M.  FP64.  The output layout is derived (query with
  * imps_plan_out_layout). */
 imps_status imps_prod_plan(imps_context ctx, imps_plan* plan, int kind, const imps_layout* X1,

@@ -81,9 +81,66 @@ def test_products(ctx, name):
         (im.TM_RIGHT_UNITY, A, B, orc.tm_unity(3, A, B)),
         (im.MPS_MPS, c.AL[0], c.AL[1], orc.mps_mul(c.AL[0], c.AL[1])),
     ]
+    # reduced-operator products: y[i] = sum_j HL[i][j] A[j] and y[i] = sum_j A[j] HR[i][j]
+    H = orc.model_ham(c.model, [1.0, -1.0, 0.2] if c.model == XXZ else [1.0, 4.0, 0.0, 0.0, 0.0])
+    HL = orc.reduced_op(H, c.d, c.AL[1], 0)
+    HR = orc.reduced_op(H, c.d, c.AR[1], 1)
+    d = c.d
+
+    cases.append((im.REDOP_MPS, HL, A, _redop_mps(HL, A, d)))
+    cases.append((im.MPS_REDOP, HR, A, _mps_redop(HR, A, d)))
     for kind, x1, x2, ref in cases:
         p = im.ProdPlan(ctx, kind, lay(x1), lay(x2))
         assert p.out_layout == lay(ref), kind
         y = torch.full((ref.nelem,), float("nan"), dtype=torch.float64, device="cuda")
         p.apply(dev(x1.data), dev(x2.data), y)
         assert relerr(y.cpu().numpy(), ref.data) < 1e-13, kind
+
+
+def _entry(ob, e):
+    s = int(ob.nblk[:e].sum())
+    n = int(ob.nblk[e])
+    sizes = ob.rows.astype(np.int64) * ob.cols
+    a = int(sizes[:s].sum())
+    b = a + int(sizes[s:s + n].sum())
+    return ob.qin[s:s + n], ob.qout[s:s + n], ob.rows[s:s + n], ob.cols[s:s + n], ob.data[a:b]
+
+
+def _single(ob, e):
+    qi, qo, r, c, dat = _entry(ob, e)
+    return orc.OB(ob.nsym, np.array([len(r)], np.int32), qi, qo, r, c, dat.copy())
+
+
+def _sum_entries(parts, nsym):
+    """Union-sum of BlockMats (BlockMat::operator+=) via the oracle's products with identities is
+    overkill; here the parts already share the block structure of the first non-empty one or are
+    disjoint, so a dict keyed by Qin accumulates them."""
+    acc = {}
+    for ob in parts:
+        for _, qi, qo, m in ob.blocks():
+            if qi in acc:
+                acc[qi] = (qo, acc[qi][1] + m)
+            else:
+                acc[qi] = (qo, m.copy())
+    keys = sorted(acc)
+    if not keys:
+        return orc.OB(nsym, np.array([0], np.int32), np.zeros((0, nsym), np.int32), np.zeros((0, nsym), np.int32),
+                      np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    return orc.OB(nsym, np.array([len(keys)], np.int32), np.array(keys, np.int32), np.array([acc[k][0] for k in keys], np.int32),
+                  np.array([acc[k][1].shape[0] for k in keys], np.int32), np.array([acc[k][1].shape[1] for k in keys], np.int32),
+                  np.concatenate([acc[k][1].reshape(-1, order="F") for k in keys]))
+
+
+def _concat(obs):
+    nsym = obs[0].nsym
+    return orc.OB(nsym, np.concatenate([o.nblk for o in obs]), np.concatenate([o.qin.reshape(-1, nsym) for o in obs]),
+                  np.concatenate([o.qout.reshape(-1, nsym) for o in obs]), np.concatenate([o.rows for o in obs]),
+                  np.concatenate([o.cols for o in obs]), np.concatenate([o.data for o in obs]))
+
+
+def _redop_mps(R, A, d):
+    return _concat([_sum_entries([orc.bmul(_single(R, i * d + j), _single(A, j)) for j in range(d)], A.nsym) for i in range(d)])
+
+
+def _mps_redop(R, A, d):
+    return _concat([_sum_entries([orc.bmul(_single(A, j), _single(R, i * d + j)) for j in range(d)], A.nsym) for i in range(d)])
