@@ -1,0 +1,166 @@
+"""FP64 CPU VUMPS with the oracle's block operations (test infrastructure).
+
+Restates the outer loop of VUMPS_main.cpp:165-354 at fixed bond structure with the oracle's
+H_AC / H_C (VUMPS_helpers.hpp:38-133, H_C as the reference's per-nonzero five-factor chain),
+InvertE_proj + unrestarted GMRES (InvEBlockFunctions.hpp:205-266, IterativeSolvers.hpp:21-149),
+GetHL / GetHR, qr with qr_pos, and ARPACK (scipy's black-box library routine) for the "SA"
+eigenpairs (eigs_rs) and the TM dominant eigenvector (TMDominantEig).  The initial state is built
+from the same seeded random arrays as imps_cpp_tools_amd.vumps.VUMPS, so both start identically.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.sparse.linalg import LinearOperator, eigs, eigsh
+
+from oracle import pyoracle as orc
+from oracle.pyoracle import OB
+
+
+def _like(ob: OB, data) -> OB:
+    return OB(ob.nsym, ob.nblk, ob.qin, ob.qout, ob.rows, ob.cols, np.ascontiguousarray(data, np.float64))
+
+
+def _add(a: OB, b: OB) -> OB:
+    """Sum of two BlockDiagMats with identical structure."""
+    assert np.array_equal(a.qin, b.qin) and np.array_equal(a.rows, b.rows)
+    return _like(a, a.data + b.data)
+
+
+def _transpose_data(ob: OB) -> np.ndarray:
+    parts = []
+    for _, _, _, m in ob.blocks():
+        parts.append(m.T.reshape(-1, order="F"))
+    return np.concatenate(parts)
+
+
+class OracleVUMPS:
+    def __init__(self, model, N, QN, keys0, sizes0, params, seed=0, eig_tol_max=1e-6, eig_tol_min=5e-16, env_tol=5e-15, env_fac=2.0):
+        self.N = N
+        self.d = 2 if model == 0 else 4
+        self.nsym = 1 if model == 0 else 2
+        self.bonds = orc.make_dims(model, N, QN, keys0, sizes0)
+        self.struct = [orc.mps_structure(model, N, QN, self.bonds[n], self.bonds[(n + 1) % N]) for n in range(N)]
+        self.H = orc.model_ham(model, params)
+        self.eig_tol_max, self.eig_tol_min, self.env_tol, self.env_fac = eig_tol_max, eig_tol_min, env_tol, env_fac
+        rng = np.random.default_rng(seed)
+        self.AL, self.AR, self.C = [], [], []
+        for n in range(N):
+            self.AL.append(orc.mps_qr(_like(self.struct[n], rng.standard_normal(self.struct[n].nelem)), 0)[0])
+            self.AR.append(orc.mps_qr(_like(self.struct[n], rng.standard_normal(self.struct[n].nelem)), 1)[0])
+        for n in range(N):
+            b = (n + 1) % N
+            self.C.append(self.identity(b, 1.0 / np.sqrt(self.bonds[b][1].sum())))
+        self.prec = np.ones(N)
+        self.LE, self.RE = {}, {}
+        for b in range(N):
+            self.LE[b], _ = self.left_env(b, env_tol)
+            self.RE[b] = self.right_env(b, env_tol)
+
+    def _b(self, b):
+        return b % self.N
+
+    def diag(self, b, data=None):
+        k, s = self.bonds[self._b(b)]
+        k = np.asarray(k, np.int32).reshape(-1, self.nsym)
+        s = np.asarray(s, np.int32)
+        n = int((s.astype(np.int64) ** 2).sum())
+        return OB(self.nsym, np.array([len(s)], np.int32), k, k, s, s, np.zeros(n) if data is None else np.asarray(data, np.float64))
+
+    def identity(self, b, scale=1.0):
+        x = self.diag(b)
+        for _, _, _, m in x.blocks():
+            m[...] = scale * np.eye(m.shape[0])
+        return x
+
+    def cell(self, As, b):
+        return [As[self._b(b + k)] for k in range(self.N)]
+
+    def fixed_point(self, direction, As, b, x0: OB) -> OB:
+        """Dominant eigenvector of the cell TM by ARPACK (TMDominantEig), trace-normalised."""
+        lay = self.diag(b)
+        n = lay.nelem
+
+        def mv(v):
+            return orc.tm_chain(direction, As, _like(lay, np.real(v)), lay).data
+
+        w, V = eigs(LinearOperator((n, n), matvec=mv, dtype=np.float64), k=1, which="LM", v0=x0.data, tol=1e-15,
+                    ncv=min(n, 20))
+        v = np.real(V[:, 0])
+        x = _like(lay, v)
+        tr = sum(np.trace(m) for _, _, _, m in x.blocks())
+        x.data = x.data / tr
+        return x
+
+    def left_env(self, b, tol):
+        b = self._b(b)
+        As = self.cell(self.AL, b)
+        Cb = self.C[self._b(b - 1)]
+        R0 = orc.bmul(Cb, orc.btrans(Cb))
+        R = self.fixed_point(1, As, b, R0)
+        HAL = orc.get_h(0, As, self.H, self.d)
+        y0 = self.LE.get(b)
+        y, flag = orc.invert_e_proj(0, As, HAL, self.identity(b), R, tol=tol, y0=y0)
+        e_cell = float(HAL.data @ _transpose_data(R))
+        return y, e_cell
+
+    def right_env(self, b, tol):
+        b = self._b(b)
+        As = self.cell(self.AR, b)
+        Cb = self.C[self._b(b - 1)]
+        L0 = orc.bmul(orc.btrans(Cb), Cb)
+        L = self.fixed_point(0, As, b, L0)
+        HAR = orc.get_h(1, As, self.H, self.d)
+        y, flag = orc.invert_e_proj(1, As, HAR, L, self.identity(b), tol=tol, y0=self.RE.get(b))
+        return y
+
+    def optm_left(self, a):
+        a = self._b(a)
+        return orc.op_tm_diag(0, self.H, self.d, orc.mps_mul(self.AL[a], self.AL[self._b(a + 1)]))
+
+    def optm_right(self, a):
+        a = self._b(a)
+        return orc.op_tm_diag(1, self.H, self.d, orc.mps_mul(self.AR[a], self.AR[self._b(a + 1)]))
+
+    def _lowest(self, apply, x0: OB, tol):
+        n = x0.nelem
+        op = LinearOperator((n, n), matvec=lambda v: apply(_like(x0, np.real(v))).data, dtype=np.float64)
+        w, V = eigsh(op, k=1, which="SA", v0=x0.data, tol=tol, ncv=min(n, 20))
+        return w[0], _like(x0, V[:, 0])
+
+    def sweep(self):
+        N, H, d = self.N, self.H, self.d
+        F = np.zeros(N)
+        e_cell = 0.0
+        for n in range(N):
+            left, right = self._b(n - 1), self._b(n + 1)
+            tol = min(max(self.prec[n] / 100.0, self.eig_tol_min), self.eig_tol_max)
+            HL = orc.reduced_op(H, d, self.AL[left], 0)
+            HR = orc.reduced_op(H, d, self.AR[right], 1)
+            EHL, EHR = self.LE[n], self.RE[right]
+            F[n] = orc.gradnorm(1, self.AR[n], self.C[left], EHL, EHR, HL, HR)
+            AC0 = orc.mat_mps(self.C[left], self.AR[n])
+            _, AC = self._lowest(lambda x: orc.hac(EHL, EHR, HL, HR, x), AC0, tol)
+            _, CL = self._lowest(lambda x: orc.hc(self.LE[n], self.RE[n], H, d, self.AL[left], self.AR[n], x), self.C[left], tol)
+            _, CR = self._lowest(lambda x: orc.hc(self.LE[right], self.RE[right], H, d, self.AL[n], self.AR[right], x), self.C[n], tol)
+            if CL.data[0] < 0:
+                CL.data = -CL.data
+            if CR.data[0] < 0:
+                CR.data = -CR.data
+            QLa, _ = orc.mps_qr(AC, 0)
+            QLc, _ = orc.mps_qr(CR, 0)
+            QRa, _ = orc.mps_qr(AC, 1)
+            QRc, _ = orc.mps_qr(CL, 1)
+            ALn = orc.mps_mat(QLa, orc.btrans(QLc))
+            ARn = orc.mat_mps(orc.btrans(QRc), QRa)
+            err_l = np.abs(AC.data - orc.mps_mat(ALn, CR).data).max()
+            err_r = np.abs(AC.data - orc.mat_mps(CL, ARn).data).max()
+            self.prec[n] = max(err_l, err_r)
+            self.AL[n], self.AR[n] = ALn, ARn
+            self.C[left], self.C[n] = CL, CR
+            etol = max(tol / self.env_fac, self.env_tol)
+            self.LE[n], e_cell = self.left_env(n, etol)
+            self.LE[right] = _add(orc.tm_ret(0, self.AL[n], self.LE[n]), self.optm_left(n - 1))
+            self.LE[self._b(n + 2)] = _add(orc.tm_ret(0, self.AL[right], self.LE[right]), self.optm_left(n))
+            self.RE[self._b(n + 2)] = self.right_env(n + 2, etol)
+            self.RE[right] = _add(orc.tm_ret(1, self.AR[right], self.RE[self._b(n + 2)]), self.optm_right(n + 1))
+        return e_cell / N, F

@@ -1,0 +1,51 @@
+"""VUMPS on the GPU against the oracle VUMPS (golden energy) and the Bethe-ansatz energy.
+
+Workload: the antiferromagnetic Heisenberg chain (XXZ, Delta = -1 in the reference's convention,
+J = 1, hz = 0), two-site cell, QN = [0], D = 64 in 9 U(1) sectors, seed 1001 — the two-site XXZ
+VUMPS of example_config_XXZ.cfg at D = 64.  The fixture tests/golden/vumps_xxz_d64.json holds the
+oracle VUMPS energy at gradient norm F <= 1e-10.  At fixed bond structure VUMPS converges to one
+fixed point, so the GPU iteration (FP64, and BF16X3 operators under FP64 refinement) must reach the
+same energy; the energy error of a state with gradient norm F is O(F^2), so at F <= 1e-9 the two
+agree far below the 1e-12 tolerance.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import imps_cpp_tools_amd as im
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+with open(os.path.join(HERE, "golden", "vumps_xxz_d64.json")) as f:
+    GOLD = json.load(f)
+
+
+def _run(prec, f_target=1e-9, max_sweeps=160):
+    from imps_cpp_tools_amd.vumps import VUMPS
+    ctx = im.Context(0)
+    v = VUMPS(ctx, im.XXZ, GOLD["N"], GOLD["QN"], np.array(GOLD["keys0"], np.int32), np.array(GOLD["sizes0"], np.int32),
+              GOLD["params"], prec=prec, seed=GOLD["seed"])
+    hist = []
+    for _ in range(max_sweeps):
+        st = v.sweep()
+        hist.append((st.energy, max(st.F)))
+        if max(st.F) <= f_target:
+            break
+    return hist
+
+
+@pytest.mark.parametrize("prec", [im.FP64, im.BF16X3], ids=["fp64", "bf16x3"])
+def test_vumps_xxz_d64_energy(prec):
+    hist = _run(prec)
+    e, F = hist[-1]
+    print("sweeps", len(hist), "energy %.15f" % e, "F %.3e" % F, flush=True)
+    assert F <= 1e-9, hist[-5:]
+    assert abs(e - GOLD["energy"]) <= 1e-12, (e, GOLD["energy"])
+    # variational: never below the exact (Bethe ansatz) ground-state energy per site
+    assert e >= GOLD["bethe_ansatz"]
+    # energies decrease monotonically once the iteration is in the quadratic regime
+    tail = [h[0] for h in hist[len(hist) // 2:]]
+    assert max(np.diff(tail)) <= 1e-12
