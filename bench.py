@@ -38,6 +38,10 @@ def main():
     ap.add_argument("--nvec", type=int, default=16, help="Krylov vectors per batched apply")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-vumps", action="store_true")
+    ap.add_argument("--precs", default="fp64,fp32,bf16x3")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -45,6 +49,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_mib:
+        n = args.copy_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_mib} MiB x 5")
+        return
     import imps_cpp_tools_amd as im
     from imps_cpp_tools_amd import workloads as wl
 
@@ -78,7 +91,9 @@ def main():
     torch.cuda.synchronize()
 
     results = {}
-    for name, prec in [("fp64", im.FP64), ("fp32", im.FP32), ("bf16x3", im.BF16X3)]:
+    precs = {"fp64": im.FP64, "fp32": im.FP32, "bf16x3": im.BF16X3}
+    for name in args.precs.split(","):
+        prec = precs[name]
         p = im.HACPlan(ctx, L["AC"], L["EHL"], L["EHR"], HLl, HRl, prec)
         p.set_operator(EHL, EHR, HL, HR)
         flops, bytes_ = p.cost()
@@ -106,7 +121,9 @@ def main():
         log(f"{name}: batched {r['batched']['applies_per_s']:.4g} applies/s ({r['batched']['tflops']:.3g} TF/s), "
             f"single {r['single']['applies_per_s']:.4g} applies/s, {r['batched']['ms_per_launch'] * 1e3:.1f} us/launch")
 
-    mixed, fp64 = results["bf16x3"], results["fp64"]
+    head = "bf16x3" if "bf16x3" in results else next(iter(results))
+    mixed = results[head]
+    rate = lambda k, lab="batched": results[k][lab]["applies_per_s"] if k in results else None  # noqa: E731
     flops = mixed["flops_per_apply"]
     achieved_tf = mixed["batched"]["tflops"]
     out = {
@@ -120,21 +137,43 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3 (split-BF16 MFMA, FP32 accumulate; FP64 in/out)",
+        "dtype": {"bf16x3": "bf16x3 (split-BF16 MFMA, FP32 accumulate; FP64 in/out)", "fp32": "f32", "fp64": "f64"}[head],
         "data": "synthetic (seeded)",
         "config": {"workload": "H_AC apply, Fermi-Hubbard U(1)xU(1) (N,M), d=4, D=1024, 121 sectors of size 1..60",
                    "nvec_per_step": NV, "flops_per_apply": flops, "ac_elements": n},
-        "mixed_vs_fp64": mixed["batched"]["applies_per_s"] / fp64["batched"]["applies_per_s"],
-        "fp64_applies_per_s": fp64["batched"]["applies_per_s"],
-        "fp32_applies_per_s": results["fp32"]["batched"]["applies_per_s"],
+        "mixed_vs_fp64": (mixed["batched"]["applies_per_s"] / rate("fp64")) if "fp64" in results else None,
+        "fp64_applies_per_s": rate("fp64"),
+        "fp32_applies_per_s": rate("fp32"),
         "single_vector_applies_per_s": {k: results[k]["single"]["applies_per_s"] for k in results},
         "vumps_iter_s": None,
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_TFLOPS["bf16x3"], "unit": "TFLOP/s",
-                     "frac": achieved_tf / PEAK_TFLOPS["bf16x3"], "traffic": None,
-                     "kernel": "grouped_gemm_kernel<BF16X3,double>",
-                     "per_launch": {"flops": NV * flops, "us": mixed["batched"]["ms_per_launch"] * 1e3},
-                     "fp64_frac_of_fp64_peak": fp64["batched"]["tflops"] / PEAK_TFLOPS["fp64"]},
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
+                     "frac": achieved_tf / PEAK_TFLOPS[head], "traffic": None,
+                     "kernel": f"grouped_gemm_kernel<{head.upper()},double>",
+                     "per_launch": {"flops": NV * flops, "us": mixed["batched"]["ms_per_launch"] * 1e3}},
     }
+    if "fp64" in results:
+        out["roofline"]["fp64_frac_of_fp64_peak"] = results["fp64"]["batched"]["tflops"] / PEAK_TFLOPS["fp64"]
+
+    # VUMPS sweep time on the two-site Heisenberg chain, D = 64 (9 U(1) sectors, seed 1001)
+    if not args.no_vumps:
+        from imps_cpp_tools_amd.vumps import VUMPS
+        keys0 = np.array([[2 * (i - 4)] for i in range(9)], np.int32)
+        sizes0 = np.array([1, 3, 7, 12, 18, 12, 7, 3, 1], np.int32)
+        vres = {}
+        for name in [k for k in ("fp64", "bf16x3") if k in results]:
+            v = VUMPS(ctx, im.XXZ, 2, [0], keys0, sizes0, [1.0, -1.0, 0.0], prec=precs[name], seed=1001)
+            t0 = time.perf_counter()
+            sweeps, st = 0, None
+            while sweeps < 200:
+                st = v.sweep()
+                sweeps += 1
+                if max(st.F) <= 1e-10:
+                    break
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            vres[name] = dict(sweeps=sweeps, s_per_sweep=dt / sweeps, seconds_to_F=dt, F=max(st.F), energy=st.energy)
+            log(f"vumps xxz D=64 {name}: {sweeps} sweeps, {dt / sweeps * 1e3:.1f} ms/sweep, F {max(st.F):.2e}, E {st.energy:.15f}")
+        out["vumps_xxz_d64"] = vres
 
     # CPU baseline: the oracle (clean-room FP64 restatement of ApplyHA) on the host, bounded sample
     if not args.no_cpu:
