@@ -90,10 +90,11 @@ imps_status imps_create(imps_context* ctx, int device) {
 imps_status imps_destroy(imps_context ctx) {
     return guarded([&] {
         if (!ctx) return IMPS_OK;
-        (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);
-        if (ctx->own) (void)hipStreamDestroy(ctx->own);
-        delete ctx;
+        if (ctx->refs > 1) {  // plans still alive: finish this context's queued work now
+            hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+            hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        }
+        ctx_release(ctx);
         return IMPS_OK;
     });
 }

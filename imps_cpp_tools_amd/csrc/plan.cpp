@@ -5,9 +5,18 @@
 
 namespace imps {
 
+void ctx_release(imps_context ctx) {
+    if (!ctx || --ctx->refs > 0) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+}
+
 imps_plan_s* new_plan(imps_context ctx, int kind, int prec) {
     if (prec < IMPS_FP64 || prec > IMPS_BF16X3) throw InvalidArg("unknown precision");
     auto* P = new imps_plan_s;
+    P->hold.hold(ctx);
     P->ctx = ctx;
     P->kind = kind;
     P->prec = prec;

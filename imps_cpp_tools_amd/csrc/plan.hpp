@@ -13,7 +13,27 @@ struct imps_context_s {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    int refs = 1;  // the caller's handle + one per live plan; freed at zero
 };
+
+namespace imps {
+
+// Drops one reference to a context; the last one synchronises and frees it.
+void ctx_release(imps_context ctx);
+
+// Reference held by a plan, so that a context outlives every plan created on it whatever the
+// order in which a caller destroys them.  Declared as the first member of imps_plan_s: it is
+// destroyed last, after the plan's device buffers.
+struct CtxHold {
+    imps_context c = nullptr;
+    CtxHold() = default;
+    CtxHold(const CtxHold&) = delete;
+    CtxHold& operator=(const CtxHold&) = delete;
+    void hold(imps_context ctx) { c = ctx; if (c) ++c->refs; }
+    ~CtxHold() { if (c) ctx_release(c); }
+};
+
+}  // namespace imps
 
 namespace imps {
 
@@ -24,6 +44,7 @@ struct EnvSolver;  // krylov.cpp
 }  // namespace imps
 
 struct imps_plan_s {
+    imps::CtxHold hold;
     imps_context ctx = nullptr;
     int kind = 0;
     int prec = IMPS_FP64;

@@ -66,6 +66,8 @@ const char* imps_last_error(void);
 const char* imps_version(void);
 /* Binds `device`, creates a default-priority stream.  Fails with IMPS_NO_DEVICE if absent. */
 imps_status imps_create(imps_context* ctx, int device);
+/* Releases the caller's handle.  Plans created on the context keep it alive until the last of
+ * them is destroyed, so contexts and plans may be destroyed in any order. */
 imps_status imps_destroy(imps_context ctx);
 /* Order all later work of this context on a caller stream (hipStream_t passed as void*); NULL
  * selects the legacy default stream.  Until called, the context uses a stream of its own. */
