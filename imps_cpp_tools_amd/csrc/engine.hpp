@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "layout.hpp"
+#include "panel.hpp"
 
 namespace imps {
 
@@ -27,5 +28,19 @@ hipError_t launch_stage(int prec, const DevStage& st, const Slots& slots, void* 
                         bool out_double, int nvec, hipStream_t stream);
 
 hipError_t launch_d2f(const double* in, float* out, long n, hipStream_t stream);
+
+// Panel programs (panel.hpp): packing of the operator store and one grouped launch per stage.
+struct DevPStage {
+    const PTask* tasks = nullptr;
+    const PChunk* chunks = nullptr;
+    int ntasks = 0;
+    int out_slot = SLOT_OUT;
+};
+hipError_t launch_panelize(int prec, const PanelDesc* d_panels, int npanels, const double* op, void* store, hipStream_t stream);
+// ksplit (1, 2, 4): waves sharing one vector's tile; the workgroup serves 4 / ksplit vectors.
+hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, const void* pstore, long pstore_bytes, const double* X, long x_stride,
+                              long x_len, const double* W, long w_stride, long w_len, double* out, long out_stride, int nvec,
+                              hipStream_t stream);
+hipError_t launch_zero(double* p, long n, hipStream_t stream);
 
 }  // namespace imps

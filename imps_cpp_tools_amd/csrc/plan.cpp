@@ -1,6 +1,7 @@
 // Plan construction helpers and execution of grouped-GEMM programs.
 #include "plan.hpp"
 
+#include <cstdlib>
 #include <cstring>
 
 namespace imps {
@@ -22,6 +23,66 @@ imps_plan_s* new_plan(imps_context ctx, int kind, int prec) {
     P->prec = prec;
     return P;
 }
+
+namespace {
+
+// IMPS_PANELS=0 in the environment selects the segment kernels for every plan (comparison runs).
+bool panels_enabled() {
+    const char* e = std::getenv("IMPS_PANELS");
+    return !(e && e[0] == '0');
+}
+
+// K-split of the panel kernel: waves sharing one vector's tile.  One vector: all four waves, so
+// that a single apply still spreads over the chip; four or more: one wave per vector.
+int panel_ksplit(int nvec) {
+    if (const char* e = std::getenv("IMPS_KSPLIT")) {
+        const int k = std::atoi(e);
+        if (k == 1 || k == 2 || k == 4) return k;
+    }
+    return nvec >= 4 ? 1 : nvec >= 2 ? 2 : 4;
+}
+
+template <typename T>
+size_t put(std::vector<char>& host, size_t pos, const std::vector<T>& v) {
+    pos = (pos + 255) & ~size_t(255);
+    if (host.size() < pos + v.size() * sizeof(T)) host.resize(pos + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(host.data() + pos, v.data(), v.size() * sizeof(T));
+    return pos;
+}
+
+void upload_panel_program(imps_plan_s* P) {
+    P->pprog = build_panel_program(P->prog, P->prec);
+    std::vector<char> host;
+    std::vector<size_t> tpos, cpos;
+    size_t pos = 0;
+    for (auto& st : P->pprog.stages) {
+        tpos.push_back(pos = put(host, pos, st.tasks));
+        pos += st.tasks.size() * sizeof(PTask);
+        cpos.push_back(pos = put(host, pos, st.chunks));
+        pos += st.chunks.size() * sizeof(PChunk);
+    }
+    const size_t ppos = put(host, pos, P->pprog.panels);
+    pos = ppos + P->pprog.panels.size() * sizeof(PanelDesc);
+    P->d_pprog.ensure(pos + 256);
+    if (pos) hip_check(hipMemcpyAsync(P->d_pprog.p, host.data(), pos, hipMemcpyHostToDevice, P->ctx->stream), "upload panel program");
+    hip_check(hipStreamSynchronize(P->ctx->stream), "upload panel program");
+    char* base = (char*)P->d_pprog.p;
+    P->dpstages.clear();
+    for (size_t i = 0; i < P->pprog.stages.size(); ++i) {
+        DevPStage ds;
+        ds.tasks = (const PTask*)(base + tpos[i]);
+        ds.chunks = (const PChunk*)(base + cpos[i]);
+        ds.ntasks = (int)P->pprog.stages[i].tasks.size();
+        ds.out_slot = P->pprog.stages[i].out_slot;
+        P->dpstages.push_back(ds);
+    }
+    P->d_panels = (const PanelDesc*)(base + ppos);
+    P->d_pstore.ensure((size_t)std::max<long>(P->pprog.nfrag, 1) * panel_frag_bytes(P->prec));
+    P->d_op.ensure(std::max<long>(P->prog.op_elems, 1) * sizeof(double));  // FP64 staging of the operator pieces
+    P->w_ready = 0;
+}
+
+}  // namespace
 
 void upload_program(imps_plan_s* P) {
     if (!P->ctx) return;
@@ -48,7 +109,9 @@ void upload_program(imps_plan_s* P) {
     }
     if (pos) hip_check(hipMemcpyAsync(P->d_prog.p, host.data(), pos, hipMemcpyHostToDevice, P->ctx->stream), "upload program");
     hip_check(hipStreamSynchronize(P->ctx->stream), "upload program");
-    P->d_op.ensure(std::max<long>(P->prog.op_elems, 1) * P->elem_bytes());
+    P->use_panels = (P->kind == K_TM || P->kind == K_HAC || P->kind == K_HC) && panel_compatible(P->prog) && panels_enabled();
+    if (P->use_panels) upload_panel_program(P);
+    else P->d_op.ensure(std::max<long>(P->prog.op_elems, 1) * P->elem_bytes());
 }
 
 void set_operator(imps_plan_s* P, const std::vector<const double*>& parts) {
@@ -60,7 +123,7 @@ void set_operator(imps_plan_s* P, const std::vector<const double*>& parts) {
         const long n = P->op_parts[i];
         if (n > 0 && !parts[i]) throw InvalidArg("operator data pointer is NULL");
         if (n > 0) {
-            if (P->prec == IMPS_FP64)
+            if (P->use_panels || P->prec == IMPS_FP64)
                 hip_check(hipMemcpyAsync((double*)P->d_op.p + off, parts[i], n * sizeof(double), hipMemcpyDeviceToDevice, P->ctx->stream),
                           "copy operator");
             else
@@ -68,6 +131,9 @@ void set_operator(imps_plan_s* P, const std::vector<const double*>& parts) {
         }
         off += n;
     }
+    if (P->use_panels)
+        hip_check(launch_panelize(P->prec, P->d_panels, (int)P->pprog.panels.size(), (const double*)P->d_op.p, P->d_pstore.p, P->ctx->stream),
+                  "pack operator panels");
     P->op_set = true;
 }
 
@@ -78,6 +144,23 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
     if (!P->op_set && !op_override && P->prog.op_elems > 0) throw InvalidArg("operator data not set");
     hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
     hipStream_t s = P->ctx->stream;
+    if (P->use_panels && !op_override) {
+        const long we = P->prog.w_elems;
+        if (we > 0 && P->w_ready < nvec) {
+            // the workspace is read past block edges (into finite data) before it is rewritten: zero it once
+            P->d_w.ensure((size_t)we * nvec * sizeof(double));
+            hip_check(launch_zero((double*)P->d_w.p, we * nvec, s), "zero workspace");
+            P->w_ready = nvec;
+        }
+        const int ks = panel_ksplit(nvec);
+        for (auto& ds : P->dpstages) {
+            const bool to_w = ds.out_slot == SLOT_W;
+            hip_check(launch_panel_stage(P->prec, ds, ks, P->d_pstore.p, P->pprog.nfrag * panel_frag_bytes(P->prec), x, P->prog.x_elems, P->prog.x_elems, (const double*)P->d_w.p, we, we,
+                                         to_w ? (double*)P->d_w.p : y, to_w ? we : P->prog.y_elems, nvec, s),
+                      "panel gemm");
+        }
+        return;
+    }
     P->d_w.ensure((size_t)std::max<long>(P->prog.w_elems, 1) * nvec * P->elem_bytes());
     Slots sl;
     sl.base[SLOT_X] = x;           sl.stride[SLOT_X] = P->prog.x_elems;  // FP64, converted in-kernel

@@ -54,6 +54,13 @@ struct imps_plan_s {
     bool op_set = false;
     imps::DevBuf d_prog, d_op, d_w;
     std::vector<imps::DevStage> dstages;
+    // panel execution (TM, H_AC, H_C): packed operator store, FP64 workspace
+    bool use_panels = false;
+    imps::PanelProgram pprog;
+    imps::DevBuf d_pprog, d_pstore;
+    const imps::PanelDesc* d_panels = nullptr;
+    std::vector<imps::DevPStage> dpstages;
+    long w_ready = 0;  // workspace vectors already zero-filled
     std::shared_ptr<imps::EnvSolver> env;  // K_ENV
     imps::QrPlanData qr;                   // K_QR
     int qr_dir = 0;

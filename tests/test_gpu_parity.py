@@ -95,11 +95,11 @@ def test_hac_parity(ctx, prec, name):
     HR = orc.reduced_op(H, c.d, c.AR[1], 1)
     EHL = random_like(c.diag(0), rng, sym=True)
     EHR = random_like(c.diag(1), rng, sym=True)
-    xs = [random_like(c.struct[0], rng) for _ in range(4)]
+    xs = [random_like(c.struct[0], rng) for _ in range(5)]
     refs = [orc.hac(EHL, EHR, HL, HR, x).data for x in xs]
     p = im.HACPlan(ctx, lay(xs[0]), lay(EHL), lay(EHR), lay(HL), lay(HR), prec)
     p.set_operator(dev(EHL.data), dev(EHR.data), dev(HL.data), dev(HR.data))
-    for nvec in (1, 4):
+    for nvec in (1, 2, 3, 4, 5):  # K-split 4, 2, 2, 1, 1 (partly filled workgroups for 3 and 5)
         X = np.concatenate([x.data for x in xs[:nvec]])
         y = torch.full((X.size,), float("nan"), dtype=torch.float64, device="cuda")
         p.apply(dev(X), y, nvec)
@@ -126,6 +126,39 @@ def test_hc_parity(ctx, prec, name):
     y = torch.full((C.nelem,), float("nan"), dtype=torch.float64, device="cuda")
     p.apply(dev(C.data), y)
     assert relerr(y.cpu().numpy(), ref) < TOL[prec]
+    # three vectors at once (the third a scaled copy of the first: H_C is linear)
+    C2 = random_like(c.diag(0), rng)
+    ref2 = orc.hc(EHL, EHR, H, c.d, AL, AR, C2).data
+    X = np.concatenate([C.data, C2.data, -2.0 * C.data])
+    y3 = torch.full((X.size,), float("nan"), dtype=torch.float64, device="cuda")
+    p.apply(dev(X), y3, 3)
+    got = y3.cpu().numpy().reshape(3, -1)
+    for g, r in zip(got, [ref, ref2, -2.0 * ref]):
+        assert relerr(g, r) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("ksplit", ["1", "2", "4"])
+def test_hac_ksplit_variants(ctx, prec, ksplit, monkeypatch):
+    """Every K-split of the panel kernel (waves per vector) gives the oracle's result, for a
+    vector count that leaves workgroups partly filled."""
+    monkeypatch.setenv("IMPS_KSPLIT", ksplit)
+    c = cell("fhub")
+    H = ham(c)
+    rng = np.random.default_rng(18)
+    HL = orc.reduced_op(H, c.d, c.AL[1], 0)
+    HR = orc.reduced_op(H, c.d, c.AR[1], 1)
+    EHL = random_like(c.diag(0), rng, sym=True)
+    EHR = random_like(c.diag(1), rng, sym=True)
+    xs = [random_like(c.struct[0], rng) for _ in range(3)]
+    p = im.HACPlan(ctx, lay(xs[0]), lay(EHL), lay(EHR), lay(HL), lay(HR), prec)
+    p.set_operator(dev(EHL.data), dev(EHR.data), dev(HL.data), dev(HR.data))
+    X = np.concatenate([x.data for x in xs])
+    y = torch.full((X.size,), float("nan"), dtype=torch.float64, device="cuda")
+    p.apply(dev(X), y, 3)
+    got = y.cpu().numpy().reshape(3, -1)
+    for v, x in enumerate(xs):
+        assert relerr(got[v], orc.hac(EHL, EHR, HL, HR, x).data) < TOL[prec]
 
 
 @pytest.mark.parametrize("prec", [im.FP64, im.FP32])
@@ -154,3 +187,10 @@ def test_full_size_hubbard_hac(ctx):
         y = torch.full((ref.size,), float("nan"), dtype=torch.float64, device="cuda")
         p.apply(dev(st["AC"].data), y)
         assert relerr(y.cpu().numpy(), ref) < TOL[prec], (prec, relerr(y.cpu().numpy(), ref))
+        # batched: 16 vectors (v-th = (v + 1) * AC; H_AC is linear), as in the benchmark
+        X = np.concatenate([(v + 1.0) * st["AC"].data for v in range(16)])
+        y16 = torch.full((X.size,), float("nan"), dtype=torch.float64, device="cuda")
+        p.apply(dev(X), y16, 16)
+        got = y16.cpu().numpy().reshape(16, -1)
+        for v in range(16):
+            assert relerr(got[v], (v + 1.0) * ref) < TOL[prec], (prec, v)
