@@ -37,10 +37,24 @@ struct DevPStage {
     int out_slot = SLOT_OUT;
 };
 hipError_t launch_panelize(int prec, const PanelDesc* d_panels, int npanels, const double* op, void* store, hipStream_t stream);
+// Split copies (panel.hpp) of the input vectors' raw blocks: grid (nblocks, nvec).
+hipError_t launch_split(int prec, const SplitBlock* d_blocks, int nblocks, const double* x, long x_stride, void* split, long split_stride,
+                        long pl, int nvec, hipStream_t stream);
+// Buffers of one panel-program apply.  Strides and sizes of split copies are bytes per vector.
+struct PanelLaunch {
+    const void* pstore = nullptr;
+    long pstore_bytes = 0;
+    const void* xs = nullptr;
+    long xs_stride = 0;
+    void* ws = nullptr;
+    long ws_stride = 0;
+    long plx = 0, plw = 0;
+    double* out = nullptr;
+    long out_stride = 0;
+    int nvec = 0;
+};
 // ksplit (1, 2, 4): waves sharing one vector's tile; the workgroup serves 4 / ksplit vectors.
-hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, const void* pstore, long pstore_bytes, const double* X, long x_stride,
-                              long x_len, const double* W, long w_stride, long w_len, double* out, long out_stride, int nvec,
-                              hipStream_t stream);
+hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, const PanelLaunch& L, hipStream_t stream);
 hipError_t launch_zero(double* p, long n, hipStream_t stream);
 
 }  // namespace imps

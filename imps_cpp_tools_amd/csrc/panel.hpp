@@ -4,18 +4,22 @@
 //
 // Operator blocks are pre-packed once per set_operator into *panels*: zero-padded, pre-converted
 // (FP64, FP32, or split BF16 hi/lo), already in the MFMA register layout, so that a wave loads a
-// 16 x KC operator fragment with one or two fully coalesced 16-byte loads per lane and no
-// conversion, clamping or masking.  A panel P(r, k) holds alpha * op(S) with r the output-side
-// index (rows of C when the block is the left factor, columns of C when it is the right factor)
-// and k the contracted index; rows >= R and k >= K are zero.  Fragment (rg, kc) covers
-// r in [16 rg, 16 rg + 16), k in [KC kc, KC kc + KC) and is stored as 64 lanes x LANE elements:
-// lane l holds r = 16 rg + l % 16 and the run of consecutive k starting at KC kc + RUN (l / 16).
+// 16 x KC operator fragment with fully coalesced 16-byte loads and no conversion or masking.
+// A panel P(r, k) holds alpha * op(S) with r the output-side index (rows of C when the block is
+// the left factor, columns of C when it is the right factor) and k the contracted index; rows
+// >= R and k >= K are zero.  Fragment (rg, kc) covers r in [16 rg, 16 rg + 16) and
+// k in [KC kc, KC kc + KC) and is stored as 64 lanes x RUN elements: lane l holds
+// r = 16 rg + l % 16 and the RUN consecutive k starting at KC kc + RUN (l / 16).
 //
-// The other factor (the "raw" operand) is read from the FP64 input vector or workspace in its
-// natural layout and converted in registers.  Reads are buffer loads; the elements of the padded
-// 32 x KC region that lie outside the block (rows/columns past the block edge, k >= K) are given
-// an out-of-range offset, for which the hardware returns zero without touching memory, so that
-// small blocks cost only the cache lines they occupy.
+// The other factor (the "raw" operand: a block of the input vector or of the workspace) is read
+// from a *split copy*: the vector converted once per apply to the compute format and stored
+// twice, as is (plane N) and with every block transposed (plane T), so that whichever way a
+// segment uses the block, the k index is contiguous and a lane's RUN consecutive k come in one
+// or two 16-byte loads.  BF16X3 keeps separate hi and lo planes.  Split copy of one vector:
+//   FP64/FP32: [N | T], each PL elements;   BF16X3: [N_hi | T_hi | N_lo | T_lo], each PL elements,
+// PL = element count + a zero pad of at least KC, so that a k-run past the end of the last block
+// stays inside the buffer (such elements meet zero panel entries).  Workspace stages write their
+// output directly in split form.
 //
 // Every task (output tile <= 32 x 32) owns an ordered list of chunks (segment, K-range of KC).
 // Tasks are ordered by decreasing chunk count so that the longest ones start first.
@@ -36,20 +40,21 @@ inline int panel_kc(int prec) { return prec == IMPS_BF16X3 ? 32 : 16; }
 inline int panel_run(int prec) { return prec == IMPS_BF16X3 ? 8 : 4; }
 // Bytes of one 64-lane fragment: BF16X3 stores a hi plane and a lo plane of 64 x 8 bf16 each.
 inline long panel_frag_bytes(int prec) { return prec == IMPS_FP64 ? 64 * 4 * 8 : prec == IMPS_FP32 ? 64 * 4 * 4 : 2 * 64 * 8 * 2; }
+// Plane length of a split copy of `len` elements, and bytes of one vector's split copy.
+inline long split_plane(long len) { return (len + 64 + 63) / 64 * 64; }
+inline long split_elem_bytes(int prec) { return prec == IMPS_FP64 ? 8 : prec == IMPS_FP32 ? 4 : 2; }
+inline long split_bytes(int prec, long len) { return (prec == IMPS_BF16X3 ? 4 : 2) * split_plane(len) * split_elem_bytes(prec); }
 
-// PCF_STAGE: the raw factor is k-contiguous in memory (s_k == 1).  Its fragment would then need
-// one cache line per output-side index for every load; instead the wave reads the whole 32 x KC
-// region with consecutive lanes on consecutive k (coalesced), parks it in LDS and reads it back
-// in the MFMA layout.
-enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_STAGE = 2, PCF_RAW_W = 4 };
+enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_RAW_W = 4 };
 
 struct PChunk {       // 32 bytes
-    long raw_off;     // element offset of raw(idx0, k0) within one vector's slot
+    long raw_off;     // element index, within the (hi) N|T planes, of raw(idx0, k0)
     int pan;          // fragment index of the panel fragment (rg0, kc)
     int pan_sub;      // fragment index step to the second 16-row group (-1 if the panel has none)
-    int s_idx, s_k;   // raw element strides along the output-side index and along k
+    int s_idx;        // raw element stride along the output-side index (k is contiguous)
     int flags;
     int krem;         // valid k in this chunk and after it (K - k0)
+    int pad;
 };
 
 struct PTask {        // 40 bytes
@@ -67,6 +72,11 @@ struct PanelDesc {    // one packed operator block
     double alpha;
 };
 
+struct SplitBlock {   // one block of the input vector that is read as a raw factor
+    long off;
+    int rows, cols;
+};
+
 struct PStage {
     int out_slot = SLOT_OUT;
     std::vector<PTask> tasks;
@@ -77,7 +87,9 @@ struct PanelProgram {
     int prec = IMPS_FP64;
     std::vector<PStage> stages;
     std::vector<PanelDesc> panels;
-    long nfrag = 0;   // panel fragments in total
+    std::vector<SplitBlock> xblocks;  // input-vector blocks to split per apply
+    long nfrag = 0;                   // panel fragments in total
+    long plx = 0, plw = 0;            // split plane lengths of the input vector and the workspace
 };
 
 // True when every segment of the program has exactly one operator-store operand.
@@ -89,12 +101,16 @@ inline bool panel_compatible(const Program& P) {
 }
 
 inline PanelProgram build_panel_program(const Program& P, int prec) {
-    // raw reads use 32-bit byte offsets within one vector; buffer sizes are 32-bit byte counts
-    if (P.x_elems >= (1L << 28) || P.w_elems >= (1L << 28)) throw InvalidArg("panel: vector too long for 32-bit offsets");
     PanelProgram Q;
     Q.prec = prec;
+    Q.plx = split_plane(P.x_elems);
+    Q.plw = split_plane(P.w_elems);
+    // buffer loads use 32-bit byte offsets within one vector's split copy
+    if (split_bytes(prec, P.x_elems) >= (1L << 31) || split_bytes(prec, P.w_elems) >= (1L << 31))
+        throw InvalidArg("panel: vector too long for 32-bit offsets");
     const int KC = panel_kc(prec);
     std::map<std::tuple<long, int, double>, int> index;  // (src, tr, alpha) -> panel
+    std::map<long, SplitBlock> xb;
     for (auto& st : P.stages) {
         PStage ps;
         ps.out_slot = st.out_slot;
@@ -125,32 +141,34 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                 }
                 const PanelDesc& pd = Q.panels[it->second];
                 if (pd.R != R || pd.K != s.K) throw InvalidArg("panel: operator block used with two shapes");
-                // raw side: element (idx, k) of the raw factor as stored
-                //   raw = A (pa false): op(A)(i, k) = a_tr ? S[k + i*ld] : S[i + k*ld]
-                //   raw = B (pa true):  op(B)(k, j) = b_tr ? S[j + k*ld] : S[k + j*ld]
-                int s_idx, s_k;
-                long roff;
-                int rslot, idx0;
-                if (pa) {
-                    rslot = s.b_slot; roff = s.b_off; idx0 = t.n0;
-                    if (s.b_tr) { s_idx = 1; s_k = s.b_ld; } else { s_idx = s.b_ld; s_k = 1; }
-                } else {
-                    rslot = s.a_slot; roff = s.a_off; idx0 = t.m0;
-                    if (s.a_tr) { s_idx = s.a_ld; s_k = 1; } else { s_idx = 1; s_k = s.a_ld; }
-                }
+                // raw side.  The stored raw block S (rows Rb, cols Cb, at boff) is used as
+                //   raw = A: op(A)(i, k) = a_tr ? S(k, i) : S(i, k)     (i runs over the output rows)
+                //   raw = B: op(B)(k, j) = b_tr ? S(j, k) : S(k, j)     (j runs over the output columns)
+                // so raw(idx, k) is S(idx, k) ("idx-major") or S(k, idx) ("k-major").  A k-major
+                // element sits in plane N at boff + k + idx*Rb; an idx-major one in plane T, where
+                // S(r, c) is stored at boff + c + r*Cb, at boff + k + idx*Cb.
+                const int rslot = pa ? s.b_slot : s.a_slot;
                 if (rslot != SLOT_X && rslot != SLOT_W) throw InvalidArg("panel: raw operand must be the vector or the workspace");
+                const long boff = pa ? s.b_off : s.a_off;
+                const int rtr = pa ? s.b_tr : s.a_tr;
+                const int Rb = pa ? (rtr ? t.N : s.K) : (rtr ? s.K : t.M);
+                const int Cb = pa ? (rtr ? s.K : t.N) : (rtr ? t.M : s.K);
+                const bool kmajor = pa ? (rtr == 0) : (rtr != 0);
+                const int idx0 = pa ? t.n0 : t.m0;
+                const long pl = rslot == SLOT_X ? Q.plx : Q.plw;
+                if (rslot == SLOT_X) xb[boff] = SplitBlock{boff, Rb, Cb};
+                const int s_idx = kmajor ? Rb : Cb;
+                const long base = kmajor ? boff : pl + boff;
                 const int rg0 = (pa ? t.m0 : t.n0) / 16;
                 for (int kc = 0; kc < pd.nkc; ++kc) {
                     PChunk c;
-                    c.raw_off = roff + (long)idx0 * s_idx + (long)kc * KC * s_k;
+                    c.raw_off = base + (long)kc * KC + (long)idx0 * s_idx;
                     c.pan = (int)(pd.frag0 + (long)rg0 * pd.nkc + kc);
                     c.pan_sub = rg0 + 1 < pd.nrg ? pd.nkc : -1;  // -1: no second row group
                     c.s_idx = s_idx;
-                    c.s_k = s_k;
-                    c.flags = (pa ? PCF_PANEL_IS_A : 0) | (rslot == SLOT_W ? PCF_RAW_W : 0) | (s_k == 1 && s_idx != 1 ? PCF_STAGE : 0);
+                    c.flags = (pa ? PCF_PANEL_IS_A : 0) | (rslot == SLOT_W ? PCF_RAW_W : 0);
                     c.krem = s.K - kc * KC;
-                    if (c.raw_off + 31L * s_idx + (long)(KC - 1) * s_k >= (1L << 28))
-                        throw InvalidArg("panel: raw operand offsets exceed 32-bit range");
+                    c.pad = 0;
                     chs.push_back(c);
                 }
             }
@@ -169,6 +187,7 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
         }
         Q.stages.push_back(std::move(ps));
     }
+    for (auto& kv : xb) Q.xblocks.push_back(kv.second);
     if (Q.nfrag * panel_frag_bytes(prec) >= (1L << 31)) throw InvalidArg("panel store too large for 32-bit offsets");
     return Q;
 }

@@ -63,6 +63,8 @@ void upload_panel_program(imps_plan_s* P) {
     }
     const size_t ppos = put(host, pos, P->pprog.panels);
     pos = ppos + P->pprog.panels.size() * sizeof(PanelDesc);
+    const size_t xpos = put(host, pos, P->pprog.xblocks);
+    pos = xpos + P->pprog.xblocks.size() * sizeof(SplitBlock);
     P->d_pprog.ensure(pos + 256);
     if (pos) hip_check(hipMemcpyAsync(P->d_pprog.p, host.data(), pos, hipMemcpyHostToDevice, P->ctx->stream), "upload panel program");
     hip_check(hipStreamSynchronize(P->ctx->stream), "upload panel program");
@@ -77,9 +79,10 @@ void upload_panel_program(imps_plan_s* P) {
         P->dpstages.push_back(ds);
     }
     P->d_panels = (const PanelDesc*)(base + ppos);
+    P->d_xblocks = (const SplitBlock*)(base + xpos);
     P->d_pstore.ensure((size_t)std::max<long>(P->pprog.nfrag, 1) * panel_frag_bytes(P->prec));
     P->d_op.ensure(std::max<long>(P->prog.op_elems, 1) * sizeof(double));  // FP64 staging of the operator pieces
-    P->w_ready = 0;
+    P->xs_ready = P->ws_ready = 0;
 }
 
 }  // namespace
@@ -145,20 +148,34 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
     hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
     hipStream_t s = P->ctx->stream;
     if (P->use_panels && !op_override) {
-        const long we = P->prog.w_elems;
-        if (we > 0 && P->w_ready < nvec) {
-            // the workspace is read past block edges (into finite data) before it is rewritten: zero it once
-            P->d_w.ensure((size_t)we * nvec * sizeof(double));
-            hip_check(launch_zero((double*)P->d_w.p, we * nvec, s), "zero workspace");
-            P->w_ready = nvec;
+        const PanelProgram& Q = P->pprog;
+        const long xsb = split_bytes(P->prec, P->prog.x_elems), wsb = split_bytes(P->prec, P->prog.w_elems);
+        // split copies: zero-filled when (re)allocated, so that pads and unused places hold zeros
+        if (P->xs_ready < nvec) {
+            P->d_xs.ensure((size_t)xsb * nvec);
+            hip_check(launch_zero((double*)P->d_xs.p, xsb * nvec / 8, s), "zero split copy");
+            P->xs_ready = nvec;
         }
+        if (P->prog.w_elems > 0 && P->ws_ready < nvec) {
+            P->d_ws.ensure((size_t)wsb * nvec);
+            hip_check(launch_zero((double*)P->d_ws.p, wsb * nvec / 8, s), "zero workspace");
+            P->ws_ready = nvec;
+        }
+        hip_check(launch_split(P->prec, P->d_xblocks, (int)Q.xblocks.size(), x, P->prog.x_elems, P->d_xs.p, xsb, Q.plx, nvec, s), "split copy");
+        PanelLaunch L;
+        L.pstore = P->d_pstore.p;
+        L.pstore_bytes = Q.nfrag * panel_frag_bytes(P->prec);
+        L.xs = P->d_xs.p;
+        L.xs_stride = xsb;
+        L.ws = P->d_ws.p;
+        L.ws_stride = wsb;
+        L.plx = Q.plx;
+        L.plw = Q.plw;
+        L.out = y;
+        L.out_stride = P->prog.y_elems;
+        L.nvec = nvec;
         const int ks = panel_ksplit(nvec);
-        for (auto& ds : P->dpstages) {
-            const bool to_w = ds.out_slot == SLOT_W;
-            hip_check(launch_panel_stage(P->prec, ds, ks, P->d_pstore.p, P->pprog.nfrag * panel_frag_bytes(P->prec), x, P->prog.x_elems, P->prog.x_elems, (const double*)P->d_w.p, we, we,
-                                         to_w ? (double*)P->d_w.p : y, to_w ? we : P->prog.y_elems, nvec, s),
-                      "panel gemm");
-        }
+        for (auto& ds : P->dpstages) hip_check(launch_panel_stage(P->prec, ds, ks, L, s), "panel gemm");
         return;
     }
     P->d_w.ensure((size_t)std::max<long>(P->prog.w_elems, 1) * nvec * P->elem_bytes());

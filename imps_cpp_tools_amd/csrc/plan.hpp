@@ -57,10 +57,11 @@ struct imps_plan_s {
     // panel execution (TM, H_AC, H_C): packed operator store, FP64 workspace
     bool use_panels = false;
     imps::PanelProgram pprog;
-    imps::DevBuf d_pprog, d_pstore;
+    imps::DevBuf d_pprog, d_pstore, d_xs, d_ws;  // program tables, panels, split copies of input and workspace
     const imps::PanelDesc* d_panels = nullptr;
+    const imps::SplitBlock* d_xblocks = nullptr;
     std::vector<imps::DevPStage> dpstages;
-    long w_ready = 0;  // workspace vectors already zero-filled
+    long xs_ready = 0, ws_ready = 0;  // vectors of the split buffers already zero-filled
     std::shared_ptr<imps::EnvSolver> env;  // K_ENV
     imps::QrPlanData qr;                   // K_QR
     int qr_dir = 0;
