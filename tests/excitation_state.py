@@ -1,0 +1,64 @@
+"""A small converged ground state for excitation tests (built with the oracle VUMPS on the CPU).
+
+Anisotropic Heisenberg chain in the reference convention Delta = -1.5 (antiferromagnetic, gapped),
+J = 1, hz = 0, two-site cell, QN = [0], bond 0: 7 sectors of sizes [1, 2, 3, 4, 3, 2, 1] (D = 16),
+seed 7, iterated to gradient norm F < 1e-12.  Also returns the null spaces NL (oracle Nullspace),
+L = C^T C and R = C C^T on bond 0 and the oracle's excitation constants."""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+from oracle import excitations as ex
+from oracle import pyoracle as orc
+
+
+@functools.lru_cache(maxsize=None)
+def state():
+    from oracle_vumps import OracleVUMPS
+    keys0 = np.array([[2 * (i - 3)] for i in range(7)], np.int32)
+    sizes0 = np.array([1, 2, 3, 4, 3, 2, 1], np.int32)
+    v = OracleVUMPS(0, 2, [0], keys0, sizes0, [1.0, -1.5, 0.0], seed=7)
+    e, F = 0.0, [1.0]
+    for _ in range(200):
+        e, F = v.sweep()
+        if max(F) < 1e-12:
+            break
+    NLob = [orc.nullspace(a, 0) for a in v.AL]
+    H = np.array(v.H)
+    AL = [ex.from_ob(a) for a in v.AL]
+    AR = [ex.from_ob(a) for a in v.AR]
+    NL = [ex.from_ob(a) for a in NLob]
+    C = [ex.from_ob(c)[0] for c in v.C]
+    H0 = H - e * np.eye(H.shape[0])
+    L = ex.bm_mul(ex.bm_t(C[-1]), C[-1])
+    R = ex.bm_mul(C[-1], ex.bm_t(C[-1]))
+    HLtot, HRtot = ex.heff_constants(AL, AR, L, R, H0)
+    return dict(v=v, E0=e, F=max(F), H=H, H0=H0, ALob=v.AL, ARob=v.AR, Cob=v.C, NLob=NLob, AL=AL, AR=AR, NL=NL, C=C, L=L, R=R,
+                HLtot=HLtot, HRtot=HRtot, N=2, d=2)
+
+
+def oracle_heff(K, p, z):
+    """Oracle ApplyHeff on the flat complex vector z (XDims order) at momentum p (units of pi per site)."""
+    s = state()
+    xd = ex.xdims(s["NL"], s["AR"], K)
+    proj = abs(p) < 1e-13 and all(k == 0 for k in K)
+    LM, RM = (None, None)
+    if proj:
+        LM, RM, _ = ex.lr_overlaps(s["AL"], s["AR"], K)
+    kfac = np.exp(1j * p * np.pi * s["N"])
+    X = ex.from_vec_list(z, xd)
+    return ex.vectorize_list(ex.apply_heff(X, kfac, s["AL"], s["AR"], s["NL"], LM, RM, s["H0"], s["HLtot"], s["HRtot"]))
+
+
+def oracle_heff_matrix(K, p):
+    s = state()
+    xd = ex.xdims(s["NL"], s["AR"], K)
+    n = sum(r * c for dv in xd for *_, r, c in dv)
+    M = np.zeros((n, n), complex)
+    for j in range(n):
+        e = np.zeros(n, complex)
+        e[j] = 1.0
+        M[:, j] = oracle_heff(K, p, e)
+    return M
