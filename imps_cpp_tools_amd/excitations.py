@@ -80,40 +80,23 @@ class DObj:
         return self.data.dim() == 2
 
 
-class ExcitationEngine:
-    """Effective Hamiltonian of the momentum-k excitation ansatz for one ground state.
+class BlockOps:
+    """Block-sparse objects on the device and the reference's block algebra over them: products
+    through the library's grouped GEMM (ProdPlan, FP64), sums over the union of sector sets,
+    restriction onto a sector set, complex scaling, unit-cell transfer-matrix chains, GMRES for
+    the geometric sums and Arnoldi for transfer-matrix spectra."""
 
-    AL, AR, NL: device FP64 arrays of the unit cell's tensors with their layouts; C: the bond
-    matrices (C[n] on bond n + 1) with layouts; H: dense d^2 x d^2 two-site Hamiltonian (E0 is
-    subtracted here); K: excitation charge.  Ground-state energy per site E0 is given by the
-    caller.  All work is FP64."""
-
-    def __init__(self, ctx: Context, AL, ALl, AR, ARl, NL, NLl, C, Cl, H: np.ndarray, d: int, E0: float, K, inv_tol: float = 1e-14):
+    def __init__(self, ctx: Context, nsym: int, d: int = 2, H: Optional[np.ndarray] = None, E0: float = 0.0, tol: float = 1e-14):
         import torch
         self.torch = torch
         self.dev = "cuda"
         self.ctx = ctx
-        self.N = len(AL)
+        self.nsym = nsym
         self.d = d
-        self.nsym = ALl[0].nsym
-        self.K = tuple(int(k) for k in np.atleast_1d(K))
-        self.tol = inv_tol
-        self.H0 = np.array(H, np.float64) - E0 * np.eye(d * d)
+        self.tol = tol
+        self.H0 = None if H is None else np.array(H, np.float64) - E0 * np.eye(d * d)
         self._plans: Dict = {}
         self._maps: Dict = {}
-        self.AL = [DObj(l, a) for a, l in zip(AL, ALl)]
-        self.AR = [DObj(l, a) for a, l in zip(AR, ARl)]
-        self.NL = [DObj(l, a) for a, l in zip(NL, NLl)]
-        self.C = [DObj(l, c) for c, l in zip(C, Cl)]
-        self.bond0 = Cl[-1]
-        # reduced operators of the single tensors (real)
-        self.HLr = [self._redop(a, LEFT) for a in self.AL]
-        self.HRr = [self._redop(a, RIGHT) for a in self.AR]
-        self.HLtot, self.HRtot = self._heff_constants()
-        Cb = self.C[-1]
-        self.LM = DObj(Cb.lay, self._transpose_blocks(Cb))
-        self.RM = Cb
-        self.xdims = self._xdims()
 
     # ---- helpers -----------------------------------------------------------------------------
     def _plan(self, key, make):
@@ -302,77 +285,6 @@ class ExcitationEngine:
     def _empty_entry(self, cplx):
         return DObj(BlockLayout(self.nsym, [0], np.zeros((0, self.nsym)), np.zeros((0, self.nsym)), [], []), self._zeros(0, cplx))
 
-    # ---- constants -----------------------------------------------------------------------------
-    def _identity(self, lay: BlockLayout) -> DObj:
-        t = np.zeros(lay.nelem)
-        for (e, q), (qo, r, c, o) in _index(lay).items():
-            t[o:o + r * c].reshape(c, r)[np.diag_indices(min(r, c))] = 1.0
-        return DObj(lay, self.torch.from_numpy(t).to(self.dev))
-
-    def _heff_constants(self):
-        """HLtot, HRtot (HeffConstants, Excitations_helpers.hpp:260-347), real."""
-        N, H = self.N, self.H0
-        AL, AR = self.AL, self.AR
-        opl = lambda AA: self.diag_part(self.tml_unity(self.apply_operator(AA, H), AA))  # noqa: E731
-        opr = lambda AA: self.diag_part(self.tmr_unity(self.apply_operator(AA, H), AA))  # noqa: E731
-        real = lambda x: DObj(x.lay, x.data[0].contiguous()) if x.cplx else x  # noqa: E731
-        HL = [real(opl(self.mps_mps(AL[-1], AL[0])))]
-        for n in range(1, N):
-            HL.append(real(self.add(opl(self.mps_mps(AL[n - 1], AL[n])), self.diag_part(self.tml(AL[n], AL[n], HL[-1])))))
-        HR = [real(opr(self.mps_mps(AR[-1], AR[0])))]
-        for n in range(N - 2, -1, -1):
-            HR.insert(0, real(self.add(opr(self.mps_mps(AR[n], AR[n + 1])), self.diag_part(self.tmr(AR[n], AR[n], HR[0])))))
-        b0 = self.bond0
-        Cb = self.C[-1]
-        R = self.prod(MPS_MATT, Cb, Cb)  # C C^T
-        L = self.prod(MATT_MPS, Cb, Cb)  # C^T C
-        one = self._identity(b0)
-        x = self.onto(HL[-1], b0)
-        pl = EnvPlan(self.ctx, LEFT, [a.lay for a in AL], b0, FP64)
-        pl.set_mps([a.data for a in AL])
-        yl = self._zeros(b0.nelem, False)
-        pl.solve(x.data, one.data, self.onto(R, b0).data, yl, tol=self.tol, maxit=20000, krylov=100, check=False)
-        pr = EnvPlan(self.ctx, RIGHT, [a.lay for a in AR], b0, FP64)
-        pr.set_mps([a.data for a in AR])
-        yr = self._zeros(b0.nelem, False)
-        pr.solve(self.onto(HR[0], b0).data, self.onto(L, b0).data, one.data, yr, tol=self.tol, maxit=20000, krylov=100, check=False)
-        EHL = [None] * N
-        EHR = [None] * N
-        EHL[N - 1] = DObj(b0, yl)
-        HLtot = [None] * N
-        HLtot[N - 1] = EHL[N - 1]
-        for n in range(N - 1):
-            EHL[n] = real(self.tml(AL[n], AL[n], EHL[(n - 1) % N]))
-            HLtot[n] = real(self.add(HL[n], EHL[n]))
-        EHR[0] = DObj(b0, yr)
-        HRtot = [None] * N
-        HRtot[0] = EHR[0]
-        for n in range(N - 1, 0, -1):
-            EHR[n] = real(self.tmr(AR[n], AR[n], EHR[(n + 1) % N]))
-            HRtot[n] = real(self.add(EHR[n], HR[n]))
-        return HLtot, HRtot
-
-    def _xdims(self) -> List[BlockLayout]:
-        """Sectors of X[n] (XDims, Excitations_helpers.hpp:639-678)."""
-        out = []
-        for n in range(self.N):
-            ml, mr = {}, {}
-            for (e, q), (qo, r, c, o) in _index(self.NL[n].lay).items():
-                ml[qo] = c
-            for (e, q), (qo, r, c, o) in _index(self.AR[(n + 1) % self.N].lay).items():
-                mr[q] = r
-            idx = {}
-            for q in sorted(ml):
-                qk = tuple(a + b for a, b in zip(q, self.K))
-                if qk in mr and ml[q] > 0 and mr[qk] > 0:
-                    idx[(0, q)] = (qk, ml[q], mr[qk], 0)
-            out.append(_layout_from_index(self.nsym, 1, idx))
-        return out
-
-    @property
-    def mtot(self) -> int:
-        return sum(l.nelem for l in self.xdims)
-
     # ---- geometric sums ------------------------------------------------------------------------
     def _gmres(self, afun, b, x0, tol):
         """Unrestarted GMRES with modified Gram-Schmidt and complex Givens rotations on complex
@@ -495,6 +407,169 @@ class ExcitationEngine:
         for a, b in zip(reversed(As), reversed(Bs)):
             X = self.tmr(a, b, X)
         return X
+
+    # ---- transfer-matrix spectra ------------------------------------------------------------------
+    def tm_sector(self, As, Bs, direction, K) -> BlockLayout:
+        """Sector list of the (mixed) TM eigenproblem in charge K: blocks q -> q + K with q over the
+        ingoing sectors of A[0] (r) or B[0] (l) (TMmixedEigs, EigsBlockFunctions.hpp:282-322)."""
+        def ml(x: DObj):
+            return {q: r for (e, q), (qo, r, c, o) in _index(x.lay).items()}
+        diml, dimr = ml(As[0]), ml(Bs[0])
+        if direction == LEFT:
+            diml, dimr = dimr, diml
+        K = tuple(int(k) for k in np.atleast_1d(K))
+        idx = {}
+        for q in sorted(diml):
+            qk = tuple(a + b for a, b in zip(q, K))
+            if qk in dimr:
+                idx[(0, q)] = (qk, diml[q], dimr[qk], 0)
+        return _layout_from_index(self.nsym, 1, idx)
+
+    def tm_eigs(self, As, Bs, direction, K, nev: int = 4, tol: float = 1e-12, krylov: int = 40, maxit: int = 100, seed: int = 0):
+        """Largest-magnitude eigenvalues of the unit-cell transfer matrix of (As, Bs) in charge sector
+        K ("LM" mode of TMEigs / TMmixedEigs, EigsBlockFunctions.hpp:177-391; TMDominantEig :91-172
+        is nev = 1 with As = Bs).  Arnoldi with modified Gram-Schmidt applied twice, restarted from
+        the sum of the wanted Ritz vectors; converged when every wanted Ritz pair has residual
+        <= tol * |theta|.  Returns (eigenvalues sorted by decreasing magnitude, eigenvectors as complex
+        objects on the sector list, TM applications)."""
+        t = self.torch
+        lay = self.tm_sector(As, Bs, direction, K)
+        n = lay.nelem
+        if n == 0:
+            raise ValueError("no transfer-matrix sectors for this charge")
+        T = (lambda v: self._chain_left(As, Bs, v)) if direction == LEFT else (lambda v: self._chain_right(As, Bs, v))
+        op = lambda z: self._c(self.onto(T(self._p(z, lay)), lay))  # noqa: E731
+        m = min(krylov, n)
+        gen = t.Generator(device="cpu").manual_seed(seed)
+        v = t.complex(t.randn(n, generator=gen, dtype=t.float64), t.randn(n, generator=gen, dtype=t.float64)).to(self.dev)
+        nmv = 0
+        for _ in range(maxit):
+            V = [v / t.linalg.vector_norm(v)]
+            Hm = np.zeros((m + 1, m), complex)
+            for j in range(m):
+                w = op(V[j])
+                nmv += 1
+                for _ in range(2):
+                    for i in range(j + 1):
+                        h = t.vdot(V[i], w).item()
+                        Hm[i, j] += h
+                        w = w - h * V[i]
+                Hm[j + 1, j] = t.linalg.vector_norm(w).item()
+                if j + 1 < m:
+                    if abs(Hm[j + 1, j]) < 1e-14:
+                        m = j + 1
+                        Hm = Hm[:m + 1, :m]
+                        break
+                    V.append(w / Hm[j + 1, j])
+            theta, S = np.linalg.eig(Hm[:m, :m])
+            order = np.argsort(-np.abs(theta))
+            theta, S = theta[order], S[:, order]
+            k = min(nev, m)
+            res = np.abs(Hm[m, m - 1]) * np.abs(S[m - 1, :k])
+            Vm = t.stack(V[:m])
+            Y = t.from_numpy(S[:, :k].T.copy()).to(self.dev) @ Vm
+            if np.all(res <= tol * np.abs(theta[:k])) or m >= n:
+                return theta[:k], [self._p(Y[i], lay) for i in range(k)], nmv
+            v = Y.sum(dim=0)
+        raise ImpsError(2, "transfer-matrix eigensolver did not converge")
+
+
+class ExcitationEngine(BlockOps):
+    """Effective Hamiltonian of the momentum-k excitation ansatz for one ground state.
+
+    AL, AR, NL: device FP64 arrays of the unit cell's tensors with their layouts; C: the bond
+    matrices (C[n] on bond n + 1) with layouts; H: dense d^2 x d^2 two-site Hamiltonian (E0 is
+    subtracted here); K: excitation charge.  Ground-state energy per site E0 is given by the
+    caller.  All work is FP64."""
+
+    def __init__(self, ctx: Context, AL, ALl, AR, ARl, NL, NLl, C, Cl, H: np.ndarray, d: int, E0: float, K, inv_tol: float = 1e-14):
+        super().__init__(ctx, ALl[0].nsym, d, H, E0, inv_tol)
+        self.N = len(AL)
+        self.K = tuple(int(k) for k in np.atleast_1d(K))
+        self.AL = [DObj(l, a) for a, l in zip(AL, ALl)]
+        self.AR = [DObj(l, a) for a, l in zip(AR, ARl)]
+        self.NL = [DObj(l, a) for a, l in zip(NL, NLl)]
+        self.C = [DObj(l, c) for c, l in zip(C, Cl)]
+        self.bond0 = Cl[-1]
+        # reduced operators of the single tensors (real)
+        self.HLr = [self._redop(a, LEFT) for a in self.AL]
+        self.HRr = [self._redop(a, RIGHT) for a in self.AR]
+        self.HLtot, self.HRtot = self._heff_constants()
+        Cb = self.C[-1]
+        self.LM = DObj(Cb.lay, self._transpose_blocks(Cb))
+        self.RM = Cb
+        self.xdims = self._xdims()
+
+    # ---- constants -----------------------------------------------------------------------------
+    def _identity(self, lay: BlockLayout) -> DObj:
+        t = np.zeros(lay.nelem)
+        for (e, q), (qo, r, c, o) in _index(lay).items():
+            t[o:o + r * c].reshape(c, r)[np.diag_indices(min(r, c))] = 1.0
+        return DObj(lay, self.torch.from_numpy(t).to(self.dev))
+
+    def _heff_constants(self):
+        """HLtot, HRtot (HeffConstants, Excitations_helpers.hpp:260-347), real."""
+        N, H = self.N, self.H0
+        AL, AR = self.AL, self.AR
+        opl = lambda AA: self.diag_part(self.tml_unity(self.apply_operator(AA, H), AA))  # noqa: E731
+        opr = lambda AA: self.diag_part(self.tmr_unity(self.apply_operator(AA, H), AA))  # noqa: E731
+        real = lambda x: DObj(x.lay, x.data[0].contiguous()) if x.cplx else x  # noqa: E731
+        HL = [real(opl(self.mps_mps(AL[-1], AL[0])))]
+        for n in range(1, N):
+            HL.append(real(self.add(opl(self.mps_mps(AL[n - 1], AL[n])), self.diag_part(self.tml(AL[n], AL[n], HL[-1])))))
+        HR = [real(opr(self.mps_mps(AR[-1], AR[0])))]
+        for n in range(N - 2, -1, -1):
+            HR.insert(0, real(self.add(opr(self.mps_mps(AR[n], AR[n + 1])), self.diag_part(self.tmr(AR[n], AR[n], HR[0])))))
+        b0 = self.bond0
+        Cb = self.C[-1]
+        R = self.prod(MPS_MATT, Cb, Cb)  # C C^T
+        L = self.prod(MATT_MPS, Cb, Cb)  # C^T C
+        one = self._identity(b0)
+        x = self.onto(HL[-1], b0)
+        pl = EnvPlan(self.ctx, LEFT, [a.lay for a in AL], b0, FP64)
+        pl.set_mps([a.data for a in AL])
+        yl = self._zeros(b0.nelem, False)
+        pl.solve(x.data, one.data, self.onto(R, b0).data, yl, tol=self.tol, maxit=20000, krylov=100, check=False)
+        pr = EnvPlan(self.ctx, RIGHT, [a.lay for a in AR], b0, FP64)
+        pr.set_mps([a.data for a in AR])
+        yr = self._zeros(b0.nelem, False)
+        pr.solve(self.onto(HR[0], b0).data, self.onto(L, b0).data, one.data, yr, tol=self.tol, maxit=20000, krylov=100, check=False)
+        EHL = [None] * N
+        EHR = [None] * N
+        EHL[N - 1] = DObj(b0, yl)
+        HLtot = [None] * N
+        HLtot[N - 1] = EHL[N - 1]
+        for n in range(N - 1):
+            EHL[n] = real(self.tml(AL[n], AL[n], EHL[(n - 1) % N]))
+            HLtot[n] = real(self.add(HL[n], EHL[n]))
+        EHR[0] = DObj(b0, yr)
+        HRtot = [None] * N
+        HRtot[0] = EHR[0]
+        for n in range(N - 1, 0, -1):
+            EHR[n] = real(self.tmr(AR[n], AR[n], EHR[(n + 1) % N]))
+            HRtot[n] = real(self.add(EHR[n], HR[n]))
+        return HLtot, HRtot
+
+    def _xdims(self) -> List[BlockLayout]:
+        """Sectors of X[n] (XDims, Excitations_helpers.hpp:639-678)."""
+        out = []
+        for n in range(self.N):
+            ml, mr = {}, {}
+            for (e, q), (qo, r, c, o) in _index(self.NL[n].lay).items():
+                ml[qo] = c
+            for (e, q), (qo, r, c, o) in _index(self.AR[(n + 1) % self.N].lay).items():
+                mr[q] = r
+            idx = {}
+            for q in sorted(ml):
+                qk = tuple(a + b for a, b in zip(q, self.K))
+                if qk in mr and ml[q] > 0 and mr[qk] > 0:
+                    idx[(0, q)] = (qk, ml[q], mr[qk], 0)
+            out.append(_layout_from_index(self.nsym, 1, idx))
+        return out
+
+    @property
+    def mtot(self) -> int:
+        return sum(l.nelem for l in self.xdims)
 
     # ---- ApplyHeff ---------------------------------------------------------------------------
     def apply(self, X: List[DObj], p: float) -> List[DObj]:

@@ -16,6 +16,7 @@ reference code it restates:
   HeffConstants           Excitations_helpers.hpp:260-347
   XDims                   Excitations_helpers.hpp:639-678
   LRoverlaps              Excitations_helpers.hpp:915-946 (TMmixedEigs, EigsBlockFunctions.hpp:282-391)
+  TM spectra (dense)      TMEigs / TMmixedEigs, EigsBlockFunctions.hpp:177-391
   ApplyHeff               Excitations_helpers.hpp:20-228
 
 Deviations: GMRES starts from y0 = 0 when no guess is given (the reference draws a random y0;
@@ -436,6 +437,22 @@ def _tm_sector_dims(A, B, direction, K: Key):
         if qk in dimr:
             dv.append((q, qk, diml[q], dimr[qk]))
     return dv
+
+
+def tm_eigs_dense(As, Bs, direction: int, K: Key):
+    """All eigenvalues (sorted by decreasing magnitude) of the unit-cell mixed TM of (As, Bs) on the
+    charge-K sector list of TMmixedEigs (EigsBlockFunctions.hpp:282-391), by a dense eigensolve."""
+    dv = _tm_sector_dims(As, Bs, direction, K)
+    n = sum(r * c for *_, r, c in dv)
+    M = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        X = from_vec(e, dv)
+        T = tml_chain(As, Bs, X) if direction == 0 else tmr_chain(As, Bs, X)
+        M[:, j] = np.real(_map_onto(T, dv))
+    w = np.linalg.eigvals(M)
+    return w[np.argsort(-np.abs(w))]
 
 
 def lr_overlaps(AL, AR, K: Key):

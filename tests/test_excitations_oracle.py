@@ -101,3 +101,14 @@ def test_heff_hermitian_and_positive(K, p):
     assert np.abs(M - M.conj().T).max() <= 1e-10 * np.abs(M).max()
     w = np.linalg.eigvalsh(0.5 * (M + M.conj().T))
     assert w[0] > 0  # excitations lie above the ground state (E0 subtracted)
+
+
+def test_tm_spectrum_dense_fixed_point():
+    """The dense TM spectrum has the isometries' fixed-point eigenvalue 1 (left-normalised AL, left
+    and right chains alike) and the mixed (AL, AR) TM of the converged state has it as well."""
+    s = es.state()
+    for direction in (0, 1):
+        w = ex.tm_eigs_dense(s["AL"], s["AL"], direction, (0,))
+        assert abs(w[0] - 1.0) < 1e-12 and abs(w[1]) < 1.0
+        wm = ex.tm_eigs_dense(s["AL"], s["AR"], direction, (0,))
+        assert abs(abs(wm[0]) - 1.0) < 1e-10

@@ -85,3 +85,22 @@ def test_excitation_energies(K, p):
     w, Y, nmv = eng.lowest(p, nev=4, tol=1e-10, krylov=30)
     print(f"K {K} p {p}: {w} ({nmv} applies)", flush=True)
     assert np.abs(w - ref).max() < 1e-9
+
+
+@pytest.mark.parametrize("mixed", [False, True], ids=["AL", "AL_AR"])
+@pytest.mark.parametrize("direction", [0, 1], ids=["l", "r"])
+@pytest.mark.parametrize("K", [(0,), (2,)], ids=["K0", "K2"])
+def test_tm_eigs(mixed, direction, K):
+    """Largest-magnitude TM eigenvalues in a charge sector (TMEigs / TMmixedEigs) against a dense
+    eigensolve of the oracle's transfer matrix."""
+    from oracle import excitations as ex
+    s = es.state()
+    eng = engine((0,))
+    As = eng.AL
+    Bs = eng.AR if mixed else eng.AL
+    ref = ex.tm_eigs_dense(s["AL"], s["AR"] if mixed else s["AL"], direction, K)
+    w, Y, nmv = eng.tm_eigs(As, Bs, direction, K, nev=4, tol=1e-12, krylov=30)
+    assert np.allclose(np.abs(w), np.abs(ref[:4]), rtol=0, atol=1e-10)
+    assert max(min(abs(x - ref)) for x in w) < 1e-10
+    if not mixed and K == (0,):
+        assert abs(w[0] - 1.0) < 1e-12  # the isometry's fixed point
