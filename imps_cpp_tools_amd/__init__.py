@@ -49,6 +49,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m imps_cpp_tools_amd.build`")
+        # torch ships its own HIP runtime under the same soname (libamdhip64.so.7) as the one this
+        # library is linked against; whichever is loaded first serves the whole process.  Load
+        # torch's first so that torch (which owns the device buffers and streams) and this library
+        # share one runtime; torch cannot start on a runtime other than its own.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         P, I, D, V = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
         LP = ctypes.POINTER(_CLayout)

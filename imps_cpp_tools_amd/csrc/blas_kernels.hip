@@ -66,7 +66,9 @@ __global__ void axpby_kernel(double a, const double* __restrict__ da, const doub
                              long n, int inv_a) {
     double aa = da ? *da : a;
     if (inv_a) aa = 1.0 / aa;
-    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) y[r] = aa * x[r] + b * y[r];
+    // BLAS convention: a zero coefficient does not read its operand (fresh buffers may hold NaN)
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
+        y[r] = (aa != 0.0 ? aa * x[r] : 0.0) + (b != 0.0 ? b * y[r] : 0.0);
 }
 
 static unsigned grid_for(long n) {

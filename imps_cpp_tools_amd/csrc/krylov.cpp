@@ -107,7 +107,7 @@ KrylovStats gmres(hipStream_t s, const LinOp& A, const double* b, double* x, dou
     W.init(n, m);
     const double bnorm = dev_norm(s, W, b);
     if (bnorm == 0) {
-        hip_check(blas_axpby(0, nullptr, false, b, 0, x, n, s), "zero");
+        hip_check(launch_zero(x, n, s), "zero");
         st.converged = true;
         return st;
     }
@@ -188,7 +188,7 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
     dx.ensure(sizeof(double) * std::max<long>(n, 1));
     const double bnorm = dev_norm(s, W, b);
     if (bnorm == 0) {
-        hip_check(blas_axpby(0, nullptr, false, b, 0, x, n, s), "zero");
+        hip_check(launch_zero(x, n, s), "zero");
         st.converged = true;
         return st;
     }
@@ -202,7 +202,7 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
         st.relres = rn / bnorm;
         if (st.relres < tol) { st.converged = true; break; }
         if (st.iters >= maxit) break;
-        hip_check(blas_axpby(0, nullptr, false, r.d(), 0, dx.d(), n, s), "zero");
+        hip_check(launch_zero(dx.d(), n, s), "zero");
         const int budget = std::min(maxit - st.iters, 4 * (m + 1));
         KrylovStats in = gmres(s, Alo, r.d(), dx.d(), std::max(inner_tol, 0.5 * tol * bnorm / rn), budget, m);
         st.iters += in.iters;
@@ -212,56 +212,76 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
 }
 
 KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol, int maxit, int m) {
+    // Thick-restart Lanczos (Wu-Simon): with full CGS2 re-orthogonalisation the projected matrix
+    // T = V^T A V is assembled column by column from the orthogonalisation coefficients; at a
+    // restart the lowest `keep` Ritz vectors (T diagonal there) and the next Lanczos direction
+    // form the new basis, so the search space is not thrown away.
     KrylovStats st;
     const long n = A.n;
     m = (int)std::max<long>(2, std::min<long>(m, n));
+    const int keep = std::max(1, std::min(m / 3, m - 2));
     Work W;
     W.init(n, m);
+    DevBuf Y;
+    Y.ensure(sizeof(double) * (size_t)n * keep);
     double nx = dev_norm(s, W, x);
     if (nx == 0) throw InvalidArg("lanczos: zero start vector");
     hip_check(blas_axpby(1.0 / nx, nullptr, false, x, 0.0, W.Vi(0), n, s), "scale");
-    std::vector<double> T, h, w, Z;
+    std::vector<double> T((size_t)m * m, 0.0), h, w, Z;
+    int j0 = 0;  // basis vectors already present (and their T block) at the start of a cycle
     while (true) {
-        // Krylov basis with full reorthogonalisation
-        T.assign((size_t)m * m, 0.0);
         int k = 0;
-        for (int j = 0; j < m; ++j) {
-            double* wv = W.Vi(j + 1 <= m ? j + 1 : m);
+        double beta = 0;
+        for (int j = j0; j < m; ++j) {
+            double* wv = W.Vi(j + 1);
             A.apply(W.Vi(j), wv);
             ++st.iters;
             cgs2(s, W, j + 1, wv, h);
-            const double beta = dev_norm(s, W, wv);
+            beta = dev_norm(s, W, wv);
             for (int i = 0; i <= j; ++i) { T[i + (size_t)j * m] = h[i]; T[j + (size_t)i * m] = h[i]; }
             k = j + 1;
-            if (j + 1 < m) {
-                T[j + 1 + (size_t)j * m] = beta;
-                T[j + (size_t)(j + 1) * m] = beta;
-            }
-            if (beta <= 1e-14 * std::abs(h[j]) || j + 1 == m || st.iters >= maxit) {
-                // Ritz pair of the k x k projection
-                std::vector<double> Tk((size_t)k * k);
-                for (int c = 0; c < k; ++c) for (int r = 0; r < k; ++r) Tk[r + (size_t)c * k] = T[r + (size_t)c * m];
-                jacobi_eig(k, Tk, w, Z);
-                int imin = (int)(std::min_element(w.begin(), w.end()) - w.begin());
-                const double theta = w[imin];
-                const double res = std::abs(beta * Z[(k - 1) + (size_t)imin * k]);
-                st.value = theta;
-                st.relres = res / std::max(std::abs(theta), 1e-300);
-                // x = V[0..k) z
-                std::vector<double> z(k);
-                for (int i = 0; i < k; ++i) z[i] = -Z[i + (size_t)imin * k];
-                h2d(s, W.dh.d(), z.data(), k);
-                hip_check(blas_axpby(0, nullptr, false, x, 0, x, n, s), "zero");
-                hip_check(blas_gemv_sub(W.V.d(), n, k, W.dh.d(), x, n, s), "ritz");
-                nx = dev_norm(s, W, x);
-                hip_check(blas_axpby(1.0 / nx, nullptr, false, x, 0.0, x, n, s), "scale");
-                break;
-            }
+            if (beta <= 1e-14 * std::max(std::abs(h[j]), 1e-300) || j + 1 == m || st.iters >= maxit) break;
+            T[j + 1 + (size_t)j * m] = beta;
+            T[j + (size_t)(j + 1) * m] = beta;
             hip_check(blas_axpby(1.0 / beta, nullptr, false, wv, 0.0, wv, n, s), "normalise");
         }
-        if (st.relres <= tol || k < 2) { st.converged = st.relres <= tol; break; }
-        if (st.iters >= maxit) break;
-        d2d(s, W.Vi(0), x, n);
+        // Ritz pairs of the k x k projection (ascending)
+        std::vector<double> Tk((size_t)k * k);
+        for (int c = 0; c < k; ++c) for (int r = 0; r < k; ++r) Tk[r + (size_t)c * k] = T[r + (size_t)c * m];
+        jacobi_eig(k, Tk, w, Z);
+        std::vector<int> order(k);
+        for (int i = 0; i < k; ++i) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return w[a] < w[b]; });
+        const int imin = order[0];
+        const double theta = w[imin];
+        st.value = theta;
+        st.relres = std::abs(beta * Z[(k - 1) + (size_t)imin * k]) / std::max(std::abs(theta), 1e-300);
+        const bool done = st.relres <= tol || k < 2 || st.iters >= maxit || beta <= 1e-14 * std::max(std::abs(theta), 1e-300);
+        const int nk = done ? 1 : std::min(keep, k - 1);
+        // Ritz vectors Y_i = V[0..k) Z[:, order[i]] for the lowest nk
+        for (int i = 0; i < nk; ++i) {
+            std::vector<double> z(k);
+            for (int r = 0; r < k; ++r) z[r] = -Z[r + (size_t)order[i] * k];
+            double* yi = Y.d() + (size_t)i * n;
+            h2d(s, W.dh.d(), z.data(), k);
+            hip_check(launch_zero(yi, n, s), "zero");  // fresh memory: not 0 * (possibly NaN) garbage
+            hip_check(blas_gemv_sub(W.V.d(), n, k, W.dh.d(), yi, n, s), "ritz");
+        }
+        if (done) {
+            d2d(s, x, Y.d(), n);
+            nx = dev_norm(s, W, x);
+            hip_check(blas_axpby(1.0 / nx, nullptr, false, x, 0.0, x, n, s), "scale");
+            st.converged = st.relres <= tol;
+            break;
+        }
+        // new basis: the nk Ritz vectors and the normalised next Lanczos vector
+        double* next = W.Vi(k);
+        hip_check(blas_axpby(1.0 / beta, nullptr, false, next, 0.0, next, n, s), "normalise");
+        d2d(s, W.Vi(nk), next, n);
+        for (int i = 0; i < nk; ++i) d2d(s, W.Vi(i), Y.d() + (size_t)i * n, n);
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int i = 0; i < nk; ++i) T[i + (size_t)i * m] = w[order[i]];
+        j0 = nk;
     }
     // true residual ||A x - theta x|| / |theta|
     A.apply(x, W.w.d());
@@ -397,7 +417,7 @@ KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, 
     d2d(s, rhs.d(), x, n);
     hip_check(blas_axpby(-trx, nullptr, false, rho_out, 1.0, rhs.d(), n, s), "rhs");
     if (y0) d2d(s, y, y0, n);
-    else hip_check(blas_axpby(0, nullptr, false, x, 0, y, n, s), "zero y");
+    else hip_check(launch_zero(y, n, s), "zero y");
     if (maxit <= 0) maxit = (int)std::min<long>(std::max<long>(20 * n, 1000), 100000);
     if (krylov <= 0) krylov = 100;
     LinOp A64 = make_op(tm64);
@@ -461,7 +481,7 @@ int EnvSolver::fixed_point(double* x, double tol, int maxit, double* delta) {
         // l chain: rho_in = R slot, rho_out = L slot; r chain: rho_in = L, rho_out = R
         const double* Lp = dir == IMPS_LEFT ? x : id.d();
         const double* Rp = dir == IMPS_LEFT ? id.d() : x;
-        KrylovStats st = solve(r0.d(), Lp, Rp, nullptr, dx.d(), std::max(0.1 * tol / std::max(dl, 1e-300), 1e-13), maxit - its, 100);
+        KrylovStats st = solve(r0.d(), Lp, Rp, nullptr, dx.d(), std::max(0.1 * tol / std::max(dl, 1e-300), 1e-13), maxit - its, 200);
         its += st.iters;
         hip_check(blas_axpby(1.0, nullptr, false, dx.d(), 1.0, x, n, s), "update");
     }

@@ -47,7 +47,7 @@ class VumpsStats:
 class VUMPS:
     def __init__(self, ctx: Context, model: int, N: int, QN, keys0, sizes0, params, prec: int = FP64, seed: int = 0,
                  eig_tol_max: float = 1e-6, eig_tol_min: float = 5e-16, env_tol: float = 5e-15, env_fac: float = 2.0,
-                 krylov: int = 20):
+                 krylov: int = 20, eig_maxit: int = 100000, env_krylov: int = 100, env_maxit: int = 20000, polar_start: bool = True):
         import torch
         self.torch = torch
         self.ctx, self.prec, self.N = ctx, prec, N
@@ -57,6 +57,8 @@ class VUMPS:
         self.slay = [mps_structure(model, N, QN, self.bonds[n], self.bonds[(n + 1) % N]) for n in range(N)]
         self.H = model_ham(model, params)
         self.eig_tol_max, self.eig_tol_min, self.env_tol, self.env_fac, self.krylov = eig_tol_max, eig_tol_min, env_tol, env_fac, krylov
+        self.eig_maxit, self.env_krylov, self.env_maxit = eig_maxit, env_krylov, env_maxit
+        self.dosvd, self.svdthresh = polar_start, 5e-7  # polar (SVD) scheme until the gauge error is below svdthresh
         self._plans: Dict = {}
         rng = np.random.default_rng(seed)
         self.AL, self.AR, self.C = [], [], []
@@ -175,12 +177,12 @@ class VUMPS:
         # fixed point of T_R over the cell: the right-direction chain of the same AL tensors
         pr = self._plan(("envR_AL", b), lambda: EnvPlan(self.ctx, RIGHT, [self.slay[s] for s in sites], self.blay[b], FP64))
         pr.set_mps([self.AL[s] for s in sites])
-        pr.fixed_point(R, tol=1e-14, maxit=5000, check=False)
+        pr.fixed_point(R, tol=1e-14, maxit=self.env_maxit, check=False)
         HAL = self._get_hl(b)
         one = self._identity(b)
         y = self._empty(self.blay[b].nelem)
         y0 = self.LE.get(b)
-        it, rr = p.solve(HAL, one, R, y, y0=y0, tol=tol, maxit=5000, krylov=100, check=False)
+        it, rr = p.solve(HAL, one, R, y, y0=y0, tol=tol, maxit=self.env_maxit, krylov=self.env_krylov, check=False)
         self.env_iters += it
         e_cell = float(self.torch.dot(HAL, self._transpose_blocks(R, b)).item())
         return y, e_cell
@@ -194,11 +196,11 @@ class VUMPS:
         L, _ = self._prod(MATT_MPS, self.blay[b], Cb, self.blay[b], Cb, ("CtC", b))  # C^T C
         pl = self._plan(("envL_AR", b), lambda: EnvPlan(self.ctx, LEFT, [self.slay[s] for s in sites], self.blay[b], FP64))
         pl.set_mps([self.AR[s] for s in sites])
-        pl.fixed_point(L, tol=1e-14, maxit=5000, check=False)
+        pl.fixed_point(L, tol=1e-14, maxit=self.env_maxit, check=False)
         HAR = self._get_hr(b)
         one = self._identity(b)
         y = self._empty(self.blay[b].nelem)
-        it, rr = p.solve(HAR, L, one, y, y0=self.RE.get(b), tol=tol, maxit=5000, krylov=100, check=False)
+        it, rr = p.solve(HAR, L, one, y, y0=self.RE.get(b), tol=tol, maxit=self.env_maxit, krylov=self.env_krylov, check=False)
         self.env_iters += it
         return y
 
@@ -261,6 +263,40 @@ class VUMPS:
         nsec = sum(1 for q in cols if cols[q] > rows[q])
         return (B / np.sqrt(nsec) if nsec else 0.0), B
 
+    def _scaled_identity(self, lay: BlockLayout, scale):
+        out = np.zeros(lay.nelem)
+        for o, r, sc in zip(lay.offsets, lay.rows, scale):
+            out[o:o + r * r].reshape(r, r)[np.diag_indices(r)] = sc
+        return self._dev(out)
+
+    def _polar(self, X, lay, direction, key, tol=1e-14, maxit=200):
+        """Isometric polar factor U V^T of the regrouped sector matrices of X (left: rows (s, q - d_s),
+        right: the transposed grouping) by the Newton-Schulz iteration X <- X (3 - X^T X) / 2
+        (left) or X <- (3 - X X^T) X / 2 (right) after scaling every sector to unit Frobenius norm;
+        it stops when X^T X (X X^T) equals the identity to `tol`.  Stands in for the SVD-based
+        polar decomposition svd(S, P, X, dir); U P of the reference."""
+        t = self.torch
+        kind_g = TM_LEFT_UNITY if direction == LEFT else TM_RIGHT_UNITY
+        for it in range(maxit):
+            G, Gl = self._prod(kind_g, lay, X, lay, X, key + ("G",))
+            if it == 0:  # per-sector scaling by the Frobenius norm, sqrt(tr G_q)
+                tr = []
+                for o, r in zip(Gl.offsets, Gl.rows):
+                    tr.append(float(G[o:o + r * r].view(r, r).diagonal().sum().item()))
+                scale = [1.0 / np.sqrt(v) if v > 0 else 0.0 for v in tr]
+                S = self._scaled_identity(Gl, scale)
+                X, _ = (self._prod(MPS_MAT, lay, X, Gl, S, key + ("s",)) if direction == LEFT
+                        else self._prod(MAT_MPS, Gl, S, lay, X, key + ("s",)))
+                continue
+            I = self._scaled_identity(Gl, [1.0] * len(Gl.rows))
+            dev = float((G - I).abs().max().item())
+            if dev < tol:
+                break
+            M = 1.5 * I - 0.5 * G
+            X, _ = (self._prod(MPS_MAT, lay, X, Gl, M, key + ("m",)) if direction == LEFT
+                    else self._prod(MAT_MPS, Gl, M, lay, X, key + ("m",)))
+        return X
+
     # ------------------------------------------------------------------ one outer iteration
     def sweep(self) -> VumpsStats:
         torch = self.torch
@@ -275,24 +311,32 @@ class VUMPS:
             hac, hac64 = self._hac_plans(n)
             AC, _ = self._prod(MAT_MPS, self.blay[n], self.C[left], self.slay[n], self.AR[n], ("CAR", n))
             F[n], G[n] = self.gradnorm(n, AC, hac64 or hac)
-            _, it, _ = eigsh(hac, AC, tol=tol, krylov=self.krylov, plan_fp64=hac64, check=False)
+            _, it, _ = eigsh(hac, AC, tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=hac64, check=False)
             self.eig_iters += it
             hcl, hcl64 = self._hc_plans(n)
             CL = self.C[left].clone()
-            _, it, _ = eigsh(hcl, CL, tol=tol, krylov=self.krylov, plan_fp64=hcl64, check=False)
+            _, it, _ = eigsh(hcl, CL, tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=hcl64, check=False)
             self.eig_iters += it
             hcr, hcr64 = self._hc_plans(n + 1)
             CR = self.C[n].clone()
-            _, it, _ = eigsh(hcr, CR, tol=tol, krylov=self.krylov, plan_fp64=hcr64, check=False)
+            _, it, _ = eigsh(hcr, CR, tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=hcr64, check=False)
             self.eig_iters += it
             CL = CL if float(CL[0].item()) >= 0 else -CL  # sign convention of VUMPS_main.cpp:201-202
             CR = CR if float(CR[0].item()) >= 0 else -CR
-            QLa, _ = self._qr(AC, self.slay[n], LEFT)
-            QLc, _ = self._qr(CR, self.blay[self._b(n + 1)], LEFT)
-            QRa, _ = self._qr(AC, self.slay[n], RIGHT)
-            QRc, _ = self._qr(CL, self.blay[n], RIGHT)
-            ALn, _ = self._prod(MPS_MATT, self.slay[n], QLa, self.blay[self._b(n + 1)], QLc, ("ALnew", n))
-            ARn, _ = self._prod(MATT_MPS, self.blay[n], QRc, self.slay[n], QRa, ("ARnew", n))
+            if self.dosvd and self.prec_n[n] < self.svdthresh:
+                self.dosvd = False
+            if self.dosvd:  # polar factors of AC CR^T and CL^T AC (the SVD scheme, VUMPS_main.cpp:213-220)
+                X, _ = self._prod(MPS_MATT, self.slay[n], AC, self.blay[self._b(n + 1)], CR, ("ACCRt", n))
+                ALn = self._polar(X, self.slay[n], LEFT, ("pl", n))
+                X, _ = self._prod(MATT_MPS, self.blay[n], CL, self.slay[n], AC, ("CLtAC", n))
+                ARn = self._polar(X, self.slay[n], RIGHT, ("pr", n))
+            else:         # QR scheme (:224-232)
+                QLa, _ = self._qr(AC, self.slay[n], LEFT)
+                QLc, _ = self._qr(CR, self.blay[self._b(n + 1)], LEFT)
+                QRa, _ = self._qr(AC, self.slay[n], RIGHT)
+                QRc, _ = self._qr(CL, self.blay[n], RIGHT)
+                ALn, _ = self._prod(MPS_MATT, self.slay[n], QLa, self.blay[self._b(n + 1)], QLc, ("ALnew", n))
+                ARn, _ = self._prod(MATT_MPS, self.blay[n], QRc, self.slay[n], QRa, ("ARnew", n))
             # precision of the gauge: max(||AC - AL CR||_inf, ||AC - CL AR||_inf)
             ALCR, _ = self._prod(MPS_MAT, self.slay[n], ALn, self.blay[self._b(n + 1)], CR, ("ALCR", n))
             CLAR, _ = self._prod(MAT_MPS, self.blay[n], CL, self.slay[n], ARn, ("CLAR", n))
