@@ -9,7 +9,10 @@ keeps the device buffers.  The update order follows the outer loop of VUMPS_main
     F(n)  = gradient norm at AC = C[n-1] AR[n]          (GradNormRight, VUMPS_helpers.hpp:147-153)
     AC    = lowest eigenvector of H_AC                   (eigs_rs "SA", :186)
     CL,CR = lowest eigenvectors of H_C on bonds n, n+1   (:199-202, sign fixed by the first entry)
-    AL[n] = Q(AC,l) Q(CR,l)^T,  AR[n] = Q(CL,r)^T Q(AC,r) (QR polar scheme, :225-232)
+    AL[n] = polar(AC CR^T), AR[n] = polar(CL^T AC)       (SVD scheme, :213-220, until the gauge
+                                                          error is below svdthresh; polar factors by
+                                                          the Newton-Schulz iteration)
+    AL[n] = Q(AC,l) Q(CR,l)^T,  AR[n] = Q(CL,r)^T Q(AC,r) (QR polar scheme, :225-232, afterwards)
     environments: LE[n] by the left geometric sum with R = fixed point of the cell TM (trueLR),
     LE[n+1], LE[n+2] by TM steps, RE[n+2] by the right geometric sum, RE[n+1] by a TM step
     (:247-305).
@@ -19,6 +22,7 @@ Environments: LE[b] holds the two-site terms left of bond b that do not touch si
 AR[n+1]; H_C on bond b uses LE[b], RE[b], AL[b-1], AR[b].
 
 Energy per site = tr(HL_b R_b) / N with HL_b the one-cell source of the left geometric sum.
+Initial state: the reference's random mixed-canonical MPS (randMPS_LR), see VUMPS._random_state.
 """
 from __future__ import annotations
 
@@ -31,6 +35,20 @@ import numpy as np
 from . import (BF16X3, FP32, FP64, LEFT, RIGHT, MAT_MPS, MATT_MPS, MPS_MAT, MPS_MATT, MPS_REDOP, REDOP_MPS, TM_LEFT_UNITY,
                TM_RIGHT_UNITY, BlockLayout, Context, EnvPlan, HACPlan, HCPlan, ProdPlan, QRPlan, RedOpPlan, TMPlan, eigsh,
                make_dims, model_d, model_ham, model_nsym, mps_structure)
+
+
+def lower_sqrt_factor(r: np.ndarray, lay: BlockLayout) -> np.ndarray:
+    """Per sector of the symmetric positive semi-definite block-diagonal R (column-major blocks),
+    the lower-triangular C with positive diagonal and C C^T = R: C = RQ factor of U sqrt(lam), with
+    R = U lam U^T (svd(U, DR, V, R); qr(C, U << sqrt(DR), r) in randMPS_LR)."""
+    out = np.zeros(lay.nelem)
+    for o, s in zip(lay.offsets, lay.rows):
+        m = r[o:o + s * s].reshape(s, s, order="F")
+        U, lam, _ = np.linalg.svd(0.5 * (m + m.T))
+        _, t = np.linalg.qr((U * np.sqrt(lam)).T)
+        t = np.where(np.diag(t) < 0, -1.0, 1.0)[:, None] * t
+        out[o:o + s * s] = t.T.reshape(-1, order="F")
+    return out
 
 
 @dataclass
@@ -60,21 +78,32 @@ class VUMPS:
         self.eig_maxit, self.env_krylov, self.env_maxit = eig_maxit, env_krylov, env_maxit
         self.dosvd, self.svdthresh = polar_start, 5e-7  # polar (SVD) scheme until the gauge error is below svdthresh
         self._plans: Dict = {}
-        rng = np.random.default_rng(seed)
-        self.AL, self.AR, self.C = [], [], []
-        for n in range(N):
-            A = self._dev(rng.standard_normal(self.slay[n].nelem))
-            self.AL.append(self._qr(A, self.slay[n], LEFT)[0])
-            A = self._dev(rng.standard_normal(self.slay[n].nelem))
-            self.AR.append(self._qr(A, self.slay[n], RIGHT)[0])
-        for n in range(N):  # C[n] on bond n+1: identity / sqrt(D), so that tr(C C^T) = 1
-            self.C.append(self._identity((n + 1) % N, 1.0 / np.sqrt(self.bonds[(n + 1) % N][1].sum())))
+        self._random_state(np.random.default_rng(seed))
         self.prec_n = np.ones(N)
         self.LE: Dict[int, object] = {}
         self.RE: Dict[int, object] = {}
         self.eig_iters = 0
         self.env_iters = 0
         self._init_environments()
+
+    def _random_state(self, rng):
+        """randMPS_LR (MPSBlockUtilities.hpp:1095-1133): left-orthonormal AL[n] from the QR of Gaussian
+        tensors, R = the trace-1 dominant right fixed point of the cell TM on bond 0 (TMDominantEig),
+        C[N-1] = the lower-triangular factor with C C^T = R, then AR[n] and C[n-1] from the right QR
+        of AL[n] C[n] for n = N-1, ..., 0, so that AL[n] C[n] = C[n-1] AR[n] on every site but the
+        last QR's leftover factor (discarded, as in the reference)."""
+        N = self.N
+        self.AL = [self._qr(self._dev(rng.standard_normal(self.slay[n].nelem)), self.slay[n], LEFT)[0] for n in range(N)]
+        pr = EnvPlan(self.ctx, RIGHT, self.slay, self.blay[0], FP64)
+        pr.set_mps(self.AL)
+        R = self._identity(0, 1.0 / float(self.bonds[0][1].sum()))
+        pr.fixed_point(R, tol=1e-14, maxit=self.env_maxit)
+        C = self._dev(lower_sqrt_factor(R.cpu().numpy(), self.blay[0]))
+        self.C, self.AR = [None] * N, [None] * N
+        for n in range(N - 1, -1, -1):
+            self.C[n] = C
+            X, _ = self._prod(MPS_MAT, self.slay[n], self.AL[n], self.blay[self._b(n + 1)], C, ("init", n))
+            self.AR[n], C = self._qr(X, self.slay[n], RIGHT)
 
     # ------------------------------------------------------------------ helpers
     def _dev(self, a):

@@ -4,8 +4,9 @@ Restates the outer loop of VUMPS_main.cpp:165-354 at fixed bond structure with t
 H_AC / H_C (VUMPS_helpers.hpp:38-133, H_C as the reference's per-nonzero five-factor chain),
 InvertE_proj + unrestarted GMRES (InvEBlockFunctions.hpp:205-266, IterativeSolvers.hpp:21-149),
 GetHL / GetHR, qr with qr_pos, and ARPACK (scipy's black-box library routine) for the "SA"
-eigenpairs (eigs_rs) and the TM dominant eigenvector (TMDominantEig).  The initial state is built
-from the same seeded random arrays as imps_cpp_tools_amd.vumps.VUMPS, so both start identically.
+eigenpairs (eigs_rs) and the TM dominant eigenvector (TMDominantEig).  The initial state is the
+reference's random mixed-canonical MPS (randMPS_LR) from the same seeded random arrays as
+imps_cpp_tools_amd.vumps.VUMPS, so both start from the same state.
 """
 from __future__ import annotations
 
@@ -33,6 +34,16 @@ def _transpose_data(ob: OB) -> np.ndarray:
     return np.concatenate(parts)
 
 
+def _lower_factor(R: OB) -> np.ndarray:
+    """Data of the block-diagonal C with C C^T = R, each block lower triangular with positive
+    diagonal (Cholesky factor of the symmetrised block; R is positive definite for a random AL)."""
+    parts = []
+    for _, _, _, m in R.blocks():
+        c = np.linalg.cholesky(0.5 * (m + m.T))
+        parts.append(c.reshape(-1, order="F"))
+    return np.concatenate(parts)
+
+
 class OracleVUMPS:
     def __init__(self, model, N, QN, keys0, sizes0, params, seed=0, eig_tol_max=1e-6, eig_tol_min=5e-16, env_tol=5e-15, env_fac=2.0):
         self.N = N
@@ -42,14 +53,17 @@ class OracleVUMPS:
         self.struct = [orc.mps_structure(model, N, QN, self.bonds[n], self.bonds[(n + 1) % N]) for n in range(N)]
         self.H = orc.model_ham(model, params)
         self.eig_tol_max, self.eig_tol_min, self.env_tol, self.env_fac = eig_tol_max, eig_tol_min, env_tol, env_fac
+        # randMPS_LR (MPSBlockUtilities.hpp:1095-1133): AL from QR of Gaussian tensors, R the trace-1
+        # dominant right fixed point on bond 0, C C^T = R with C lower triangular, then
+        # AL[n] C[n] = C[n-1] AR[n] by right QRs from the last site down
         rng = np.random.default_rng(seed)
-        self.AL, self.AR, self.C = [], [], []
-        for n in range(N):
-            self.AL.append(orc.mps_qr(_like(self.struct[n], rng.standard_normal(self.struct[n].nelem)), 0)[0])
-            self.AR.append(orc.mps_qr(_like(self.struct[n], rng.standard_normal(self.struct[n].nelem)), 1)[0])
-        for n in range(N):
-            b = (n + 1) % N
-            self.C.append(self.identity(b, 1.0 / np.sqrt(self.bonds[b][1].sum())))
+        self.AL = [orc.mps_qr(_like(self.struct[n], rng.standard_normal(self.struct[n].nelem)), 0)[0] for n in range(N)]
+        R = self.fixed_point(1, self.AL, 0, self.identity(0, 1.0 / float(self.bonds[0][1].sum())))
+        C = _like(self.diag(0), _lower_factor(R))
+        self.C, self.AR = [None] * N, [None] * N
+        for n in range(N - 1, -1, -1):
+            self.C[n] = C
+            self.AR[n], C = orc.mps_qr(orc.mps_mat(self.AL[n], C), 1)
         self.prec = np.ones(N)
         self.LE, self.RE = {}, {}
         for b in range(N):

@@ -89,9 +89,11 @@ def test_invert_e_proj_fac_reduces_to_invert_e_proj():
     Rob = ex.to_ob([R], 1)
     xob = orc.OB.like(Rob, rng.standard_normal(Rob.nelem))
     one = ex.eye(R)
-    y1, flag = ex.invert_e_proj_fac(s["AL"], s["AL"], ex.from_ob(xob)[0], one, R, 0, 1.0, 1e-14)
+    # 1e-13: on this 44-dimensional system the unrestarted GMRES of both oracles reaches its full
+    # Krylov dimension at 1e-14 without meeting the round-off-limited residual test
+    y1, flag = ex.invert_e_proj_fac(s["AL"], s["AL"], ex.from_ob(xob)[0], one, R, 0, 1.0, 1e-13)
     assert flag > 0
-    y2, _ = orc.invert_e_proj(0, s["ALob"], xob, ex.to_ob([one], 1), Rob, tol=1e-14)
+    y2, _ = orc.invert_e_proj(0, s["ALob"], xob, ex.to_ob([one], 1), Rob, tol=1e-13)
     assert np.linalg.norm(ex.vectorize(y1) - _ob_vec(y2)) / np.linalg.norm(_ob_vec(y2)) < 1e-10
 
 

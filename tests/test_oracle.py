@@ -375,3 +375,29 @@ def test_oracle_vumps_first_sweeps():
     assert es[-1] < es[0]
     assert max(hist[-1][1]) < max(hist[0][1])
     assert es[-1] > 0.25 - np.log(2.0)
+
+
+def test_oracle_random_state_is_mixed_canonical():
+    """randMPS_LR restated in the oracle VUMPS: AL left- and AR right-orthonormal, C lower triangular
+    with positive diagonal and tr(C C^T) = 1, AL[n] C[n] = C[n-1] AR[n] on every site, R = C C^T the
+    right fixed point of the AL cell (so the last right QR returns C[N-1] itself)."""
+    import json
+    import os
+    from oracle_vumps import OracleVUMPS
+    with open(os.path.join(os.path.dirname(__file__), "golden", "vumps_xxz_d64.json")) as f:
+        g = json.load(f)
+    v = OracleVUMPS(0, g["N"], g["QN"], np.array(g["keys0"], np.int32), np.array(g["sizes0"], np.int32), g["params"],
+                    seed=g["seed"])
+    N = g["N"]
+    for n in range(N):
+        for direction, A in ((0, v.AL[n]), (1, v.AR[n])):
+            for _, _, _, m in orc.tm_unity(direction, A).blocks():
+                assert np.abs(m - np.eye(m.shape[0])).max() < 1e-13
+        tr = 0.0
+        for _, _, _, m in v.C[n].blocks():
+            assert np.abs(np.triu(m, 1)).max() == 0.0 and (np.diag(m) > 0).all()
+            tr += float((m * m).sum())
+        assert abs(tr - 1.0) < 1e-13
+        lhs = orc.mps_mat(v.AL[n], v.C[n]).data
+        rhs = orc.mat_mps(v.C[(n - 1) % N], v.AR[n]).data
+        assert np.abs(lhs - rhs).max() < 1e-12
