@@ -9,7 +9,9 @@ the GPU by ReducedOp).
 
 Prints ONE JSON line on stdout.  The value is vector-applies/s of the mixed-precision path
 (split-BF16 on MFMA, FP32 accumulation); FP64 and FP32 rates, single-vector rates, the roofline of
-the grouped kernel and the CPU oracle timed on the host (cpu_baseline) are reported alongside.
+the panel GEMM kernel (its own launch time from HIP events on the launch stream), VUMPS iteration
+times (Heisenberg D = 64 to convergence; Hubbard D = 1024 first iteration) and the CPU oracle timed
+on the host (cpu_baseline) are reported alongside.
 """
 import argparse
 import json
@@ -24,6 +26,22 @@ sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 157.3}  # bf16x3 is priced against FP32 MFMA
 HBM_PEAK_GBS = 8000.0
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "panel_gemm_kernel_hac_hubbard_d1024.md")
+
+
+def committed_traffic(prec_name: str, nvec: int):
+    """HBM bytes per launch of the panel GEMM kernel from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE summary (copy-calibrated), for this precision and batch; None if not recorded."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    key = f"traffic {prec_name} nvec {nvec}:"
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if line.startswith(key):
+                return float(line.split(":")[1].split()[0])
+    return None
 
 
 def log(*a):
@@ -112,20 +130,28 @@ def main():
             torch.cuda.synchronize()
             wall = time.perf_counter() - t0
             ms = e0.elapsed_time(e1) / args.steps
+            # the GEMM kernels alone: HIP events recorded by the library on this stream around the
+            # panel-GEMM launches of each apply (the split copy of the input stays outside)
+            p.timing(True)
+            for _ in range(args.steps):
+                p.apply(X, Y, nv)
+            kms, kcount = p.timing(False)
             r[label] = dict(nvec=nv, ms_per_launch=ms, applies_per_s=nv / (ms * 1e-3), wall_s=wall,
-                            tflops=nv * flops / (ms * 1e-3) / 1e12)
+                            tflops=nv * flops / (ms * 1e-3) / 1e12, kernel_us=kms / kcount * 1e3,
+                            kernel_tflops=nv * flops / (kms / kcount * 1e-3) / 1e12)
         r["flops_per_apply"] = flops
         r["bytes_per_apply"] = bytes_
         r["y"] = Y[:n].cpu().numpy()
         results[name] = r
         log(f"{name}: batched {r['batched']['applies_per_s']:.4g} applies/s ({r['batched']['tflops']:.3g} TF/s), "
-            f"single {r['single']['applies_per_s']:.4g} applies/s, {r['batched']['ms_per_launch'] * 1e3:.1f} us/launch")
+            f"single {r['single']['applies_per_s']:.4g} applies/s, {r['batched']['ms_per_launch'] * 1e3:.1f} us/apply, "
+            f"GEMM kernels {r['batched']['kernel_us']:.1f} us/apply")
 
     head = "bf16x3" if "bf16x3" in results else next(iter(results))
     mixed = results[head]
     rate = lambda k, lab="batched": results[k][lab]["applies_per_s"] if k in results else None  # noqa: E731
     flops = mixed["flops_per_apply"]
-    achieved_tf = mixed["batched"]["tflops"]
+    achieved_tf = mixed["batched"]["kernel_tflops"]
     out = {
         "metric": "H_AC applies/s & VUMPS iter time, D=1024 Hubbard: mixed vs FP64, same GPU",
         "value": mixed["batched"]["applies_per_s"],
@@ -147,12 +173,13 @@ def main():
         "single_vector_applies_per_s": {k: results[k]["single"]["applies_per_s"] for k in results},
         "vumps_iter_s": None,
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
-                     "frac": achieved_tf / PEAK_TFLOPS[head], "traffic": None,
-                     "kernel": f"grouped_gemm_kernel<{head.upper()},double>",
-                     "per_launch": {"flops": NV * flops, "us": mixed["batched"]["ms_per_launch"] * 1e3}},
+                     "frac": achieved_tf / PEAK_TFLOPS[head], "traffic": committed_traffic(head, NV),
+                     "kernel": f"panel_gemm_kernel<{head.upper()}, K-split {1 if NV >= 4 else (2 if NV >= 2 else 4)}>",
+                     "per_launch": {"flops": NV * flops, "us": mixed["batched"]["kernel_us"],
+                                    "apply_us_incl_split_copy": mixed["batched"]["ms_per_launch"] * 1e3}},
     }
     if "fp64" in results:
-        out["roofline"]["fp64_frac_of_fp64_peak"] = results["fp64"]["batched"]["tflops"] / PEAK_TFLOPS["fp64"]
+        out["roofline"]["fp64_frac_of_fp64_peak"] = results["fp64"]["batched"]["kernel_tflops"] / PEAK_TFLOPS["fp64"]
 
     # VUMPS sweep time on the two-site Heisenberg chain, D = 64 (9 U(1) sectors, seed 1001)
     if not args.no_vumps:
@@ -164,7 +191,7 @@ def main():
             v = VUMPS(ctx, im.XXZ, 2, [0], keys0, sizes0, [1.0, -1.0, 0.0], prec=precs[name], seed=1001)
             t0 = time.perf_counter()
             sweeps, st = 0, None
-            while sweeps < 200:
+            while sweeps < 400:
                 st = v.sweep()
                 sweeps += 1
                 if max(st.F) <= 1e-10:
@@ -174,6 +201,29 @@ def main():
             vres[name] = dict(sweeps=sweeps, s_per_sweep=dt / sweeps, seconds_to_F=dt, F=max(st.F), energy=st.energy)
             log(f"vumps xxz D=64 {name}: {sweeps} sweeps, {dt / sweeps * 1e3:.1f} ms/sweep, F {max(st.F):.2e}, E {st.energy:.15f}")
         out["vumps_xxz_d64"] = vres
+        # one outer iteration on the D = 1024 Hubbard structure from the seeded random mixed-canonical
+        # state (randMPS_LR), solves to the reference's tolerances; plans built beforehand by the
+        # same iteration on an identical state
+        k0, s0 = wl.fhub_profile(cfg["nside"], cfg["peak"], 1, cfg["total"])
+        hres = {}
+        for name in [k for k in ("fp64", "bf16x3") if k in results]:
+            mk = lambda plans: VUMPS(ctx, cfg["model"], cfg["N"], cfg["QN"], k0, s0, cfg["params"], prec=precs[name],  # noqa: E731
+                                     seed=1003, krylov=30, env_krylov=300, plans=plans)
+            w = mk(None)
+            w.sweep()
+            v = mk(w._plans)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = v.sweep()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            hres[name] = dict(s_per_iter=dt, energy=st.energy, F=max(st.F), eig_iters=st.eig_iters, env_iters=st.env_iters)
+            log(f"vumps hubbard D=1024 {name}: first iteration {dt:.3f} s ({st.eig_iters} Lanczos, {st.env_iters} GMRES steps), "
+                f"E {st.energy:.12f}, F {max(st.F):.2e}")
+        out["vumps_hubbard_d1024_first_iter"] = hres
+        if "fp64" in hres and head in hres:
+            out["vumps_iter_s"] = {"fp64": hres["fp64"]["s_per_iter"], head: hres[head]["s_per_iter"],
+                                   "mixed_vs_fp64": hres["fp64"]["s_per_iter"] / hres[head]["s_per_iter"]}
 
     # CPU baseline: the oracle (clean-room FP64 restatement of ApplyHA) on the host, bounded sample
     if not args.no_cpu:

@@ -68,6 +68,7 @@ def lib():
             "imps_synchronize": (I, [P]),
             "imps_plan_destroy": (I, [P]),
             "imps_plan_cost": (I, [P, dp, dp]),
+            "imps_plan_timing": (I, [P, I, dp, ip]),
             "imps_plan_shape": (I, [P, ip, ip]),
             "imps_plan_out_layout": (I, [P, ip, ip, ctypes.POINTER(ctypes.c_long), ip, ip, ip, ip, ip]),
             "imps_tm_plan": (I, [P, ctypes.POINTER(P), I, LP, LP, LP, LP, I]),
@@ -340,6 +341,13 @@ class Plan:
         f, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().imps_plan_cost(self.h, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+    def timing(self, enable: bool):
+        """GPU ms of the GEMM kernels of the applies timed since the previous call and their count;
+        then switches the per-apply event timing on or off (imps_plan_timing)."""
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _check(lib().imps_plan_timing(self.h, int(enable), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def shape(self):
         a, t = ctypes.c_int(), ctypes.c_int()

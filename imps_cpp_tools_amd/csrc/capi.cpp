@@ -124,6 +124,17 @@ imps_status imps_plan_destroy(imps_plan plan) {
     });
 }
 
+imps_status imps_plan_timing(imps_plan P, int enable, double* ms, int* launches) {
+    return guarded([&] {
+        if (!P) throw InvalidArg("plan is NULL");
+        if (P->ctx) hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+        const double t = take_timing(P, launches);
+        if (ms) *ms = t;
+        P->timing = enable != 0;
+        return IMPS_OK;
+    });
+}
+
 imps_status imps_plan_cost(imps_plan P, double* flops, double* bytes) {
     return guarded([&] {
         if (!P) throw InvalidArg("plan is NULL");

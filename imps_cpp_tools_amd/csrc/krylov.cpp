@@ -11,9 +11,30 @@ namespace imps {
 
 namespace {
 
+// Pinned host staging for device-to-host reads (one buffer per host thread, grown on demand and
+// kept for the life of the process): the copy is a direct DMA instead of a staged pageable copy.
+struct PinnedStage {
+    double* p = nullptr;
+    size_t cap = 0;
+    double* get(size_t n) {
+        if (n > cap) {
+            if (p) hip_check(hipHostFree(p), "hipHostFree");
+            p = nullptr;
+            const size_t c = std::max<size_t>(n, 4096);
+            hip_check(hipHostMalloc((void**)&p, c * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+            cap = c;
+        }
+        return p;
+    }
+};
+thread_local PinnedStage pinned_stage;
+
 void d2h(hipStream_t s, double* h, const double* d, size_t n) {
-    hip_check(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    if (n == 0) return;
+    double* p = pinned_stage.get(n);
+    hip_check(hipMemcpyAsync(p, d, n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
     hip_check(hipStreamSynchronize(s), "sync");
+    std::copy(p, p + n, h);
 }
 // synchronous: the host source may be a short-lived temporary
 void h2d(hipStream_t s, double* d, const double* h, size_t n) {
@@ -34,7 +55,7 @@ struct Work {
         m = m_;
         V.ensure(sizeof(double) * (size_t)n * (m + 1));
         w.ensure(sizeof(double) * (size_t)std::max<long>(n, 1));
-        dh.ensure(sizeof(double) * (size_t)(2 * (m + 2)));
+        dh.ensure(sizeof(double) * (size_t)(2 * (m + 2)));  // >= 2k + 1 for k <= m + 1 (cgs2)
         scratch.ensure(sizeof(double) * blas_scratch_doubles(m + 2));
     }
     double* Vi(int i) const { return V.d() + (size_t)i * n; }
@@ -47,16 +68,21 @@ double dev_norm(hipStream_t s, Work& W, const double* v) {
     return std::sqrt(std::max(h, 0.0));
 }
 
-// w <- w - V[0..k) (V^T w), twice (CGS2); returns the k projection coefficients (summed)
-void cgs2(hipStream_t s, Work& W, int k, double* w, std::vector<double>& h) {
-    h.assign(k, 0.0);
-    std::vector<double> t(k);
+// w <- w - V[0..k) (V^T w), twice (CGS2); h = the k projection coefficients (summed over both
+// passes); returns ||w|| afterwards.  Coefficients stay on the device between the passes, so the
+// step costs one device-to-host transfer (2k + 1 values).
+double cgs2(hipStream_t s, Work& W, int k, double* w, std::vector<double>& h) {
+    double* dh = W.dh.d();
     for (int pass = 0; pass < 2; ++pass) {
-        hip_check(blas_multidot(W.V.d(), W.n, k, w, W.n, W.dh.d(), W.scratch.d(), s), "multidot");
-        hip_check(blas_gemv_sub(W.V.d(), W.n, k, W.dh.d(), w, W.n, s), "gemv");
-        d2h(s, t.data(), W.dh.d(), k);
-        for (int i = 0; i < k; ++i) h[i] += t[i];
+        hip_check(blas_multidot(W.V.d(), W.n, k, w, W.n, dh + pass * k, W.scratch.d(), s), "multidot");
+        hip_check(blas_gemv_sub(W.V.d(), W.n, k, dh + pass * k, w, W.n, s), "gemv");
     }
+    hip_check(blas_multidot(w, 0, 1, w, W.n, dh + 2 * k, W.scratch.d(), s), "norm");
+    std::vector<double> t(2 * k + 1);
+    d2h(s, t.data(), dh, 2 * k + 1);
+    h.resize(k);
+    for (int i = 0; i < k; ++i) h[i] = t[i] + t[k + i];
+    return std::sqrt(std::max(t[2 * k], 0.0));
 }
 
 // Cyclic Jacobi eigen-decomposition of a symmetric n x n matrix (column-major, FP64).
@@ -128,8 +154,7 @@ KrylovStats gmres(hipStream_t s, const LinOp& A, const double* b, double* x, dou
             double* w = W.Vi(j + 1);
             A.apply(W.Vi(j), w);
             ++st.iters;
-            cgs2(s, W, j + 1, w, h);
-            const double hn = dev_norm(s, W, w);
+            const double hn = cgs2(s, W, j + 1, w, h);
             for (int i = 0; i <= j; ++i) H[i + (size_t)j * (m + 1)] = h[i];
             H[j + 1 + (size_t)j * (m + 1)] = hn;
             for (int i = 0; i < j; ++i) {  // previous Givens rotations
@@ -192,8 +217,9 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
         st.converged = true;
         return st;
     }
-    // Each inner solve only needs to reduce the residual by a constant factor; it is capped at
-    // 4 restart cycles so that stagnation in low precision cannot use up the budget.
+    // Each inner solve only needs to reduce the residual by a constant factor; it is capped at one
+    // Krylov cycle: in low precision the residual estimate of an ill-conditioned operator
+    // stagnates near eps_low * cond, and only the FP64 residual of the next round can tell.
     while (st.iters < maxit) {
         A64.apply(x, r.d());
         ++st.iters;
@@ -203,7 +229,7 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
         if (st.relres < tol) { st.converged = true; break; }
         if (st.iters >= maxit) break;
         hip_check(launch_zero(dx.d(), n, s), "zero");
-        const int budget = std::min(maxit - st.iters, 4 * (m + 1));
+        const int budget = std::min(maxit - st.iters, m + 1);
         KrylovStats in = gmres(s, Alo, r.d(), dx.d(), std::max(inner_tol, 0.5 * tol * bnorm / rn), budget, m);
         st.iters += in.iters;
         hip_check(blas_axpby(1.0, nullptr, false, dx.d(), 1.0, x, n, s), "update");
@@ -236,8 +262,7 @@ KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol,
             double* wv = W.Vi(j + 1);
             A.apply(W.Vi(j), wv);
             ++st.iters;
-            cgs2(s, W, j + 1, wv, h);
-            beta = dev_norm(s, W, wv);
+            beta = cgs2(s, W, j + 1, wv, h);
             for (int i = 0; i <= j; ++i) { T[i + (size_t)j * m] = h[i]; T[j + (size_t)i * m] = h[i]; }
             k = j + 1;
             if (beta <= 1e-14 * std::max(std::abs(h[j]), 1e-300) || j + 1 == m || st.iters >= maxit) break;
@@ -405,10 +430,11 @@ KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, 
                 in = o;
                 if (k < ncell - 1) off += bond_elems[k];
             }
-            const double tr = trace_in(v);
+            // tr(v rho_in) stays on the device and scales rho_out there (no host round trip)
+            hip_check(blas_multidot(rhoT.d(), 0, 1, v, n, dd.d(), dd.d() + 4, s), "trace");
             d2d(s, out, v, n);
             hip_check(blas_axpby(-1.0, nullptr, false, tmp.d(), 1.0, out, n, s), "1-T");
-            hip_check(blas_axpby(tr, nullptr, false, rho_out, 1.0, out, n, s), "projector");
+            hip_check(blas_axpby(1.0, dd.d(), false, rho_out, 1.0, out, n, s), "projector");
         };
         return op;
     };

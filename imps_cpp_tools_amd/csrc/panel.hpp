@@ -72,10 +72,12 @@ struct PanelDesc {    // one packed operator block
     double alpha;
 };
 
-struct SplitBlock {   // one block of the input vector that is read as a raw factor
-    long off;
+struct SplitBlock {   // one tile (at most SPLIT_TILE x SPLIT_TILE) of an input-vector block that is
+    long off;         // read as a raw factor: block offset and shape, tile origin (r0, c0)
     int rows, cols;
+    int r0, c0;
 };
+constexpr int SPLIT_TILE = 64;
 
 struct PStage {
     int out_slot = SLOT_OUT;
@@ -87,7 +89,7 @@ struct PanelProgram {
     int prec = IMPS_FP64;
     std::vector<PStage> stages;
     std::vector<PanelDesc> panels;
-    std::vector<SplitBlock> xblocks;  // input-vector blocks to split per apply
+    std::vector<SplitBlock> xblocks;  // tiles of the input-vector blocks to split per apply
     long nfrag = 0;                   // panel fragments in total
     long plx = 0, plw = 0;            // split plane lengths of the input vector and the workspace
 };
@@ -156,7 +158,7 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                 const bool kmajor = pa ? (rtr == 0) : (rtr != 0);
                 const int idx0 = pa ? t.n0 : t.m0;
                 const long pl = rslot == SLOT_X ? Q.plx : Q.plw;
-                if (rslot == SLOT_X) xb[boff] = SplitBlock{boff, Rb, Cb};
+                if (rslot == SLOT_X) xb[boff] = SplitBlock{boff, Rb, Cb, 0, 0};
                 const int s_idx = kmajor ? Rb : Cb;
                 const long base = kmajor ? boff : pl + boff;
                 const int rg0 = (pa ? t.m0 : t.n0) / 16;
@@ -187,7 +189,14 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
         }
         Q.stages.push_back(std::move(ps));
     }
-    for (auto& kv : xb) Q.xblocks.push_back(kv.second);
+    for (auto& kv : xb)
+        for (int r0 = 0; r0 < kv.second.rows; r0 += SPLIT_TILE)
+            for (int c0 = 0; c0 < kv.second.cols; c0 += SPLIT_TILE) {
+                SplitBlock t = kv.second;
+                t.r0 = r0;
+                t.c0 = c0;
+                Q.xblocks.push_back(t);
+            }
     if (Q.nfrag * panel_frag_bytes(prec) >= (1L << 31)) throw InvalidArg("panel store too large for 32-bit offsets");
     return Q;
 }

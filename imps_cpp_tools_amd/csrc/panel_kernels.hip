@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "engine.hpp"
 #include "panel.hpp"
@@ -160,19 +161,66 @@ __device__ __forceinline__ void put_split(void* __restrict__ sp, long pl, long n
     }
 }
 
-// grid (nblocks, nvec): one workgroup per (block, vector).  Element (r, c) of a block: plane N
-// at off + r + c*rows (as stored), plane T at off + c + r*cols.
+// The two halves of put_split: plane N (as stored) and plane T (transposed).
 template <int PREC>
-__global__ __launch_bounds__(256) void split_kernel(const SplitBlock* __restrict__ blocks, const double* __restrict__ x, long x_stride,
+__device__ __forceinline__ void put_split_n(void* __restrict__ sp, long pl, long n_at, double v) {
+    if constexpr (PREC == IMPS_FP64) {
+        static_cast<double*>(sp)[n_at] = v;
+    } else if constexpr (PREC == IMPS_FP32) {
+        static_cast<float*>(sp)[n_at] = (float)v;
+    } else {
+        __bf16* p = static_cast<__bf16*>(sp);
+        __bf16 h, l;
+        split_bf16((float)v, h, l);
+        p[n_at] = h;
+        p[2 * pl + n_at] = l;
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ void put_split_t(void* __restrict__ sp, long pl, long t_at, double v) {
+    if constexpr (PREC == IMPS_FP64) {
+        static_cast<double*>(sp)[pl + t_at] = v;
+    } else if constexpr (PREC == IMPS_FP32) {
+        static_cast<float*>(sp)[pl + t_at] = (float)v;
+    } else {
+        __bf16* p = static_cast<__bf16*>(sp);
+        __bf16 h, l;
+        split_bf16((float)v, h, l);
+        p[pl + t_at] = h;
+        p[3 * pl + t_at] = l;
+    }
+}
+
+// grid (ntiles, nvec): one workgroup per (tile of at most 64 x 64 block elements, vector).  Element
+// (r, c) of a block goes to plane N at off + r + c*rows (as stored) and to plane T at
+// off + c + r*cols.  The tile is read column by column (lanes along r) and written to plane N in
+// the same order; plane T is written row by row (lanes along c) from an LDS copy, so that both
+// planes are stored in contiguous runs.
+template <int PREC>
+__global__ __launch_bounds__(256) void split_kernel(const SplitBlock* __restrict__ tiles, const double* __restrict__ x, long x_stride,
                                                     char* __restrict__ split, long split_stride, long pl) {
-    const SplitBlock b = blocks[blockIdx.x];
+    using T = typename std::conditional<PREC == IMPS_FP64, double, float>::type;
+    __shared__ T tile[SPLIT_TILE][SPLIT_TILE + 1];
+    const SplitBlock b = tiles[blockIdx.x];
     const long v = blockIdx.y;
     const double* xv = x + x_stride * v;
     void* sp = split + split_stride * v;
-    const int n = b.rows * b.cols;
-    for (int e = threadIdx.x; e < n; e += blockDim.x) {
-        const int r = e % b.rows, c = e / b.rows;
-        put_split<PREC>(sp, pl, b.off + e, b.off + c + (long)r * b.cols, xv[b.off + e]);
+    const int tr = min(SPLIT_TILE, b.rows - b.r0), tc = min(SPLIT_TILE, b.cols - b.c0);
+    const int lane = threadIdx.x % SPLIT_TILE, grp = threadIdx.x / SPLIT_TILE;
+    constexpr int NG = 256 / SPLIT_TILE;
+    if (lane < tr) {
+        for (int j = grp; j < tc; j += NG) {
+            const long at = b.off + (b.r0 + lane) + (long)(b.c0 + j) * b.rows;
+            const T val = (T)xv[at];
+            tile[lane][j] = val;
+            put_split_n<PREC>(sp, pl, at, (double)val);
+        }
+    }
+    __syncthreads();
+    if (lane < tc) {
+        for (int i = grp; i < tr; i += NG)
+            put_split_t<PREC>(sp, pl, b.off + (b.c0 + lane) + (long)(b.r0 + i) * b.cols, (double)tile[i][lane]);
     }
 }
 

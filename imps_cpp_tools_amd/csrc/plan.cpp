@@ -4,6 +4,10 @@
 #include <cstdlib>
 #include <cstring>
 
+imps_plan_s::~imps_plan_s() {
+    for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+}
+
 namespace imps {
 
 void ctx_release(imps_context ctx) {
@@ -140,6 +144,17 @@ void set_operator(imps_plan_s* P, const std::vector<const double*>& parts) {
     P->op_set = true;
 }
 
+// records the next timing event of an apply on stream s (no-op unless the plan's timing is on)
+static void time_mark(imps_plan_s* P, hipStream_t s) {
+    if (!P->timing) return;
+    if (P->tev_used == P->tev.size()) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        P->tev.push_back(e);
+    }
+    hip_check(hipEventRecord(P->tev[P->tev_used++], s), "hipEventRecord");
+}
+
 void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_override) {
     if (!P->ctx) throw InvalidArg("plan was created without a context");
     if (nvec < 1) throw InvalidArg("nvec must be >= 1");
@@ -175,7 +190,9 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
         L.out_stride = P->prog.y_elems;
         L.nvec = nvec;
         const int ks = panel_ksplit(nvec);
+        time_mark(P, s);
         for (auto& ds : P->dpstages) hip_check(launch_panel_stage(P->prec, ds, ks, L, s), "panel gemm");
+        time_mark(P, s);
         return;
     }
     P->d_w.ensure((size_t)std::max<long>(P->prog.w_elems, 1) * nvec * P->elem_bytes());
@@ -184,12 +201,30 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
     sl.base[SLOT_OP] = op_override ? (const void*)op_override : P->d_op.p;
     sl.stride[SLOT_OP] = 0;
     sl.base[SLOT_W] = P->d_w.p;    sl.stride[SLOT_W] = P->prog.w_elems;
+    time_mark(P, s);
     for (auto& ds : P->dstages) {
         if (ds.out_slot == SLOT_W)
             hip_check(launch_stage(P->prec, ds, sl, P->d_w.p, P->prog.w_elems, P->prec == IMPS_FP64, nvec, s), "grouped gemm");
         else
             hip_check(launch_stage(P->prec, ds, sl, y, P->prog.y_elems, true, nvec, s), "grouped gemm");
     }
+    time_mark(P, s);
+}
+
+double take_timing(imps_plan_s* P, int* launches) {
+    double ms = 0.0;
+    const size_t pairs = P->tev_used / 2;
+    if (pairs > 0) {
+        hip_check(hipEventSynchronize(P->tev[2 * pairs - 1]), "timing event wait");
+        for (size_t i = 0; i < pairs; ++i) {
+            float e = 0.0f;
+            hip_check(hipEventElapsedTime(&e, P->tev[2 * i], P->tev[2 * i + 1]), "timing event elapsed");
+            ms += e;
+        }
+    }
+    if (launches) *launches = (int)pairs;
+    P->tev_used = 0;
+    return ms;
 }
 
 }  // namespace imps

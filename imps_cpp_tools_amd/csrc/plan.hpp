@@ -62,10 +62,16 @@ struct imps_plan_s {
     const imps::SplitBlock* d_xblocks = nullptr;
     std::vector<imps::DevPStage> dpstages;
     long xs_ready = 0, ws_ready = 0;  // vectors of the split buffers already zero-filled
+    // opt-in GPU timing of the GEMM launches of each apply (events recorded on the plan's stream
+    // around the grouped/panel kernels, split copies excluded); pairs are read by imps_plan_timing
+    bool timing = false;
+    std::vector<hipEvent_t> tev;
+    size_t tev_used = 0;
     std::shared_ptr<imps::EnvSolver> env;  // K_ENV
     imps::QrPlanData qr;                   // K_QR
     int qr_dir = 0;
     size_t elem_bytes() const { return prec == IMPS_FP64 ? 8 : 4; }
+    ~imps_plan_s();
 };
 
 namespace imps {
@@ -76,5 +82,7 @@ void set_operator(imps_plan_s* P, const std::vector<const double*>& parts);
 // y = op(x) for nvec vectors on the context stream; op_override (FP64 plans) replaces the operator
 // store by a caller buffer (two-operand products)
 void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_override = nullptr);
+// GPU milliseconds of the timed GEMM launches since the last call (waits for them), then resets
+double take_timing(imps_plan_s* P, int* launches);
 
 }  // namespace imps

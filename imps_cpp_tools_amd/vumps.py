@@ -65,7 +65,10 @@ class VumpsStats:
 class VUMPS:
     def __init__(self, ctx: Context, model: int, N: int, QN, keys0, sizes0, params, prec: int = FP64, seed: int = 0,
                  eig_tol_max: float = 1e-6, eig_tol_min: float = 5e-16, env_tol: float = 5e-15, env_fac: float = 2.0,
-                 krylov: int = 20, eig_maxit: int = 100000, env_krylov: int = 100, env_maxit: int = 20000, polar_start: bool = True):
+                 krylov: int = 20, eig_maxit: int = 100000, env_krylov: int = 100, env_maxit: int = 20000, polar_start: bool = True,
+                 plans: Optional[Dict] = None):
+        """`plans`: the plan cache of another VUMPS over the same model and bond structure, reused
+        (plans hold structure only; every use sets its operator data)."""
         import torch
         self.torch = torch
         self.ctx, self.prec, self.N = ctx, prec, N
@@ -77,7 +80,8 @@ class VUMPS:
         self.eig_tol_max, self.eig_tol_min, self.env_tol, self.env_fac, self.krylov = eig_tol_max, eig_tol_min, env_tol, env_fac, krylov
         self.eig_maxit, self.env_krylov, self.env_maxit = eig_maxit, env_krylov, env_maxit
         self.dosvd, self.svdthresh = polar_start, 5e-7  # polar (SVD) scheme until the gauge error is below svdthresh
-        self._plans: Dict = {}
+        self._plans: Dict = {} if plans is None else plans
+        self._units: Dict = {}
         self._random_state(np.random.default_rng(seed))
         self.prec_n = np.ones(N)
         self.LE: Dict[int, object] = {}
@@ -191,9 +195,13 @@ class VUMPS:
         return h
 
     def _env_plan(self, direction, b, As_layouts_key):
+        """Environment solver of the cell from site b, FP64 at every working precision: the
+        geometric sum's operator 1 - T has the TM gap as its smallest scale, so a low-precision T
+        gains nothing on a single right-hand side (each step is latency-bound) and its rounding,
+        amplified by 1 / gap, forces extra refinement rounds."""
         N = self.N
         sites = [self._b(b + k) for k in range(N)]
-        return self._plan(("env", direction, b), lambda: EnvPlan(self.ctx, direction, [self.slay[s] for s in sites], self.blay[b], self.prec))
+        return self._plan(("env", direction, b), lambda: EnvPlan(self.ctx, direction, [self.slay[s] for s in sites], self.blay[b], FP64))
 
     def _left_env(self, b, tol):
         """LE[b] = InvertE_proj(AL cell from b, GetHL, 1, R, l) with R the cell fixed point (trueLR)."""
@@ -298,6 +306,15 @@ class VUMPS:
             out[o:o + r * r].reshape(r, r)[np.diag_indices(r)] = sc
         return self._dev(out)
 
+    def _unit(self, lay: BlockLayout, key):
+        """Identity on the block-diagonal layout `lay`, built once per key."""
+        k = ("unit",) + tuple(key)
+        u = self._units.get(k)
+        if u is None:
+            u = self._scaled_identity(lay, [1.0] * len(lay.rows))
+            self._units[k] = u
+        return u
+
     def _polar(self, X, lay, direction, key, tol=1e-14, maxit=200):
         """Isometric polar factor U V^T of the regrouped sector matrices of X (left: rows (s, q - d_s),
         right: the transposed grouping) by the Newton-Schulz iteration X <- X (3 - X^T X) / 2
@@ -317,7 +334,7 @@ class VUMPS:
                 X, _ = (self._prod(MPS_MAT, lay, X, Gl, S, key + ("s",)) if direction == LEFT
                         else self._prod(MAT_MPS, Gl, S, lay, X, key + ("s",)))
                 continue
-            I = self._scaled_identity(Gl, [1.0] * len(Gl.rows))
+            I = self._unit(Gl, key)
             dev = float((G - I).abs().max().item())
             if dev < tol:
                 break

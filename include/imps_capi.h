@@ -78,6 +78,12 @@ imps_status imps_plan_destroy(imps_plan plan);
  * and bytes (each operand block read once and each output block written once at the working
  * precision).  Either pointer may be NULL. */
 imps_status imps_plan_cost(imps_plan plan, double* flops, double* bytes);
+/* Measurement hook (no counterpart in the reference; for benchmarks and profiling): with timing
+ * enabled, every apply of the plan records HIP events on the plan's stream just before its first
+ * and just after its last GEMM kernel (the per-apply split copy of the input is outside the pair).
+ * A call returns the summed GPU milliseconds and the number of timed applies since the previous
+ * call (waiting for the last event), resets both and then sets timing on (enable != 0) or off. */
+imps_status imps_plan_timing(imps_plan plan, int enable, double* ms, int* launches);
 /* Number of grouped kernel launches per apply and total work tiles per vector. */
 imps_status imps_plan_shape(imps_plan plan, int* launches, int* tiles);
 /* Output structure of a plan whose output layout the library derives (imps_redop_plan):
