@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <type_traits>
 
 #include "engine.hpp"
@@ -254,10 +255,46 @@ __device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amd
     F.r[1] = load_frg<PREC>(rs, (kok && 16 + li < nidx) ? e0 + es : kOOB, lo);
 }
 
-// acc[a][b] += (col-side fragment b)(row-side fragment a), the transposed sub-tile (a, b).
-template <int PREC>
+// acc[a][b] += (col-side fragment b)(row-side fragment a), the transposed sub-tile (a, b), for
+// a < NA, b < NB as template arguments (shared-panel kernel, one loop per tile shape): the MFMA
+// sequence of a chunk is straight-line code.  With runtime counts in that kernel's loop, the
+// compiler placed an accumulator read one instruction after a branch from the last MFMA writing
+// it, without the MFMA result wait states (wrong values, found by the full-size parity test).
+template <int PREC, int NA, int NB>
 __device__ __forceinline__ void mma_tiles(typename PTraits<PREC>::Acc (&acc)[2][2], const typename PTraits<PREC>::Frg (&row)[2],
-                                          const typename PTraits<PREC>::Frg (&col)[2], int na, int nb) {
+                                          const typename PTraits<PREC>::Frg (&col)[2]) {
+    if constexpr (PREC == IMPS_BF16X3) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(col[b].hi, row[a].lo, acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(col[b].lo, row[a].hi, acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(col[b].hi, row[a].hi, acc[a][b], 0, 0, 0);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if constexpr (PREC == IMPS_FP64)
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(col[b].v[q], row[a].v[q], acc[a][b], 0, 0, 0);
+                    else
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(col[b].v[q], row[a].v[q], acc[a][b], 0, 0, 0);
+                }
+            }
+        }
+    }
+}
+
+// The same with runtime tile counts na, nb <= 2 (per-wave kernel; its generated code is checked
+// for MFMA result reads across branches by the test suite).
+template <int PREC>
+__device__ __forceinline__ void mma_tiles_n(typename PTraits<PREC>::Acc (&acc)[2][2], const typename PTraits<PREC>::Frg (&row)[2],
+                                            const typename PTraits<PREC>::Frg (&col)[2], int na, int nb) {
     if constexpr (PREC == IMPS_BF16X3) {
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
@@ -292,7 +329,31 @@ __device__ __forceinline__ void mma_tiles(typename PTraits<PREC>::Acc (&acc)[2][
 // Rows of C come from op(A), columns from op(B): the panel is A or B as the chunk says.  The
 // operands are chosen by value (selects), so that the MFMA sequence exists once.
 template <int PREC>
-__device__ __forceinline__ void compute_chunk(typename PTraits<PREC>::Acc (&acc)[2][2], const Frag<PREC>& F, bool panel_is_a, int na, int nb) {
+__device__ __forceinline__ void select_operands(typename PTraits<PREC>::Frg (&row)[2], typename PTraits<PREC>::Frg (&col)[2], const Frag<PREC>& F,
+                                                bool panel_is_a) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        if constexpr (PREC == IMPS_BF16X3) {
+            row[s].hi = panel_is_a ? F.p[s].hi : F.r[s].hi;
+            row[s].lo = panel_is_a ? F.p[s].lo : F.r[s].lo;
+            col[s].hi = panel_is_a ? F.r[s].hi : F.p[s].hi;
+            col[s].lo = panel_is_a ? F.r[s].lo : F.p[s].lo;
+        } else {
+            row[s].v = panel_is_a ? F.p[s].v : F.r[s].v;
+            col[s].v = panel_is_a ? F.r[s].v : F.p[s].v;
+        }
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ void compute_chunk_n(typename PTraits<PREC>::Acc (&acc)[2][2], const Frag<PREC>& F, bool panel_is_a, int na, int nb) {
+    typename PTraits<PREC>::Frg row[2], col[2];
+    select_operands<PREC>(row, col, F, panel_is_a);
+    mma_tiles_n<PREC>(acc, row, col, na, nb);
+}
+
+template <int PREC, int NA, int NB>
+__device__ __forceinline__ void compute_chunk(typename PTraits<PREC>::Acc (&acc)[2][2], const Frag<PREC>& F, bool panel_is_a) {
     typename PTraits<PREC>::Frg row[2], col[2];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -306,7 +367,21 @@ __device__ __forceinline__ void compute_chunk(typename PTraits<PREC>::Acc (&acc)
             col[s].v = panel_is_a ? F.r[s].v : F.p[s].v;
         }
     }
-    mma_tiles<PREC>(acc, row, col, na, nb);
+    mma_tiles<PREC, NA, NB>(acc, row, col);
+}
+
+// Calls f(integral_constant<NA>, integral_constant<NB>) for the runtime tile counts na, nb in {1, 2}.
+template <typename F>
+__device__ __forceinline__ void with_tile_shape(int na, int nb, F&& f) {
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    if (na == 2) {
+        if (nb == 2) f(I2{}, I2{});
+        else f(I2{}, I1{});
+    } else {
+        if (nb == 2) f(I1{}, I2{});
+        else f(I1{}, I1{});
+    }
 }
 
 // Store the transposed accumulator sub-tile (a, b): to the FP64 output (column-major block), or,
@@ -369,24 +444,26 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
         const unsigned xlo = (unsigned)(2 * A.plx) * EB, wlo = (unsigned)(2 * A.plw) * EB;  // lo planes (BF16X3)
         const int nrow = t.M - t.m0, ncol = t.N - t.n0;  // raw-side extent: columns if the panel is A, rows otherwise
         const int cend = t.ch0 + t.nch;
-        int c = t.ch0 + part;
-        Frag<PREC> F0, F1;
-        PChunk d0 = A.chunks[c];
-        load_chunk<PREC>(F0, d0, rp, rx, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
-        while (true) {
-            // prefetch the next chunk (the current one again at the end: same count of loads)
-            const int c1 = c + S < cend ? c + S : c;
-            const PChunk d1 = A.chunks[c1];
-            load_chunk<PREC>(F1, d1, rp, rx, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
-            compute_chunk<PREC>(acc, F0, (d0.flags & PCF_PANEL_IS_A) != 0, na, nb);
-            if (c + S >= cend) break;
-            c = c1;
-            const int c2 = c + S < cend ? c + S : c;
-            d0 = A.chunks[c2];
+        {
+            int c = t.ch0 + part;
+            Frag<PREC> F0, F1;
+            PChunk d0 = A.chunks[c];
             load_chunk<PREC>(F0, d0, rp, rx, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
-            compute_chunk<PREC>(acc, F1, (d1.flags & PCF_PANEL_IS_A) != 0, na, nb);
-            if (c + S >= cend) break;
-            c = c2;
+            while (true) {
+                // prefetch the next chunk (the current one again at the end: same count of loads)
+                const int c1 = c + S < cend ? c + S : c;
+                const PChunk d1 = A.chunks[c1];
+                load_chunk<PREC>(F1, d1, rp, rx, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                compute_chunk_n<PREC>(acc, F0, (d0.flags & PCF_PANEL_IS_A) != 0, na, nb);
+                if (c + S >= cend) break;
+                c = c1;
+                const int c2 = c + S < cend ? c + S : c;
+                d0 = A.chunks[c2];
+                load_chunk<PREC>(F0, d0, rp, rx, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                compute_chunk_n<PREC>(acc, F1, (d1.flags & PCF_PANEL_IS_A) != 0, na, nb);
+                if (c + S >= cend) break;
+                c = c2;
+            }
         }
     }
 
@@ -421,8 +498,130 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
     }
 }
 
+// K-split 1, four vectors per workgroup: the four waves run one task (one chunk list) for four
+// vectors, so the panel fragments of a chunk, identical for all of them, are loaded once per
+// workgroup into a double-buffered LDS copy (16-byte units, fragment f at units [f*HU, (f+1)*HU))
+// and every wave reads them from there; each wave loads only its own vector's raw fragments.
+// Every wave, including those past the last vector, takes part in the loads and barriers.
+template <int PREC>
+__global__ __launch_bounds__(64 * kPWaves) void panel_gemm_shared_kernel(const PArgs A) {
+    using TR = PTraits<PREC>;
+    using Acc = typename TR::Acc;
+    constexpr int RUN = TR::RUN;
+    constexpr unsigned EB = PREC == IMPS_FP64 ? 8 : PREC == IMPS_FP32 ? 4 : 2;
+    constexpr unsigned FB = PREC == IMPS_FP32 ? 64 * 16 : 64 * 32;  // bytes of one panel fragment
+    constexpr int HU = FB / 16;                                     // 16-byte units per fragment
+    __shared__ pi4_t lds[2][2 * HU];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ti = blockIdx.x / A.nvg, vg = blockIdx.x - ti * A.nvg;
+    const int v = vg * kPWaves + wave;
+    const bool active = v < A.nvec;
+    const PTask t = A.tasks[ti];
+    const int na = min(2, (t.M - t.m0 + 15) >> 4), nb = min(2, (t.N - t.n0 + 15) >> 4);
+
+    Acc acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = Acc{0, 0, 0, 0};
+
+    const int vs = active ? v : 0;
+    const __amdgpu_buffer_rsrc_t rx = rsrc(A.xs + A.xs_stride * vs, A.xs_stride);
+    const __amdgpu_buffer_rsrc_t rw = rsrc(A.ws + A.ws_stride * vs, A.ws_stride);
+    const __amdgpu_buffer_rsrc_t rp = rsrc(A.pstore, A.pstore_bytes);
+    const unsigned xlo = (unsigned)(2 * A.plx) * EB, wlo = (unsigned)(2 * A.plw) * EB;  // lo planes (BF16X3)
+    const int nrow = t.M - t.m0, ncol = t.N - t.n0;
+    // this thread's 16-byte unit of a chunk's panel data (fragment rg0, then rg0 + 1)
+    auto panel_unit = [&](const PChunk& c) -> pi4_t {
+        if (tid >= 2 * HU) return pi4_t{0, 0, 0, 0};
+        const int f = tid / HU, w = tid - f * HU;
+        const unsigned off = f == 0 ? (unsigned)c.pan * FB + 16u * w
+                                    : (c.pan_sub >= 0 ? (unsigned)(c.pan + c.pan_sub) * FB + 16u * w : kOOB);
+        return ld16(rp, off);
+    };
+    // raw fragments of output-side indices idx0 + [0, 16) and idx0 + [16, 32) (zeros when inactive)
+    auto load_raw = [&](Frag<PREC>& F, const PChunk& c) {
+        const bool w = (c.flags & PCF_RAW_W) != 0;
+        const __amdgpu_buffer_rsrc_t rs = w ? rw : rx;
+        const unsigned lo = w ? wlo : xlo;
+        const int nidx = active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0;
+        const bool kok = RUN * lk < c.krem;
+        const unsigned e0 = (unsigned)(c.raw_off + (long)li * c.s_idx + RUN * lk) * EB;
+        const unsigned es = (unsigned)(16 * c.s_idx) * EB;
+        F.r[0] = load_frg<PREC>(rs, (kok && li < nidx) ? e0 : kOOB, lo);
+        F.r[1] = load_frg<PREC>(rs, (kok && 16 + li < nidx) ? e0 + es : kOOB, lo);
+    };
+    // panel fragments of the chunk in LDS buffer b, in the lane layout of load_frg
+    auto read_panels = [&](Frag<PREC>& F, int b) {
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            if constexpr (PREC == IMPS_BF16X3) {
+                F.p[f].hi = __builtin_bit_cast(pbf8_t, lds[b][f * HU + lane]);
+                F.p[f].lo = __builtin_bit_cast(pbf8_t, lds[b][f * HU + 64 + lane]);
+            } else if constexpr (PREC == IMPS_FP32) {
+                F.p[f].v = __builtin_bit_cast(pf4_t, lds[b][f * HU + lane]);
+            } else {
+                const pd2_t x0 = __builtin_bit_cast(pd2_t, lds[b][f * HU + 2 * lane]);
+                const pd2_t x1 = __builtin_bit_cast(pd2_t, lds[b][f * HU + 2 * lane + 1]);
+                F.p[f].v = pd4_t{x0[0], x0[1], x1[0], x1[1]};
+            }
+        }
+    };
+
+    const int cend = t.ch0 + t.nch;
+    if (t.nch > 0) with_tile_shape(na, nb, [&](auto NA_, auto NB_) {
+        constexpr int NA = decltype(NA_)::value, NB = decltype(NB_)::value;
+        int c = t.ch0;
+        PChunk d = A.chunks[c];
+        Frag<PREC> F, G;
+        pi4_t unit = panel_unit(d);
+        load_raw(F, d);
+        if (tid < 2 * HU) lds[0][tid] = unit;
+        __syncthreads();
+        int buf = 0;
+        while (true) {
+            const bool more = c + 1 < cend;
+            const PChunk dn = A.chunks[more ? c + 1 : c];
+            if (more) {
+                unit = panel_unit(dn);
+                load_raw(G, dn);
+            }
+            read_panels(F, buf);
+            if (active) compute_chunk<PREC, NA, NB>(acc, F, (d.flags & PCF_PANEL_IS_A) != 0);
+            if (!more) break;
+            if (tid < 2 * HU) lds[buf ^ 1][tid] = unit;
+            __syncthreads();
+            buf ^= 1;
+            ++c;
+            d = dn;
+            F.r[0] = G.r[0];
+            F.r[1] = G.r[1];
+        }
+    });
+    if (!active) return;
+    double* C = A.out ? A.out + A.out_stride * v + t.c_off : nullptr;
+    void* wsplit = A.wout ? (void*)(A.wout + A.ws_stride * v) : nullptr;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+            if (a < na && b < nb) store_sub<PREC>(acc[a][b], C, wsplit, A.plw, t, a, b, li, lk);
+}
+
 __global__ void zero_kernel(double* __restrict__ p, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// K-split 1 in BF16X3 uses the shared-panel kernel (measured faster there; in FP64 and FP32 the
+// per-wave kernel is faster); IMPS_PANEL_SHARED=0 selects the per-wave kernel (comparison runs)
+bool panel_shared_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("IMPS_PANEL_SHARED");
+        return !(e && e[0] == '0');
+    }();
+    return on;
 }
 
 template <int PREC, int S>
@@ -433,7 +632,14 @@ hipError_t launch_ps(const DevPStage& st, PArgs a, hipStream_t stream) {
     a.chunks = st.chunks;
     const long nblk = (long)st.ntasks * a.nvg;
     if (nblk > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((panel_gemm_kernel<PREC, S>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+    if constexpr (S == 1) {
+        if (PREC == IMPS_BF16X3 && panel_shared_enabled())
+            hipLaunchKernelGGL((panel_gemm_shared_kernel<PREC>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+        else
+            hipLaunchKernelGGL((panel_gemm_kernel<PREC, S>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL((panel_gemm_kernel<PREC, S>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+    }
     return hipGetLastError();
 }
 
