@@ -25,8 +25,11 @@ dense eigensolve (the reference uses ARPACK) and scaled so that tr(LM RM) = 1, t
 InvEfun_proj_fac's trace(x*rho) forms (the reference leaves ARPACK's normalisation; the projected
 quantities do not depend on it for excitation vectors).
 Parity of whole ApplyHeff iterate sequences against the reference is unpinned (no reference
-fixtures exist); tests pin this module by Hermiticity of H_eff(k), by the explicit geometric-sum
-identity of the inverses and by the dispersion of the Heisenberg chain.
+fixtures exist).  tests/test_excitations_oracle.py pins this module by: its block algebra (mixed
+TMs, MPS products, ApplyOperator, operator TMs) against the separately pinned C++ oracle; the
+explicit geometric series sum_m (fac T)^m x for InvertE_fac; the reduction of InvertE_proj_fac at
+fac = 1 to the C++ oracle's InvertE_proj; Hermiticity of the assembled H_eff(k) and positivity of
+its spectrum above the ground state; the fixed-point eigenvalue 1 of the dense TM spectra.
 """
 from __future__ import annotations
 
