@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vumps", action="store_true")
+    ap.add_argument("--no-exc", action="store_true")
+    ap.add_argument("--exc-momenta", type=int, default=4, help="momenta of the excitation sample (of 32)")
     ap.add_argument("--precs", default="fp64,fp32,bf16x3")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
@@ -224,6 +226,51 @@ def main():
         if "fp64" in hres and head in hres:
             out["vumps_iter_s"] = {"fp64": hres["fp64"]["s_per_iter"], head: hres[head]["s_per_iter"],
                                    "mixed_vs_fp64": hres["fp64"]["s_per_iter"] / hres[head]["s_per_iter"]}
+
+    # Quasiparticle excitations on the D = 512 Hubbard structure (81 (N, M) sectors): ApplyHeff
+    # (two nested geometric sums per apply) and the 4 lowest modes per momentum on a sample of the
+    # 32 momenta p = j / 31 (units of pi per site), K = (0, 0).  The state is the seeded random
+    # mixed-canonical MPS of that structure (a converged ground state at this D is reached by bond
+    # growth, not built here); the cost per apply depends on the structure and on the geometric
+    # sums' iteration counts.
+    if not args.no_exc:
+        from imps_cpp_tools_amd.excitations import ExcitationEngine
+        from imps_cpp_tools_amd.nullspace import left_nullspace
+        from imps_cpp_tools_amd.vumps import VUMPS
+        ec = wl.HUBBARD_512
+        k5, s5 = wl.fhub_profile(ec["nside"], ec["peak"], 1, ec["total"])
+        t0 = time.perf_counter()
+        gs = VUMPS(ctx, ec["model"], ec["N"], ec["QN"], k5, s5, ec["params"], prec=im.FP64, seed=1005, env_krylov=300)
+        _, ecell = gs._left_env(0, 1e-14)
+        nls = [left_nullspace(ctx, gs.AL[n], gs.slay[n], seed=n) for n in range(ec["N"])]
+        eng = ExcitationEngine(ctx, gs.AL, gs.slay, gs.AR, gs.slay, [a for a, _ in nls], [l for _, l in nls], gs.C,
+                               [gs.blay[(n + 1) % ec["N"]] for n in range(ec["N"])], gs.H, gs.d, ecell / ec["N"], (0, 0))
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        zr = np.random.default_rng(7)
+        z = torch.from_numpy(zr.standard_normal(eng.mtot) + 1j * zr.standard_normal(eng.mtot)).cuda()
+        eng.matvec(z, 0.5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            eng.matvec(z, 0.5)
+        torch.cuda.synchronize()
+        t_apply = (time.perf_counter() - t0) / 5
+        ps = [j / 31 for j in np.linspace(1, 30, args.exc_momenta).round().astype(int)]
+        per = []
+        for p in ps:
+            t0 = time.perf_counter()
+            w, _, nmv = eng.lowest(p, nev=4, tol=1e-8, krylov=40)
+            torch.cuda.synchronize()
+            per.append((p, time.perf_counter() - t0, nmv))
+        s_mom = float(np.mean([x[1] for x in per]))
+        out["excitations_hubbard_d512"] = {
+            "precision": "fp64", "x_dim_complex": eng.mtot, "setup_s": setup,
+            "applyheff_per_s": 1.0 / t_apply, "applyheff_s_at_p0.5": t_apply,
+            "momenta_sample": [round(p, 4) for p in ps], "s_per_momentum_4_lowest": s_mom,
+            "applyheff_per_momentum": [x[2] for x in per], "est_s_32_momenta": 32 * s_mom}
+        log(f"excitations hubbard D=512: ApplyHeff {t_apply * 1e3:.1f} ms, 4 lowest {s_mom:.2f} s per momentum "
+            f"({[x[2] for x in per]} applies), setup {setup:.2f} s")
 
     # CPU baseline: the oracle (clean-room FP64 restatement of ApplyHA) on the host, bounded sample
     if not args.no_cpu:

@@ -287,8 +287,11 @@ class BlockOps:
 
     # ---- geometric sums ------------------------------------------------------------------------
     def _gmres(self, afun, b, x0, tol):
-        """Unrestarted GMRES with modified Gram-Schmidt and complex Givens rotations on complex
-        device vectors (gmres, IterativeSolvers.hpp:21-149)."""
+        """Unrestarted GMRES with complex Givens rotations on complex device vectors (gmres,
+        IterativeSolvers.hpp:21-149; same stopping rule |g(j+1)| < tol ||b||).  The Arnoldi vectors
+        are orthogonalised by classical Gram-Schmidt applied twice (the reference: modified
+        Gram-Schmidt) on a device basis, so that a step makes two batched inner-product passes and
+        one host transfer (coefficients and norm) instead of one per basis vector."""
         t = self.torch
         dim = b.numel()
         kry = dim
@@ -297,7 +300,9 @@ class BlockOps:
             return t.zeros_like(b), 0
         r = b - afun(x0)
         rnorm = t.linalg.vector_norm(r).item()
-        V = [r / rnorm]
+        cap = min(kry + 1, 64)
+        V = t.empty((cap, dim), dtype=b.dtype, device=b.device)
+        V[0] = r / rnorm
         Hm = np.zeros((kry + 1, kry + 1), complex)
         c = np.zeros(kry + 1, complex)
         s = np.zeros(kry + 1, complex)
@@ -306,19 +311,28 @@ class BlockOps:
         kdim, flag = 0, -1
         for j in range(kry):
             w = afun(V[j])
-            for i in range(j + 1):
-                Hm[i, j] = t.vdot(V[i], w).item()
-                w = w - Hm[i, j] * V[i]
-            Hm[j + 1, j] = t.linalg.vector_norm(w).item()
+            Vj = V[:j + 1]
+            h1 = t.mv(Vj.conj(), w)
+            w = w - t.mv(Vj.transpose(0, 1), h1)
+            h2 = t.mv(Vj.conj(), w)
+            w = w - t.mv(Vj.transpose(0, 1), h2)
+            hn = t.linalg.vector_norm(w)
+            coef = t.cat((h1 + h2, hn.to(h1.dtype).reshape(1))).cpu().numpy()
+            Hm[:j + 1, j] = coef[:j + 1]
+            Hm[j + 1, j] = coef[j + 1].real
             for i in range(j):
-                h1, h2 = Hm[i, j], Hm[i + 1, j]
-                Hm[i, j] = c[i] * h1 + s[i] * h2
-                Hm[i + 1, j] = -np.conj(s[i]) * h1 + c[i] * h2
+                h1_, h2_ = Hm[i, j], Hm[i + 1, j]
+                Hm[i, j] = c[i] * h1_ + s[i] * h2_
+                Hm[i + 1, j] = -np.conj(s[i]) * h1_ + c[i] * h2_
             beta = np.linalg.norm(Hm[j:j + 2, j])
             if abs(beta) < 2e-16:
                 flag = -2
                 break
-            V.append(w / Hm[j + 1, j])
+            if j + 1 >= V.shape[0]:
+                grown = t.empty((min(kry + 1, 2 * V.shape[0]), dim), dtype=b.dtype, device=b.device)
+                grown[:V.shape[0]] = V
+                V = grown
+            V[j + 1] = w / Hm[j + 1, j]
             habs = abs(Hm[j, j])
             phase = Hm[j, j] / habs if habs > 0 else 1.0
             s[j] = phase * np.conj(Hm[j + 1, j]) / beta
@@ -334,8 +348,7 @@ class BlockOps:
         x = x0.clone()
         if kdim:
             alpha = np.linalg.solve(np.triu(Hm[:kdim, :kdim]), g[:kdim])
-            for i in range(kdim):
-                x = x + complex(alpha[i]) * V[i]
+            x = x + t.mv(V[:kdim].transpose(0, 1), t.from_numpy(alpha).to(device=b.device, dtype=b.dtype))
         return x, flag
 
     def _c(self, x: DObj):

@@ -71,6 +71,8 @@ def fhub_profile(nside: int, peak: int, lo: int = 1, total: int | None = None):
 
 HUBBARD_1024 = dict(model=FHUB, N=2, QN=[2, 0], nside=11, peak=60, total=1024, params=[1.0, 4.0, 0.0, 0.0, 0.0])
 TM_512 = dict(model=XXZ, N=2, QN=[0], nsec=17, peak=64, lo=2, total=512)
+# quasiparticle excitations on the Hubbard chain, D = 512 over 81 (N, M) sectors
+HUBBARD_512 = dict(model=FHUB, N=2, QN=[2, 0], nside=9, peak=32, total=512, params=[1.0, 4.0, 0.0, 0.0, 0.0])
 
 
 def hubbard_bonds(cfg=HUBBARD_1024):

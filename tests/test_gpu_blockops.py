@@ -144,3 +144,27 @@ def _redop_mps(R, A, d):
 
 def _mps_redop(R, A, d):
     return _concat([_sum_entries([orc.bmul(_single(A, j), _single(R, i * d + j)) for j in range(d)], A.nsym) for i in range(d)])
+
+
+@pytest.mark.parametrize("name", ["xxz", "fhub", "hub1024"])
+def test_left_nullspace(ctx, name):
+    """GPU Nullspace(AL, l): the oracle's structure exactly; NL^T AL = 0, NL^T NL = 1; and the
+    null-space projection NL (NL^T X) of a random tensor X equals the oracle's (basis-free: the
+    two bases differ by a rotation of the columns)."""
+    from imps_cpp_tools_amd.nullspace import left_nullspace
+    c = cell(name)
+    A = c.AL[0]
+    NL, nl = left_nullspace(ctx, dev(A.data), lay(A), seed=5)
+    ref = orc.nullspace(A, 0)
+    assert nl == lay(ref)
+    nld = NL.cpu().numpy()
+    ob = orc.OB.like(nl, nld)
+    cross = orc.tm_unity(0, A, ob).data            # sum_s NL_s^T A_s
+    assert np.abs(cross).max() < 1e-13
+    for _, _, _, m in orc.tm_unity(0, ob).blocks():  # sum_s NL_s^T NL_s
+        assert np.abs(m - np.eye(m.shape[0])).max() < 1e-13
+    X = orc.OB.like(A, np.random.default_rng(6).standard_normal(A.nelem))
+    def proj(N):
+        Z = orc.tm_unity(0, X, N)                     # sum_s N_s^T X_s
+        return orc.mps_mat(N, Z).data
+    assert relerr(proj(ob), proj(ref)) < 1e-12

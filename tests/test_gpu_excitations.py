@@ -104,3 +104,24 @@ def test_tm_eigs(mixed, direction, K):
     assert max(min(abs(x - ref)) for x in w) < 1e-10
     if not mixed and K == (0,):
         assert abs(w[0] - 1.0) < 1e-12  # the isometry's fixed point
+
+
+def test_energies_with_gpu_nullspace():
+    """The excitation energies do not depend on the null-space basis: the engine built on the GPU's
+    Nullspace (a different orthonormal basis of the same space) gives the dense oracle's energies."""
+    from imps_cpp_tools_amd.excitations import ExcitationEngine
+    from imps_cpp_tools_amd.nullspace import left_nullspace
+    s = es.state()
+    ctx = im.Context(0)
+    AL = [_dev(a) for a in s["ALob"]]
+    NLs = [left_nullspace(ctx, a, _lay(ob), seed=9) for a, ob in zip(AL, s["ALob"])]
+    for (nl, nll), ob in zip(NLs, s["NLob"]):
+        assert nll == _lay(ob)
+    K, p = (2,), 0.3
+    eng = ExcitationEngine(ctx, AL, [_lay(a) for a in s["ALob"]], [_dev(a) for a in s["ARob"]], [_lay(a) for a in s["ARob"]],
+                           [a for a, _ in NLs], [l for _, l in NLs], [_dev(c) for c in s["Cob"]], [_lay(c) for c in s["Cob"]],
+                           s["H"], s["d"], s["E0"], K)
+    M = es.oracle_heff_matrix(K, p)
+    ref = np.linalg.eigvalsh(0.5 * (M + M.conj().T))[:4]
+    w, _, _ = eng.lowest(p, nev=4, tol=1e-10, krylov=30)
+    assert np.abs(w - ref).max() < 1e-9
