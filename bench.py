@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vumps", action="store_true")
     ap.add_argument("--no-exc", action="store_true")
+    ap.add_argument("--no-tm", action="store_true")
     ap.add_argument("--exc-momenta", type=int, default=4, help="momenta of the excitation sample (of 32)")
     ap.add_argument("--precs", default="fp64,fp32,bf16x3")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -226,6 +227,54 @@ def main():
         if "fp64" in hres and head in hres:
             out["vumps_iter_s"] = {"fp64": hres["fp64"]["s_per_iter"], head: hres[head]["s_per_iter"],
                                    "mixed_vs_fp64": hres["fp64"]["s_per_iter"] / hres[head]["s_per_iter"]}
+
+    # Transfer-matrix microbenchmark: left and right TM apply of one site, anisotropic Heisenberg
+    # U(1), d = 2, D = 512 over 17 sectors of sizes 2..64 (Gaussian in charge), seeded normal
+    # entries; FP32 and BF16X3 against FP64, 1 and `nvec` vectors per launch.
+    if not args.no_tm:
+        tc = wl.TM_512
+        tk, ts = wl.xxz_profile(tc["nsec"], tc["peak"], tc["lo"], tc["total"])
+        tb = im.make_dims(tc["model"], tc["N"], tc["QN"], tk, ts)
+        tl = im.mps_structure(tc["model"], tc["N"], tc["QN"], tb[0], tb[1])
+        tA = dev(wl.random_data(tl, np.random.default_rng(1002), scale_by_rows=True))
+        tres = {}
+        for direction, dname in ((im.LEFT, "left"), (im.RIGHT, "right")):
+            xl = wl.diag_layout(1, tb[0] if direction == im.LEFT else tb[1])
+            yl = wl.diag_layout(1, tb[1] if direction == im.LEFT else tb[0])
+            tx = dev(np.random.default_rng(1003).standard_normal(xl.nelem * NV))
+            ref = None
+            for name in ("fp64", "fp32", "bf16x3"):
+                tp = im.TMPlan(ctx, direction, tl, None, xl, yl, precs[name])
+                tp.set_mps(tA)
+                tflops, _ = tp.cost()
+                ty = torch.empty(yl.nelem * NV, dtype=torch.float64, device="cuda")
+                r = {}
+                for label, nv in (("batched", NV), ("single", 1)):
+                    for _ in range(args.warmup):
+                        tp.apply(tx, ty, nv)
+                    torch.cuda.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    for _ in range(args.steps):
+                        tp.apply(tx, ty, nv)
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    ms = e0.elapsed_time(e1) / args.steps
+                    tp.timing(True)
+                    for _ in range(args.steps):
+                        tp.apply(tx, ty, nv)
+                    kms, kn = tp.timing(False)
+                    r[label] = {"applies_per_s": nv / (ms * 1e-3), "us_per_apply": ms * 1e3, "gemm_us": kms / kn * 1e3,
+                                "gemm_tflops": nv * tflops / (kms / kn * 1e-3) / 1e12}
+                y1 = ty[:yl.nelem].cpu().numpy()
+                if ref is None:
+                    ref = y1
+                r["relerr_vs_fp64"] = float(np.linalg.norm(y1 - ref) / np.linalg.norm(ref))
+                r["flops_per_apply"] = tflops
+                tres.setdefault(dname, {})[name] = r
+            log(f"tm D=512 {dname}: " + ", ".join(f"{k} {v['batched']['applies_per_s']:.3g}/s (x{NV}), {v['single']['applies_per_s']:.3g}/s (x1)"
+                                                  for k, v in tres[dname].items()))
+        out["tm_xxz_d512"] = tres
 
     # Quasiparticle excitations on the D = 512 Hubbard structure (81 (N, M) sectors): ApplyHeff
     # (two nested geometric sums per apply) and the 4 lowest modes per momentum on a sample of the
