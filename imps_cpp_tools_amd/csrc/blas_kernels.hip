@@ -1,10 +1,10 @@
 // FP64 vector kernels for the Krylov drivers (GMRES, Lanczos): inner products against a block of
 // basis vectors, their linear combinations, scaling.  HBM-bound BLAS-1/2 work.
 //
-// Determinism: every reduction is two-stage with a fixed tree.  Stage one: each workgroup reduces a
-// fixed contiguous range of rows (grid-stride order fixed by the launch shape) into a partial per
-// basis vector; stage two: one workgroup sums the partials in index order.  Results are bitwise
-// reproducible for a given problem size.
+// Determinism: every reduction is two-stage with a fixed order.  Stage one: workgroup (c, i) reduces
+// row chunk c of basis vector i (thread strides, then a fixed wave64 shuffle tree and the four wave
+// sums in order) into a partial; stage two sums the chunk partials of each vector in chunk order.
+// Results are bitwise reproducible for a given problem size.
 #include <hip/hip_runtime.h>
 
 #include "blas.hpp"
@@ -12,42 +12,41 @@
 namespace imps {
 
 constexpr int kBlasThreads = 256;
-constexpr int kRedBlocks = 256;  // stage-one workgroups of a reduction (partials per basis vector)
+constexpr int kRedChunks = 64;     // at most this many row chunks per reduction
+constexpr long kChunkRows = 2048;  // at least this many rows per chunk (fewer chunks for short vectors)
 
-// sum over the workgroup of one double per thread (fixed tree); result valid in thread 0
+// sum over the workgroup of one double per thread: wave64 shuffle tree, then the wave sums in order;
+// result valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* sh) {
-    const int t = threadIdx.x;
-    sh[t] = v;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int s = kBlasThreads / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] += sh[t + s];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kBlasThreads / 64; ++k) r += sh[k];
     return r;
 }
 
-// partial[b*nb + i] = sum over rows of block b of V[i*ldv + r] * w[r], i < nb
+// grid (nchunks, nb): partial[c*nb + i] = sum over the rows of chunk c of V[i*ldv + r] * w[r]
 __global__ __launch_bounds__(kBlasThreads) void multidot_partial(const double* __restrict__ V, long ldv, int nb,
                                                                  const double* __restrict__ w, long n, double* __restrict__ partial) {
-    __shared__ double sh[kBlasThreads];
+    __shared__ double sh[kBlasThreads / 64];
+    const int c = blockIdx.x, i = blockIdx.y;
     const long per = (n + gridDim.x - 1) / gridDim.x;
-    const long r0 = blockIdx.x * per, r1 = min(n, r0 + per);
-    for (int i = 0; i < nb; ++i) {
-        double s = 0;
-        for (long r = r0 + threadIdx.x; r < r1; r += kBlasThreads) s += V[i * ldv + r] * w[r];
-        s = block_sum(s, sh);
-        if (threadIdx.x == 0) partial[(long)blockIdx.x * nb + i] = s;
-    }
+    const long r0 = c * per, r1 = min(n, r0 + per);
+    const double* __restrict__ v = V + i * ldv;
+    double s = 0;
+    for (long r = r0 + threadIdx.x; r < r1; r += kBlasThreads) s += v[r] * w[r];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[(long)c * nb + i] = s;
 }
 
-// out[i] = sum_b partial[b*nb + i] (b ascending), one thread per i
-__global__ void multidot_final(const double* __restrict__ partial, int nblocks, int nb, double* __restrict__ out) {
+// out[i] = sum_c partial[c*nb + i] (c ascending), one thread per i
+__global__ void multidot_final(const double* __restrict__ partial, int nchunks, int nb, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nb) return;
     double s = 0;
-    for (int b = 0; b < nblocks; ++b) s += partial[(long)b * nb + i];
+    for (int c = 0; c < nchunks; ++c) s += partial[(long)c * nb + i];
     out[i] = s;
 }
 
@@ -78,9 +77,11 @@ static unsigned grid_for(long n) {
 
 hipError_t blas_multidot(const double* V, long ldv, int nb, const double* w, long n, double* out, double* scratch, hipStream_t s) {
     if (nb <= 0) return hipSuccess;
-    const int nblk = (int)std::min<long>(kRedBlocks, std::max<long>(1, (n + kBlasThreads - 1) / kBlasThreads));
-    hipLaunchKernelGGL(multidot_partial, dim3(nblk), dim3(kBlasThreads), 0, s, V, ldv, nb, w, n, scratch);
-    hipLaunchKernelGGL(multidot_final, dim3((nb + 63) / 64), dim3(64), 0, s, scratch, nblk, nb, out);
+    if (nb > 65535) return hipErrorInvalidValue;
+    const int nch = (int)std::min<long>(kRedChunks, std::max<long>(1, n / kChunkRows));
+    // one chunk: the partials are the results
+    hipLaunchKernelGGL(multidot_partial, dim3(nch, nb), dim3(kBlasThreads), 0, s, V, ldv, nb, w, n, nch == 1 ? out : scratch);
+    if (nch > 1) hipLaunchKernelGGL(multidot_final, dim3((nb + 63) / 64), dim3(64), 0, s, scratch, nch, nb, out);
     return hipGetLastError();
 }
 
@@ -96,6 +97,6 @@ hipError_t blas_axpby(double a, const double* da, bool inv_a, const double* x, d
     return hipGetLastError();
 }
 
-size_t blas_scratch_doubles(int nb) { return (size_t)kRedBlocks * (size_t)(nb < 1 ? 1 : nb); }
+size_t blas_scratch_doubles(int nb) { return (size_t)kRedChunks * (size_t)(nb < 1 ? 1 : nb); }
 
 }  // namespace imps
