@@ -144,6 +144,7 @@ def main():
                             kernel_tflops=nv * flops / (kms / kcount * 1e-3) / 1e12)
         r["flops_per_apply"] = flops
         r["bytes_per_apply"] = bytes_
+        r["schedule"] = {"batched": p.schedule(NV), "single": p.schedule(1)}  # (K-split, shared panels), autotuned
         r["y"] = Y[:n].cpu().numpy()
         results[name] = r
         log(f"{name}: batched {r['batched']['applies_per_s']:.4g} applies/s ({r['batched']['tflops']:.3g} TF/s), "
@@ -174,6 +175,7 @@ def main():
         "fp64_applies_per_s": rate("fp64"),
         "fp32_applies_per_s": rate("fp32"),
         "single_vector_applies_per_s": {k: results[k]["single"]["applies_per_s"] for k in results},
+        "panel_schedules": {k: results[k]["schedule"] for k in results},
         "vumps_iter_s": None,
         "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": PEAK_TFLOPS[head], "unit": "TFLOP/s",
                      "frac": achieved_tf / PEAK_TFLOPS[head], "traffic": committed_traffic(head, NV),
@@ -270,6 +272,7 @@ def main():
                 if ref is None:
                     ref = y1
                 r["relerr_vs_fp64"] = float(np.linalg.norm(y1 - ref) / np.linalg.norm(ref))
+                r["schedule"] = {"batched": tp.schedule(NV), "single": tp.schedule(1)}
                 r["flops_per_apply"] = tflops
                 tres.setdefault(dname, {})[name] = r
             log(f"tm D=512 {dname}: " + ", ".join(f"{k} {v['batched']['applies_per_s']:.3g}/s (x{NV}), {v['single']['applies_per_s']:.3g}/s (x1)"

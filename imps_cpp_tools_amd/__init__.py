@@ -69,6 +69,7 @@ def lib():
             "imps_plan_destroy": (I, [P]),
             "imps_plan_cost": (I, [P, dp, dp]),
             "imps_plan_timing": (I, [P, I, dp, ip]),
+            "imps_plan_schedule": (I, [P, I, ip, ip]),
             "imps_plan_shape": (I, [P, ip, ip]),
             "imps_plan_out_layout": (I, [P, ip, ip, ctypes.POINTER(ctypes.c_long), ip, ip, ip, ip, ip]),
             "imps_tm_plan": (I, [P, ctypes.POINTER(P), I, LP, LP, LP, LP, I]),
@@ -348,6 +349,12 @@ class Plan:
         ms, n = ctypes.c_double(), ctypes.c_int()
         _check(lib().imps_plan_timing(self.h, int(enable), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def schedule(self, nvec: int = 1):
+        """(ksplit, shared) of the panel kernels for applies of nvec vectors (imps_plan_schedule)."""
+        k, sh = ctypes.c_int(), ctypes.c_int()
+        _check(lib().imps_plan_schedule(self.h, nvec, ctypes.byref(k), ctypes.byref(sh)))
+        return k.value, bool(sh.value)
 
     def shape(self):
         a, t = ctypes.c_int(), ctypes.c_int()

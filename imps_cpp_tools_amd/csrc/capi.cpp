@@ -135,6 +135,18 @@ imps_status imps_plan_timing(imps_plan P, int enable, double* ms, int* launches)
     });
 }
 
+imps_status imps_plan_schedule(imps_plan P, int nvec, int* ksplit, int* shared) {
+    return guarded([&] {
+        if (!P) throw InvalidArg("plan is NULL");
+        if (nvec < 1) throw InvalidArg("nvec must be >= 1");
+        const int sc = panel_schedule_of(P, nvec);
+        static const int ks[4] = {1, 1, 2, 4};
+        if (ksplit) *ksplit = P->use_panels ? ks[sc] : 0;
+        if (shared) *shared = P->use_panels && sc == 1 ? 1 : 0;
+        return IMPS_OK;
+    });
+}
+
 imps_status imps_plan_cost(imps_plan P, double* flops, double* bytes) {
     return guarded([&] {
         if (!P) throw InvalidArg("plan is NULL");

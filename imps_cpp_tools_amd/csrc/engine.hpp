@@ -54,7 +54,10 @@ struct PanelLaunch {
     int nvec = 0;
 };
 // ksplit (1, 2, 4): waves sharing one vector's tile; the workgroup serves 4 / ksplit vectors.
-hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, const PanelLaunch& L, hipStream_t stream);
+// ksplit in {1, 2, 4}: waves sharing one vector's tile; shared (ksplit 1 only): the shared-panel kernel
+hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, bool shared, const PanelLaunch& L, hipStream_t stream);
+// the schedule used without the autotuner: shared-panel kernel for K-split 1 in BF16X3
+bool panel_shared_default(int prec);
 hipError_t launch_zero(double* p, long n, hipStream_t stream);
 
 }  // namespace imps

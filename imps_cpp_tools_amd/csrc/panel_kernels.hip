@@ -614,8 +614,9 @@ __global__ void zero_kernel(double* __restrict__ p, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
-// K-split 1 in BF16X3 uses the shared-panel kernel (measured faster there; in FP64 and FP32 the
-// per-wave kernel is faster); IMPS_PANEL_SHARED=0 selects the per-wave kernel (comparison runs)
+// Default schedule (without the autotuner): K-split 1 in BF16X3 uses the shared-panel kernel
+// (measured faster there; in FP64 and FP32 the per-wave kernel is); IMPS_PANEL_SHARED=0 selects the
+// per-wave kernel (comparison runs)
 bool panel_shared_enabled() {
     static const bool on = [] {
         const char* e = std::getenv("IMPS_PANEL_SHARED");
@@ -625,7 +626,7 @@ bool panel_shared_enabled() {
 }
 
 template <int PREC, int S>
-hipError_t launch_ps(const DevPStage& st, PArgs a, hipStream_t stream) {
+hipError_t launch_ps(const DevPStage& st, PArgs a, bool shared, hipStream_t stream) {
     constexpr int V = kPWaves / S;
     a.nvg = (a.nvec + V - 1) / V;
     a.tasks = st.tasks;
@@ -633,7 +634,7 @@ hipError_t launch_ps(const DevPStage& st, PArgs a, hipStream_t stream) {
     const long nblk = (long)st.ntasks * a.nvg;
     if (nblk > 0x7fffffffL) return hipErrorInvalidValue;
     if constexpr (S == 1) {
-        if (PREC == IMPS_BF16X3 && panel_shared_enabled())
+        if (shared)
             hipLaunchKernelGGL((panel_gemm_shared_kernel<PREC>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
         else
             hipLaunchKernelGGL((panel_gemm_kernel<PREC, S>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
@@ -644,11 +645,11 @@ hipError_t launch_ps(const DevPStage& st, PArgs a, hipStream_t stream) {
 }
 
 template <int PREC>
-hipError_t launch_p(const DevPStage& st, int ksplit, const PArgs& a, hipStream_t stream) {
+hipError_t launch_p(const DevPStage& st, int ksplit, bool shared, const PArgs& a, hipStream_t stream) {
     switch (ksplit) {
-        case 1: return launch_ps<PREC, 1>(st, a, stream);
-        case 2: return launch_ps<PREC, 2>(st, a, stream);
-        case 4: return launch_ps<PREC, 4>(st, a, stream);
+        case 1: return launch_ps<PREC, 1>(st, a, shared, stream);
+        case 2: return launch_ps<PREC, 2>(st, a, false, stream);
+        case 4: return launch_ps<PREC, 4>(st, a, false, stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -694,7 +695,9 @@ hipError_t launch_split(int prec, const SplitBlock* d_blocks, int nblocks, const
     return hipGetLastError();
 }
 
-hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, const PanelLaunch& L, hipStream_t stream) {
+bool panel_shared_default(int prec) { return prec == IMPS_BF16X3 && panel_shared_enabled(); }
+
+hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, bool shared, const PanelLaunch& L, hipStream_t stream) {
     if (st.ntasks == 0 || L.nvec == 0) return hipSuccess;
     PArgs a{};
     a.pstore = (const char*)L.pstore;
@@ -710,9 +713,9 @@ hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, const P
     a.wout = st.out_slot == SLOT_W ? (char*)L.ws : nullptr;
     a.nvec = L.nvec;
     switch (prec) {
-        case IMPS_FP64: return launch_p<IMPS_FP64>(st, ksplit, a, stream);
-        case IMPS_FP32: return launch_p<IMPS_FP32>(st, ksplit, a, stream);
-        case IMPS_BF16X3: return launch_p<IMPS_BF16X3>(st, ksplit, a, stream);
+        case IMPS_FP64: return launch_p<IMPS_FP64>(st, ksplit, shared, a, stream);
+        case IMPS_FP32: return launch_p<IMPS_FP32>(st, ksplit, shared, a, stream);
+        case IMPS_BF16X3: return launch_p<IMPS_BF16X3>(st, ksplit, shared, a, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -36,14 +36,72 @@ bool panels_enabled() {
     return !(e && e[0] == '0');
 }
 
-// K-split of the panel kernel: waves sharing one vector's tile.  One vector: all four waves, so
-// that a single apply still spreads over the chip; four or more: one wave per vector.
-int panel_ksplit(int nvec) {
+// Panel schedules: K-split (waves sharing one vector's tile) and, for K-split 1, per-wave or shared
+// (LDS) panel loads.  Encoded 0 = K-split 1, 1 = K-split 1 shared, 2 = K-split 2, 3 = K-split 4.
+constexpr int kSchedKsplit[4] = {1, 1, 2, 4};
+constexpr bool kSchedShared[4] = {false, true, false, false};
+
+// IMPS_KSPLIT forces a K-split (comparison runs and tests); -1 when unset
+int forced_ksplit() {
     if (const char* e = std::getenv("IMPS_KSPLIT")) {
         const int k = std::atoi(e);
         if (k == 1 || k == 2 || k == 4) return k;
     }
-    return nvec >= 4 ? 1 : nvec >= 2 ? 2 : 4;
+    return -1;
+}
+
+// The rule used without tuning: one vector -> all four waves on its tile (a single apply still
+// spreads over the chip); 2-3 vectors -> two waves; four or more -> one wave per vector, with the
+// shared-panel kernel in BF16X3.
+int default_schedule(int prec, int nvec) {
+    const int k = forced_ksplit() > 0 ? forced_ksplit() : nvec >= 4 ? 1 : nvec >= 2 ? 2 : 4;
+    return k == 4 ? 3 : k == 2 ? 2 : (panel_shared_default(prec) ? 1 : 0);
+}
+
+// IMPS_AUTOTUNE=0 keeps the default rule (bitwise-reproducible runs: schedules sum in different orders)
+bool autotune_enabled() {
+    const char* e = std::getenv("IMPS_AUTOTUNE");
+    return !(e && e[0] == '0') && forced_ksplit() < 0;
+}
+
+void launch_panel_schedule(imps_plan_s* P, int sched, const PanelLaunch& L, hipStream_t s) {
+    for (auto& ds : P->dpstages)
+        hip_check(launch_panel_stage(P->prec, ds, kSchedKsplit[sched], kSchedShared[sched], L, s), "panel gemm");
+}
+
+// Autotuner: on the first apply of a plan with nvec vectors, every schedule runs the GEMM stages
+// once to warm up and three times under HIP events on the plan's stream; the fastest is kept for
+// the plan and that vector count.  Skipped while the stream is being captured into a graph.
+int choose_schedule(imps_plan_s* P, int nvec, const PanelLaunch& L, hipStream_t s) {
+    auto it = P->sched.find(nvec);
+    if (it != P->sched.end()) return it->second;
+    const int def = default_schedule(P->prec, nvec);
+    if (!autotune_enabled()) return def;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hip_check(hipStreamIsCapturing(s, &cap), "hipStreamIsCapturing");
+    if (cap != hipStreamCaptureStatusNone) return def;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    int best = def;
+    float best_ms = 0.0f;
+    for (int sc = 0; sc < 4; ++sc) {
+        launch_panel_schedule(P, sc, L, s);
+        hip_check(hipEventRecord(e0, s), "hipEventRecord");
+        for (int r = 0; r < 3; ++r) launch_panel_schedule(P, sc, L, s);
+        hip_check(hipEventRecord(e1, s), "hipEventRecord");
+        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (sc == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = sc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->sched[nvec] = best;
+    return best;
 }
 
 template <typename T>
@@ -189,9 +247,9 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
         L.out = y;
         L.out_stride = P->prog.y_elems;
         L.nvec = nvec;
-        const int ks = panel_ksplit(nvec);
+        const int sc = choose_schedule(P, nvec, L, s);
         time_mark(P, s);
-        for (auto& ds : P->dpstages) hip_check(launch_panel_stage(P->prec, ds, ks, L, s), "panel gemm");
+        launch_panel_schedule(P, sc, L, s);
         time_mark(P, s);
         return;
     }
@@ -209,6 +267,11 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
             hip_check(launch_stage(P->prec, ds, sl, y, P->prog.y_elems, true, nvec, s), "grouped gemm");
     }
     time_mark(P, s);
+}
+
+int panel_schedule_of(const imps_plan_s* P, int nvec) {
+    auto it = P->sched.find(nvec);
+    return it != P->sched.end() ? it->second : default_schedule(P->prec, nvec);
 }
 
 double take_timing(imps_plan_s* P, int* launches) {

@@ -1,6 +1,7 @@
 // Plans: a compiled grouped-GEMM Program, its device copy and the operator store.
 #pragma once
 
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -62,6 +63,7 @@ struct imps_plan_s {
     const imps::SplitBlock* d_xblocks = nullptr;
     std::vector<imps::DevPStage> dpstages;
     long xs_ready = 0, ws_ready = 0;  // vectors of the split buffers already zero-filled
+    std::map<int, int> sched;         // autotuned panel schedule per vector count (see panel_schedule)
     // opt-in GPU timing of the GEMM launches of each apply (events recorded on the plan's stream
     // around the grouped/panel kernels, split copies excluded); pairs are read by imps_plan_timing
     bool timing = false;
@@ -84,5 +86,8 @@ void set_operator(imps_plan_s* P, const std::vector<const double*>& parts);
 void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_override = nullptr);
 // GPU milliseconds of the timed GEMM launches since the last call (waits for them), then resets
 double take_timing(imps_plan_s* P, int* launches);
+// panel schedule for nvec vectors: 0 = K-split 1, 1 = K-split 1 with shared panels, 2 = K-split 2,
+// 3 = K-split 4 (the tuned one if the plan has applied nvec vectors, else the default rule)
+int panel_schedule_of(const imps_plan_s* P, int nvec);
 
 }  // namespace imps

@@ -84,6 +84,11 @@ imps_status imps_plan_cost(imps_plan plan, double* flops, double* bytes);
  * A call returns the summed GPU milliseconds and the number of timed applies since the previous
  * call (waiting for the last event), resets both and then sets timing on (enable != 0) or off. */
 imps_status imps_plan_timing(imps_plan plan, int enable, double* ms, int* launches);
+/* Introspection (no counterpart in the reference): the schedule of the panel GEMM kernels for applies
+ * of nvec vectors -- the autotuned one once the plan has applied nvec vectors, the default rule
+ * before that or with IMPS_AUTOTUNE=0.  ksplit = waves sharing one vector's output tile (1, 2, 4;
+ * 0 for plans without panel kernels), shared = 1 when panel fragments are shared through LDS. */
+imps_status imps_plan_schedule(imps_plan plan, int nvec, int* ksplit, int* shared);
 /* Number of grouped kernel launches per apply and total work tiles per vector. */
 imps_status imps_plan_shape(imps_plan plan, int* launches, int* tiles);
 /* Output structure of a plan whose output layout the library derives (imps_redop_plan):
