@@ -18,12 +18,21 @@ import imps_cpp_tools_amd as im
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True)
+def fixed_schedules(monkeypatch):
+    """The panel-kernel autotuner chooses schedules by timing, and schedules sum in different orders;
+    the early sweeps' solves stop at tolerances of ~1e-6, so such rounding differences change the
+    trajectory (and the sweep count to F <= 1e-9, 100-400 sweeps).  The fixed schedule rule makes
+    every run follow the same one."""
+    monkeypatch.setenv("IMPS_AUTOTUNE", "0")
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 with open(os.path.join(HERE, "golden", "vumps_xxz_d64.json")) as f:
     GOLD = json.load(f)
 
 
-def _run(prec, f_target=1e-9, max_sweeps=400):
+def _run(prec, f_target=1e-9, max_sweeps=600):
     from imps_cpp_tools_amd.vumps import VUMPS
     ctx = im.Context(0)
     v = VUMPS(ctx, im.XXZ, GOLD["N"], GOLD["QN"], np.array(GOLD["keys0"], np.int32), np.array(GOLD["sizes0"], np.int32),
