@@ -2,6 +2,8 @@
 #include "krylov.hpp"
 
 #include <algorithm>
+#include <deque>
+#include <memory>
 #include <cmath>
 #include <map>
 
@@ -45,9 +47,68 @@ void d2d(hipStream_t s, double* d, const double* src, size_t n) {
     if (n) hip_check(hipMemcpyAsync(d, src, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
 }
 
+// Device scratch of the Krylov drivers, reused across calls: a pool per host thread and device;
+// a call takes free buffers (the smallest that fits, else grows one) and gives them back when it
+// ends, so repeated solves do not allocate.  The pools live for the process (never freed, which
+// keeps the HIP runtime's teardown order out of the picture).
+class BufPool {
+  public:
+    static BufPool& get() {
+        thread_local std::map<int, BufPool*> pools;
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "hipGetDevice");
+        BufPool*& p = pools[dev];
+        if (!p) p = new BufPool();
+        return *p;
+    }
+    DevBuf* take(size_t bytes) {
+        Slot* fit = nullptr;      // smallest free buffer that holds `bytes`
+        Slot* largest = nullptr;  // else the largest free one, grown
+        for (auto& s : slots_) {
+            if (s.used) continue;
+            if (s.buf->bytes >= bytes && (!fit || s.buf->bytes < fit->buf->bytes)) fit = &s;
+            if (!largest || s.buf->bytes > largest->buf->bytes) largest = &s;
+        }
+        Slot* best = fit ? fit : largest;
+        if (!best) {
+            slots_.push_back(Slot{std::make_unique<DevBuf>(), false});
+            best = &slots_.back();
+        }
+        best->used = true;
+        best->buf->ensure(std::max<size_t>(bytes, 8));
+        return best->buf.get();
+    }
+    void give(DevBuf* b) {
+        for (auto& s : slots_)
+            if (s.buf.get() == b) s.used = false;
+    }
+
+  private:
+    struct Slot {
+        std::unique_ptr<DevBuf> buf;
+        bool used;
+    };
+    std::deque<Slot> slots_;  // stable addresses
+};
+
+// A pooled device buffer held for the lifetime of this object.
+struct PoolBuf {
+    DevBuf* b = nullptr;
+    PoolBuf() = default;
+    PoolBuf(const PoolBuf&) = delete;
+    PoolBuf& operator=(const PoolBuf&) = delete;
+    ~PoolBuf() { if (b) BufPool::get().give(b); }
+    void ensure(size_t bytes) {
+        if (b && b->bytes >= bytes) return;
+        if (b) BufPool::get().give(b);
+        b = BufPool::get().take(bytes);
+    }
+    double* d() const { return b->d(); }
+};
+
 // Scratch for one Krylov run: basis V (m+1 vectors), w, and reduction buffers.
 struct Work {
-    DevBuf V, w, dh, scratch;
+    PoolBuf V, w, dh, scratch;
     long n = 0;
     int m = 0;
     void init(long n_, int m_) {
@@ -208,7 +269,7 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
     const long n = A64.n;
     Work W;
     W.init(n, 1);
-    DevBuf r, dx;
+    PoolBuf r, dx;
     r.ensure(sizeof(double) * std::max<long>(n, 1));
     dx.ensure(sizeof(double) * std::max<long>(n, 1));
     const double bnorm = dev_norm(s, W, b);
@@ -248,7 +309,7 @@ KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol,
     const int keep = std::max(1, std::min(m / 3, m - 2));
     Work W;
     W.init(n, m);
-    DevBuf Y;
+    PoolBuf Y;
     Y.ensure(sizeof(double) * (size_t)n * keep);
     double nx = dev_norm(s, W, x);
     if (nx == 0) throw InvalidArg("lanczos: zero start vector");
@@ -409,7 +470,7 @@ KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, 
                 for (int i = 0; i < b.rows; ++i) t[b.off + j + (long)i * b.rows] = h[b.off + i + (long)j * b.rows];
         h2d(s, rhoT.d(), t.data(), n);
     }
-    DevBuf dd;
+    PoolBuf dd;
     dd.ensure(sizeof(double) * (blas_scratch_doubles(1) + 4));
     auto trace_in = [&](const double* v) {
         double t = 0;
@@ -477,7 +538,7 @@ int EnvSolver::fixed_point(double* x, double tol, int maxit, double* delta) {
     // GMRES solve of the environment form; a second round only polishes rounding.
     hipStream_t s = ctx->stream;
     const long n = xlay.nelem;
-    DevBuf id, dd, r0, dx;
+    PoolBuf id, dd, r0, dx;
     id.ensure(sizeof(double) * std::max<long>(n, 1));
     r0.ensure(sizeof(double) * std::max<long>(n, 1));
     dx.ensure(sizeof(double) * std::max<long>(n, 1));
