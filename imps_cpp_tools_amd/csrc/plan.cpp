@@ -58,10 +58,18 @@ int default_schedule(int prec, int nvec) {
     return k == 4 ? 3 : k == 2 ? 2 : (panel_shared_default(prec) ? 1 : 0);
 }
 
+// IMPS_SCHEDULE=0..3 forces one schedule (tests of every path the autotuner may pick); -1 when unset
+int forced_schedule() {
+    if (const char* e = std::getenv("IMPS_SCHEDULE")) {
+        if (e[0] >= '0' && e[0] <= '3' && e[1] == '\0') return e[0] - '0';
+    }
+    return -1;
+}
+
 // IMPS_AUTOTUNE=0 keeps the default rule (bitwise-reproducible runs: schedules sum in different orders)
 bool autotune_enabled() {
     const char* e = std::getenv("IMPS_AUTOTUNE");
-    return !(e && e[0] == '0') && forced_ksplit() < 0;
+    return !(e && e[0] == '0') && forced_ksplit() < 0 && forced_schedule() < 0;
 }
 
 void launch_panel_schedule(imps_plan_s* P, int sched, const PanelLaunch& L, hipStream_t s) {
@@ -73,6 +81,7 @@ void launch_panel_schedule(imps_plan_s* P, int sched, const PanelLaunch& L, hipS
 // once to warm up and three times under HIP events on the plan's stream; the fastest is kept for
 // the plan and that vector count.  Skipped while the stream is being captured into a graph.
 int choose_schedule(imps_plan_s* P, int nvec, const PanelLaunch& L, hipStream_t s) {
+    if (forced_schedule() >= 0) return forced_schedule();
     auto it = P->sched.find(nvec);
     if (it != P->sched.end()) return it->second;
     const int def = default_schedule(P->prec, nvec);
@@ -270,6 +279,7 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
 }
 
 int panel_schedule_of(const imps_plan_s* P, int nvec) {
+    if (forced_schedule() >= 0) return forced_schedule();
     auto it = P->sched.find(nvec);
     return it != P->sched.end() ? it->second : default_schedule(P->prec, nvec);
 }

@@ -194,3 +194,39 @@ def test_full_size_hubbard_hac(ctx):
         got = y16.cpu().numpy().reshape(16, -1)
         for v in range(16):
             assert relerr(got[v], (v + 1.0) * ref) < TOL[prec], (prec, v)
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2, 3], ids=["ks1", "ks1-shared", "ks2", "ks4"])
+def test_every_schedule_every_precision(ctx, sched, monkeypatch):
+    """The autotuner may pick any panel schedule for any precision: each one (forced through
+    IMPS_SCHEDULE) against the oracle, full-size D = 1024 Hubbard H_AC at 1, 3 and 16 vectors and the
+    Heisenberg TM (both directions) at 5 vectors."""
+    import synth_full
+    monkeypatch.setenv("IMPS_SCHEDULE", str(sched))
+    st = synth_full.hubbard_1024(seed=1003)
+    ref = orc.hac(st["EHL"], st["EHR"], st["HL"], st["HR"], st["AC"]).data
+    for prec in PRECS:
+        p = im.HACPlan(ctx, lay(st["AC"]), lay(st["EHL"]), lay(st["EHR"]), lay(st["HL"]), lay(st["HR"]), prec)
+        p.set_operator(dev(st["EHL"].data), dev(st["EHR"].data), dev(st["HL"].data), dev(st["HR"].data))
+        assert p.schedule(16) == ({0: 1, 1: 1, 2: 2, 3: 4}[sched], sched == 1)
+        for nv in (1, 3, 16):
+            X = np.concatenate([(v + 1.0) * st["AC"].data for v in range(nv)])
+            y = torch.full((X.size,), float("nan"), dtype=torch.float64, device="cuda")
+            p.apply(dev(X), y, nv)
+            got = y.cpu().numpy().reshape(nv, -1)
+            for v in range(nv):
+                assert relerr(got[v], (v + 1.0) * ref) < TOL[prec], (prec, nv, v)
+    c = cell("xxz")
+    rng = np.random.default_rng(31)
+    for direction in (0, 1):
+        xl, yl = (c.diag(0), c.diag(1)) if direction == 0 else (c.diag(1), c.diag(0))
+        xs = [random_like(xl, rng) for _ in range(5)]
+        refs = [orc.tm_mixed(direction, c.AL[0], None, x, yl).data for x in xs]
+        for prec in PRECS:
+            p = im.TMPlan(ctx, direction, lay(c.AL[0]), None, lay(xl), lay(yl), prec)
+            p.set_mps(dev(c.AL[0].data))
+            y = torch.full((yl.nelem * 5,), float("nan"), dtype=torch.float64, device="cuda")
+            p.apply(dev(np.concatenate([x.data for x in xs])), y, 5)
+            got = y.cpu().numpy().reshape(5, -1)
+            for v in range(5):
+                assert relerr(got[v], refs[v]) < TOL[prec], (direction, prec, v)
