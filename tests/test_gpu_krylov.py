@@ -165,3 +165,44 @@ def test_eigsh_hc(ctx):
     x = dev(C0.data)
     e, it, res = im.eigsh(p, x, tol=1e-12, krylov=30)
     assert abs(e - e_dense) < 1e-11 * abs(e_dense), (e, e_dense)
+
+
+@pytest.mark.parametrize("name", ["xxz", "fhub"])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_fixed_point_parity(ctx, name, direction):
+    """TMDominantEig (trueLR fixed points) on the GPU (EnvPlan.fixed_point, from the identity) against
+    the dense dominant eigenvector of the oracle's cell TM, both trace-normalised."""
+    c = cell(name)
+    As = c.AL if direction == 1 else c.AR  # R of the AL cell (right TM), L of the AR cell (left TM)
+    ref = fixed_point(c, As, direction)
+    p = im.EnvPlan(ctx, direction, [lay(a) for a in As], lay(c.diag(0)), im.FP64)
+    p.set_mps([dev(a.data) for a in As])
+    x0 = identity(c)
+    x0.data /= sum(np.trace(m) for _, _, _, m in x0.blocks())
+    x = dev(x0.data)
+    it, delta = p.fixed_point(x, tol=1e-14, maxit=2000)
+    assert delta < 1e-12
+    assert relerr(x.cpu().numpy(), ref.data) < 1e-10
+
+
+def test_gradnorm_matches_oracle():
+    """GradNormRight at the seeded random mixed-canonical D = 64 Heisenberg start: the GPU VUMPS's F
+    (H_AC on the GPU, null-space-free form) against the oracle's (explicit right null space)."""
+    import json
+    import os
+    from imps_cpp_tools_amd.vumps import VUMPS
+    from oracle_vumps import OracleVUMPS
+    with open(os.path.join(os.path.dirname(__file__), "golden", "vumps_xxz_d64.json")) as f:
+        g = json.load(f)
+    args = (g["N"], g["QN"], np.array(g["keys0"], np.int32), np.array(g["sizes0"], np.int32), g["params"])
+    v = VUMPS(im.Context(0), im.XXZ, *args, seed=g["seed"])
+    o = OracleVUMPS(0, *args, seed=g["seed"])
+    for n in range(g["N"]):
+        left = (n - 1) % g["N"]
+        hac, _ = v._hac_plans(n)
+        AC, _ = v._prod(im.MAT_MPS, v.blay[n], v.C[left], v.slay[n], v.AR[n], ("CAR", n))
+        F_gpu, _ = v.gradnorm(n, AC, hac)
+        HL = orc.reduced_op(o.H, o.d, o.AL[left], 0)
+        HR = orc.reduced_op(o.H, o.d, o.AR[(n + 1) % g["N"]], 1)
+        F_ref = orc.gradnorm(1, o.AR[n], o.C[left], o.LE[n], o.RE[(n + 1) % g["N"]], HL, HR)
+        assert abs(F_gpu - F_ref) <= 1e-9 * F_ref, (n, F_gpu, F_ref)
