@@ -60,11 +60,15 @@ __global__ __launch_bounds__(kBlasThreads) void gemv_sub(const double* __restric
     }
 }
 
-// y[r] = a*x[r] + b*y[r]  (a, b read from device scalars when given, else the immediates)
+// y[r] = a*x[r] + b*y[r]  (a read from the device scalar da when given, as `mode` says)
 __global__ void axpby_kernel(double a, const double* __restrict__ da, const double* __restrict__ x, double b, double* __restrict__ y,
-                             long n, int inv_a) {
-    double aa = da ? *da : a;
-    if (inv_a) aa = 1.0 / aa;
+                             long n, int mode) {
+    double aa = a;
+    if (da) {
+        aa = *da;
+        if (mode == AXPBY_INV_DA) aa = 1.0 / aa;
+        else if (mode == AXPBY_INV_SQRT_DA) aa = aa > 0.0 ? 1.0 / sqrt(aa) : 0.0;
+    }
     // BLAS convention: a zero coefficient does not read its operand (fresh buffers may hold NaN)
     for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x)
         y[r] = (aa != 0.0 ? aa * x[r] : 0.0) + (b != 0.0 ? b * y[r] : 0.0);
@@ -91,9 +95,9 @@ hipError_t blas_gemv_sub(const double* V, long ldv, int nb, const double* h, dou
     return hipGetLastError();
 }
 
-hipError_t blas_axpby(double a, const double* da, bool inv_a, const double* x, double b, double* y, long n, hipStream_t s) {
+hipError_t blas_axpby(double a, const double* da, int mode, const double* x, double b, double* y, long n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(kBlasThreads), 0, s, a, da, x, b, y, n, inv_a ? 1 : 0);
+    hipLaunchKernelGGL(axpby_kernel, dim3(grid_for(n)), dim3(kBlasThreads), 0, s, a, da, x, b, y, n, mode);
     return hipGetLastError();
 }
 

@@ -106,20 +106,25 @@ struct PoolBuf {
     double* d() const { return b->d(); }
 };
 
-// Scratch for one Krylov run: basis V (m+1 vectors), w, and reduction buffers.
+// Scratch for one Krylov run: basis V (m+1 vectors), w, reduction buffers, and the raw CGS2
+// coefficients of up to m + 1 steps (step j at coef + j * cstride).
 struct Work {
-    PoolBuf V, w, dh, scratch;
+    PoolBuf V, w, dh, scratch, coef;
     long n = 0;
     int m = 0;
+    int cstride = 0;
     void init(long n_, int m_) {
         n = n_;
         m = m_;
+        cstride = 2 * (m + 1) + 1;  // >= 2k + 1 for k <= m + 1
         V.ensure(sizeof(double) * (size_t)n * (m + 1));
         w.ensure(sizeof(double) * (size_t)std::max<long>(n, 1));
-        dh.ensure(sizeof(double) * (size_t)(2 * (m + 2)));  // >= 2k + 1 for k <= m + 1 (cgs2)
+        dh.ensure(sizeof(double) * (size_t)(2 * (m + 2)));
         scratch.ensure(sizeof(double) * blas_scratch_doubles(m + 2));
+        coef.ensure(sizeof(double) * (size_t)cstride * (m + 1));
     }
     double* Vi(int i) const { return V.d() + (size_t)i * n; }
+    double* Ci(int j) const { return coef.d() + (size_t)j * cstride; }
 };
 
 double dev_norm(hipStream_t s, Work& W, const double* v) {
@@ -129,18 +134,22 @@ double dev_norm(hipStream_t s, Work& W, const double* v) {
     return std::sqrt(std::max(h, 0.0));
 }
 
-// w <- w - V[0..k) (V^T w), twice (CGS2); h = the k projection coefficients (summed over both
-// passes); returns ||w|| afterwards.  Coefficients stay on the device between the passes, so the
-// step costs one device-to-host transfer (2k + 1 values).
-double cgs2(hipStream_t s, Work& W, int k, double* w, std::vector<double>& h) {
-    double* dh = W.dh.d();
+// One Krylov step's orthogonalisation, on the device only: w <- w - V[0..k) (V^T w), twice (CGS2),
+// then w <- w / ||w|| (a zero w stays zero).  The 2k + 1 raw coefficients (first-pass projections,
+// second-pass projections, ||w||^2 before the scaling) stay at dc for the host to collect later,
+// so consecutive steps queue on the stream without a host round trip.
+void cgs2_dev(hipStream_t s, Work& W, int k, double* w, double* dc) {
     for (int pass = 0; pass < 2; ++pass) {
-        hip_check(blas_multidot(W.V.d(), W.n, k, w, W.n, dh + pass * k, W.scratch.d(), s), "multidot");
-        hip_check(blas_gemv_sub(W.V.d(), W.n, k, dh + pass * k, w, W.n, s), "gemv");
+        hip_check(blas_multidot(W.V.d(), W.n, k, w, W.n, dc + pass * k, W.scratch.d(), s), "multidot");
+        hip_check(blas_gemv_sub(W.V.d(), W.n, k, dc + pass * k, w, W.n, s), "gemv");
     }
-    hip_check(blas_multidot(w, 0, 1, w, W.n, dh + 2 * k, W.scratch.d(), s), "norm");
-    std::vector<double> t(2 * k + 1);
-    d2h(s, t.data(), dh, 2 * k + 1);
+    hip_check(blas_multidot(w, 0, 1, w, W.n, dc + 2 * k, W.scratch.d(), s), "norm");
+    hip_check(blas_axpby(0.0, dc + 2 * k, AXPBY_INV_SQRT_DA, w, 0.0, w, W.n, s), "normalise");
+}
+
+// Host side of cgs2_dev: h = the k projection coefficients (both passes summed) from the raw
+// values t; returns ||w|| before the scaling.
+double cgs2_coeffs(const double* t, int k, std::vector<double>& h) {
     h.resize(k);
     for (int i = 0; i < k; ++i) h[i] = t[i] + t[k + i];
     return std::sqrt(std::max(t[2 * k], 0.0));
@@ -198,44 +207,62 @@ KrylovStats gmres(hipStream_t s, const LinOp& A, const double* b, double* x, dou
         st.converged = true;
         return st;
     }
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h, raw;
+    // Steps are queued in batches and their coefficients read once per batch; the batch length
+    // follows the convergence rate seen so far (about the steps still needed, 1 to 16), so few
+    // steps are computed past convergence.  Those extra steps do not enter the solution.
+    int batch = 8;
     while (st.iters < maxit) {
         // r = b - A x  -> V0
         A.apply(x, W.Vi(0));
         ++st.iters;
-        hip_check(blas_axpby(1.0, nullptr, false, b, -1.0, W.Vi(0), n, s), "residual");
+        hip_check(blas_axpby(1.0, nullptr, AXPBY_DA, b, -1.0, W.Vi(0), n, s), "residual");
         const double beta = dev_norm(s, W, W.Vi(0));
         st.relres = beta / bnorm;
         if (beta < tol * bnorm) { st.converged = true; break; }
-        hip_check(blas_axpby(1.0 / beta, nullptr, false, W.Vi(0), 0.0, W.Vi(0), n, s), "scale");
+        hip_check(blas_axpby(1.0 / beta, nullptr, AXPBY_DA, W.Vi(0), 0.0, W.Vi(0), n, s), "scale");
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int k = 0;
-        for (int j = 0; j < m && st.iters < maxit; ++j) {
-            double* w = W.Vi(j + 1);
-            A.apply(W.Vi(j), w);
-            ++st.iters;
-            const double hn = cgs2(s, W, j + 1, w, h);
-            for (int i = 0; i <= j; ++i) H[i + (size_t)j * (m + 1)] = h[i];
-            H[j + 1 + (size_t)j * (m + 1)] = hn;
-            for (int i = 0; i < j; ++i) {  // previous Givens rotations
-                const double t1 = H[i + (size_t)j * (m + 1)], t2 = H[i + 1 + (size_t)j * (m + 1)];
-                H[i + (size_t)j * (m + 1)] = cs[i] * t1 + sn[i] * t2;
-                H[i + 1 + (size_t)j * (m + 1)] = -sn[i] * t1 + cs[i] * t2;
+        bool stop = false;
+        for (int j0 = 0; j0 < m && st.iters < maxit && !stop;) {
+            const int q = std::min({batch, m - j0, maxit - st.iters});
+            for (int j = j0; j < j0 + q; ++j) {
+                A.apply(W.Vi(j), W.Vi(j + 1));
+                ++st.iters;
+                cgs2_dev(s, W, j + 1, W.Vi(j + 1), W.Ci(j - j0));
             }
-            const double hjj = H[j + (size_t)j * (m + 1)];
-            const double beta2 = std::hypot(hjj, hn);
-            k = j + 1;
-            if (beta2 == 0) break;
-            cs[j] = hjj / beta2;
-            sn[j] = hn / beta2;
-            H[j + (size_t)j * (m + 1)] = beta2;
-            H[j + 1 + (size_t)j * (m + 1)] = 0;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] *= cs[j];
-            st.relres = std::abs(g[j + 1]) / bnorm;
-            if (hn > 0) hip_check(blas_axpby(1.0 / hn, nullptr, false, w, 0.0, w, n, s), "normalise");
-            if (st.relres < tol || hn == 0) break;
+            raw.resize((size_t)q * W.cstride);
+            d2h(s, raw.data(), W.Ci(0), raw.size());
+            const double rel0 = st.relres;
+            for (int j = j0; j < j0 + q; ++j) {
+                const double hn = cgs2_coeffs(raw.data() + (size_t)(j - j0) * W.cstride, j + 1, h);
+                for (int i = 0; i <= j; ++i) H[i + (size_t)j * (m + 1)] = h[i];
+                H[j + 1 + (size_t)j * (m + 1)] = hn;
+                for (int i = 0; i < j; ++i) {  // previous Givens rotations
+                    const double t1 = H[i + (size_t)j * (m + 1)], t2 = H[i + 1 + (size_t)j * (m + 1)];
+                    H[i + (size_t)j * (m + 1)] = cs[i] * t1 + sn[i] * t2;
+                    H[i + 1 + (size_t)j * (m + 1)] = -sn[i] * t1 + cs[i] * t2;
+                }
+                const double hjj = H[j + (size_t)j * (m + 1)];
+                const double beta2 = std::hypot(hjj, hn);
+                k = j + 1;
+                if (beta2 == 0) { stop = true; break; }
+                cs[j] = hjj / beta2;
+                sn[j] = hn / beta2;
+                H[j + (size_t)j * (m + 1)] = beta2;
+                H[j + 1 + (size_t)j * (m + 1)] = 0;
+                g[j + 1] = -sn[j] * g[j];
+                g[j] *= cs[j];
+                st.relres = std::abs(g[j + 1]) / bnorm;
+                if (st.relres < tol || hn == 0) { stop = true; break; }
+            }
+            j0 += q;
+            if (!stop) {
+                const double rate = st.relres > 0 && rel0 > 0 ? std::pow(st.relres / rel0, 1.0 / q) : 1.0;
+                const double need = rate < 1.0 ? std::ceil(std::log(tol / st.relres) / std::log(rate)) : 16.0;
+                batch = (int)std::min(16.0, std::max(1.0, need));
+            }
         }
         // x += V[0..k) * R^{-1} g
         std::vector<double> y(k, 0.0);
@@ -247,7 +274,6 @@ KrylovStats gmres(hipStream_t s, const LinOp& A, const double* b, double* x, dou
         for (auto& v : y) v = -v;  // gemv_sub subtracts
         h2d(s, W.dh.d(), y.data(), k);
         hip_check(blas_gemv_sub(W.V.d(), n, k, W.dh.d(), x, n, s), "update");
-        hip_check(hipStreamSynchronize(s), "sync");
         if (st.relres < tol) {
             // confirm with the true residual on the next pass
             continue;
@@ -256,7 +282,7 @@ KrylovStats gmres(hipStream_t s, const LinOp& A, const double* b, double* x, dou
     if (!st.converged && st.iters >= maxit) {
         // final true residual
         A.apply(x, W.Vi(0));
-        hip_check(blas_axpby(1.0, nullptr, false, b, -1.0, W.Vi(0), n, s), "residual");
+        hip_check(blas_axpby(1.0, nullptr, AXPBY_DA, b, -1.0, W.Vi(0), n, s), "residual");
         st.relres = dev_norm(s, W, W.Vi(0)) / bnorm;
         st.converged = st.relres < tol;
     }
@@ -284,7 +310,7 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
     while (st.iters < maxit) {
         A64.apply(x, r.d());
         ++st.iters;
-        hip_check(blas_axpby(1.0, nullptr, false, b, -1.0, r.d(), n, s), "residual");
+        hip_check(blas_axpby(1.0, nullptr, AXPBY_DA, b, -1.0, r.d(), n, s), "residual");
         const double rn = dev_norm(s, W, r.d());
         st.relres = rn / bnorm;
         if (st.relres < tol) { st.converged = true; break; }
@@ -293,7 +319,7 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
         const int budget = std::min(maxit - st.iters, m + 1);
         KrylovStats in = gmres(s, Alo, r.d(), dx.d(), std::max(inner_tol, 0.5 * tol * bnorm / rn), budget, m);
         st.iters += in.iters;
-        hip_check(blas_axpby(1.0, nullptr, false, dx.d(), 1.0, x, n, s), "update");
+        hip_check(blas_axpby(1.0, nullptr, AXPBY_DA, dx.d(), 1.0, x, n, s), "update");
     }
     return st;
 }
@@ -302,7 +328,9 @@ KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol,
     // Thick-restart Lanczos (Wu-Simon): with full CGS2 re-orthogonalisation the projected matrix
     // T = V^T A V is assembled column by column from the orthogonalisation coefficients; at a
     // restart the lowest `keep` Ritz vectors (T diagonal there) and the next Lanczos direction
-    // form the new basis, so the search space is not thrown away.
+    // form the new basis, so the search space is not thrown away.  A cycle's steps run on the
+    // device back to back; the host reads their coefficients once, at the end of the cycle, and
+    // assembles T there (steps after a breakdown are computed and then ignored).
     KrylovStats st;
     const long n = A.n;
     m = (int)std::max<long>(2, std::min<long>(m, n));
@@ -313,23 +341,29 @@ KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol,
     Y.ensure(sizeof(double) * (size_t)n * keep);
     double nx = dev_norm(s, W, x);
     if (nx == 0) throw InvalidArg("lanczos: zero start vector");
-    hip_check(blas_axpby(1.0 / nx, nullptr, false, x, 0.0, W.Vi(0), n, s), "scale");
-    std::vector<double> T((size_t)m * m, 0.0), h, w, Z;
+    hip_check(blas_axpby(1.0 / nx, nullptr, AXPBY_DA, x, 0.0, W.Vi(0), n, s), "scale");
+    std::vector<double> T((size_t)m * m, 0.0), h, w, Z, raw;
     int j0 = 0;  // basis vectors already present (and their T block) at the start of a cycle
     while (true) {
-        int k = 0;
-        double beta = 0;
+        int jend = j0;  // every cycle takes at least one step (j0 <= m - 2)
         for (int j = j0; j < m; ++j) {
-            double* wv = W.Vi(j + 1);
-            A.apply(W.Vi(j), wv);
+            A.apply(W.Vi(j), W.Vi(j + 1));
             ++st.iters;
-            beta = cgs2(s, W, j + 1, wv, h);
+            cgs2_dev(s, W, j + 1, W.Vi(j + 1), W.Ci(j));
+            jend = j + 1;
+            if (st.iters >= maxit) break;
+        }
+        raw.resize((size_t)(jend - j0) * W.cstride);
+        d2h(s, raw.data(), W.Ci(j0), raw.size());
+        int k = j0;
+        double beta = 0;
+        for (int j = j0; j < jend; ++j) {
+            beta = cgs2_coeffs(raw.data() + (size_t)(j - j0) * W.cstride, j + 1, h);
             for (int i = 0; i <= j; ++i) { T[i + (size_t)j * m] = h[i]; T[j + (size_t)i * m] = h[i]; }
             k = j + 1;
-            if (beta <= 1e-14 * std::max(std::abs(h[j]), 1e-300) || j + 1 == m || st.iters >= maxit) break;
+            if (beta <= 1e-14 * std::max(std::abs(h[j]), 1e-300) || j + 1 == jend) break;
             T[j + 1 + (size_t)j * m] = beta;
             T[j + (size_t)(j + 1) * m] = beta;
-            hip_check(blas_axpby(1.0 / beta, nullptr, false, wv, 0.0, wv, n, s), "normalise");
         }
         // Ritz pairs of the k x k projection (ascending)
         std::vector<double> Tk((size_t)k * k);
@@ -344,26 +378,27 @@ KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol,
         st.relres = std::abs(beta * Z[(k - 1) + (size_t)imin * k]) / std::max(std::abs(theta), 1e-300);
         const bool done = st.relres <= tol || k < 2 || st.iters >= maxit || beta <= 1e-14 * std::max(std::abs(theta), 1e-300);
         const int nk = done ? 1 : std::min(keep, k - 1);
-        // Ritz vectors Y_i = V[0..k) Z[:, order[i]] for the lowest nk
-        for (int i = 0; i < nk; ++i) {
-            std::vector<double> z(k);
-            for (int r = 0; r < k; ++r) z[r] = -Z[r + (size_t)order[i] * k];
-            double* yi = Y.d() + (size_t)i * n;
-            h2d(s, W.dh.d(), z.data(), k);
-            hip_check(launch_zero(yi, n, s), "zero");  // fresh memory: not 0 * (possibly NaN) garbage
-            hip_check(blas_gemv_sub(W.V.d(), n, k, W.dh.d(), yi, n, s), "ritz");
+        // Ritz vectors Y_i = V[0..k) Z[:, order[i]] for the lowest nk (coefficients in one transfer)
+        {
+            std::vector<double> z((size_t)nk * k);
+            for (int i = 0; i < nk; ++i)
+                for (int r = 0; r < k; ++r) z[(size_t)i * k + r] = -Z[r + (size_t)order[i] * k];
+            h2d(s, W.coef.d(), z.data(), z.size());
+            for (int i = 0; i < nk; ++i) {
+                double* yi = Y.d() + (size_t)i * n;
+                hip_check(launch_zero(yi, n, s), "zero");  // fresh memory: not 0 * (possibly NaN) garbage
+                hip_check(blas_gemv_sub(W.V.d(), n, k, W.coef.d() + (size_t)i * k, yi, n, s), "ritz");
+            }
         }
         if (done) {
             d2d(s, x, Y.d(), n);
             nx = dev_norm(s, W, x);
-            hip_check(blas_axpby(1.0 / nx, nullptr, false, x, 0.0, x, n, s), "scale");
+            hip_check(blas_axpby(1.0 / nx, nullptr, AXPBY_DA, x, 0.0, x, n, s), "scale");
             st.converged = st.relres <= tol;
             break;
         }
-        // new basis: the nk Ritz vectors and the normalised next Lanczos vector
-        double* next = W.Vi(k);
-        hip_check(blas_axpby(1.0 / beta, nullptr, false, next, 0.0, next, n, s), "normalise");
-        d2d(s, W.Vi(nk), next, n);
+        // new basis: the nk Ritz vectors and the next Lanczos vector (normalised by its step)
+        d2d(s, W.Vi(nk), W.Vi(k), n);
         for (int i = 0; i < nk; ++i) d2d(s, W.Vi(i), Y.d() + (size_t)i * n, n);
         std::fill(T.begin(), T.end(), 0.0);
         for (int i = 0; i < nk; ++i) T[i + (size_t)i * m] = w[order[i]];
@@ -371,7 +406,7 @@ KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol,
     }
     // true residual ||A x - theta x|| / |theta|
     A.apply(x, W.w.d());
-    hip_check(blas_axpby(-st.value, nullptr, false, x, 1.0, W.w.d(), n, s), "resid");
+    hip_check(blas_axpby(-st.value, nullptr, AXPBY_DA, x, 1.0, W.w.d(), n, s), "resid");
     st.relres = dev_norm(s, W, W.w.d()) / std::max(std::abs(st.value), 1e-300);
     st.converged = st.relres <= std::max(tol, 1e-14) * 10;
     return st;
@@ -494,15 +529,15 @@ KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, 
             // tr(v rho_in) stays on the device and scales rho_out there (no host round trip)
             hip_check(blas_multidot(rhoT.d(), 0, 1, v, n, dd.d(), dd.d() + 4, s), "trace");
             d2d(s, out, v, n);
-            hip_check(blas_axpby(-1.0, nullptr, false, tmp.d(), 1.0, out, n, s), "1-T");
-            hip_check(blas_axpby(1.0, dd.d(), false, rho_out, 1.0, out, n, s), "projector");
+            hip_check(blas_axpby(-1.0, nullptr, AXPBY_DA, tmp.d(), 1.0, out, n, s), "1-T");
+            hip_check(blas_axpby(1.0, dd.d(), AXPBY_DA, rho_out, 1.0, out, n, s), "projector");
         };
         return op;
     };
     // right-hand side: x - tr(x rho_in) rho_out  (InvEBlockFunctions.hpp:243, :252)
     const double trx = trace_in(x);
     d2d(s, rhs.d(), x, n);
-    hip_check(blas_axpby(-trx, nullptr, false, rho_out, 1.0, rhs.d(), n, s), "rhs");
+    hip_check(blas_axpby(-trx, nullptr, AXPBY_DA, rho_out, 1.0, rhs.d(), n, s), "rhs");
     if (y0) d2d(s, y, y0, n);
     else hip_check(launch_zero(y, n, s), "zero y");
     if (maxit <= 0) maxit = (int)std::min<long>(std::max<long>(20 * n, 1000), 100000);
@@ -559,10 +594,10 @@ int EnvSolver::fixed_point(double* x, double tol, int maxit, double* delta) {
     for (int round = 0; round < 6; ++round) {
         const double tr = dotd(id.d(), x);
         if (tr == 0) throw Numerical("fixed point: iterate has zero trace");
-        hip_check(blas_axpby(1.0 / tr, nullptr, false, x, 0.0, x, n, s), "normalise");
+        hip_check(blas_axpby(1.0 / tr, nullptr, AXPBY_DA, x, 0.0, x, n, s), "normalise");
         tm_apply(x, r0.d());
         ++its;
-        hip_check(blas_axpby(-1.0, nullptr, false, x, 1.0, r0.d(), n, s), "T(x) - x");
+        hip_check(blas_axpby(-1.0, nullptr, AXPBY_DA, x, 1.0, r0.d(), n, s), "T(x) - x");
         dl = std::sqrt(std::max(dotd(r0.d(), r0.d()), 0.0));
         if (dl < tol || its >= maxit) break;
         // l chain: rho_in = R slot, rho_out = L slot; r chain: rho_in = L, rho_out = R
@@ -570,7 +605,7 @@ int EnvSolver::fixed_point(double* x, double tol, int maxit, double* delta) {
         const double* Rp = dir == IMPS_LEFT ? id.d() : x;
         KrylovStats st = solve(r0.d(), Lp, Rp, nullptr, dx.d(), std::max(0.1 * tol / std::max(dl, 1e-300), 1e-13), maxit - its, 200);
         its += st.iters;
-        hip_check(blas_axpby(1.0, nullptr, false, dx.d(), 1.0, x, n, s), "update");
+        hip_check(blas_axpby(1.0, nullptr, AXPBY_DA, dx.d(), 1.0, x, n, s), "update");
     }
     if (delta) *delta = dl;
     return its;
