@@ -14,6 +14,7 @@ namespace imps {
 constexpr int kBlasThreads = 256;
 constexpr int kRedChunks = 64;     // at most this many row chunks per reduction
 constexpr long kChunkRows = 2048;  // at least this many rows per chunk (fewer chunks for short vectors)
+constexpr long kMaxGrid = 2048;    // workgroups of the row-parallel kernels (grid-stride beyond)
 
 // sum over the workgroup of one double per thread: wave64 shuffle tree, then the wave sums in order;
 // result valid in thread 0
@@ -36,6 +37,7 @@ __global__ __launch_bounds__(kBlasThreads) void multidot_partial(const double* _
     const long r0 = c * per, r1 = min(n, r0 + per);
     const double* __restrict__ v = V + i * ldv;
     double s = 0;
+#pragma unroll 4
     for (long r = r0 + threadIdx.x; r < r1; r += kBlasThreads) s += v[r] * w[r];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) partial[(long)c * nb + i] = s;
@@ -60,6 +62,66 @@ __global__ __launch_bounds__(kBlasThreads) void gemv_sub(const double* __restric
     }
 }
 
+// Summed chunk partials of inner product i (chunk order); every consumer sums them the same way.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int nch, int nb, int i) {
+    double t = 0;
+#pragma unroll 4
+    for (int c = 0; c < nch; ++c) t += p[(long)c * nb + i];
+    return t;
+}
+
+// w[r] -= sum_i V[i*ldv + r] * h[i], h[i] summed from the multidot chunk partials p by every
+// workgroup into LDS (workgroup 0 also stores h at hout); with qpart, qpart[workgroup] = that
+// workgroup's part of ||w||^2 after the update.  Dynamic LDS: nb + 4 doubles.
+__global__ __launch_bounds__(kBlasThreads) void gemv_sub_partials(const double* __restrict__ V, long ldv, int nb, const double* __restrict__ p,
+                                                                  int nch, double* __restrict__ hout, double* __restrict__ w, long n,
+                                                                  double* __restrict__ qpart) {
+    extern __shared__ double hs[];
+    for (int i = threadIdx.x; i < nb; i += kBlasThreads) {
+        const double t = sum_partials(p, nch, nb, i);
+        hs[i] = t;
+        if (blockIdx.x == 0) hout[i] = t;
+    }
+    __syncthreads();
+    double q = 0;
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
+        double s = 0;
+        int i = 0;
+        for (; i + 4 <= nb; i += 4) {  // four independent loads in flight
+            const double a0 = V[i * ldv + r], a1 = V[(i + 1) * ldv + r], a2 = V[(i + 2) * ldv + r], a3 = V[(i + 3) * ldv + r];
+            s += a0 * hs[i];
+            s += a1 * hs[i + 1];
+            s += a2 * hs[i + 2];
+            s += a3 * hs[i + 3];
+        }
+        for (; i < nb; ++i) s += V[i * ldv + r] * hs[i];
+        const double x = w[r] - s;
+        w[r] = x;
+        q += x * x;
+    }
+    if (!qpart) return;
+    q = block_sum(q, hs + nb);
+    if (threadIdx.x == 0) qpart[blockIdx.x] = q;
+}
+
+// y[r] /= sqrt(Q) with Q = sum of the nq partials (fixed order, in every workgroup; y = 0 when
+// Q <= 0); workgroup 0 stores Q at qout.
+__global__ __launch_bounds__(kBlasThreads) void scale_inv_norm(const double* __restrict__ qpart, int nq, double* __restrict__ qout,
+                                                               double* __restrict__ y, long n) {
+    __shared__ double sh[kBlasThreads / 64 + 1];
+    double t = 0;
+    for (int b = threadIdx.x; b < nq; b += kBlasThreads) t += qpart[b];
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) {
+        sh[kBlasThreads / 64] = t;
+        if (blockIdx.x == 0) *qout = t;
+    }
+    __syncthreads();
+    const double Q = sh[kBlasThreads / 64];
+    const double a = Q > 0.0 ? 1.0 / sqrt(Q) : 0.0;
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) y[r] = a != 0.0 ? a * y[r] : 0.0;
+}
+
 // y[r] = a*x[r] + b*y[r]  (a read from the device scalar da when given, as `mode` says)
 __global__ void axpby_kernel(double a, const double* __restrict__ da, const double* __restrict__ x, double b, double* __restrict__ y,
                              long n, int mode) {
@@ -76,7 +138,7 @@ __global__ void axpby_kernel(double a, const double* __restrict__ da, const doub
 
 static unsigned grid_for(long n) {
     long g = (n + kBlasThreads - 1) / kBlasThreads;
-    return (unsigned)(g < 1 ? 1 : (g > 2048 ? 2048 : g));
+    return (unsigned)(g < 1 ? 1 : (g > kMaxGrid ? kMaxGrid : g));
 }
 
 hipError_t blas_multidot(const double* V, long ldv, int nb, const double* w, long n, double* out, double* scratch, hipStream_t s) {
@@ -86,6 +148,32 @@ hipError_t blas_multidot(const double* V, long ldv, int nb, const double* w, lon
     // one chunk: the partials are the results
     hipLaunchKernelGGL(multidot_partial, dim3(nch, nb), dim3(kBlasThreads), 0, s, V, ldv, nb, w, n, nch == 1 ? out : scratch);
     if (nch > 1) hipLaunchKernelGGL(multidot_final, dim3((nb + 63) / 64), dim3(64), 0, s, scratch, nch, nb, out);
+    return hipGetLastError();
+}
+
+int blas_chunks(long n) { return (int)std::min<long>(kRedChunks, std::max<long>(1, n / kChunkRows)); }
+int blas_gemv_grid(long n) { return (int)grid_for(n); }
+
+hipError_t blas_multidot_partials(const double* V, long ldv, int nb, const double* w, long n, double* partials, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    if (nb > 65535) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(multidot_partial, dim3(blas_chunks(n), nb), dim3(kBlasThreads), 0, s, V, ldv, nb, w, n, partials);
+    return hipGetLastError();
+}
+
+hipError_t blas_gemv_sub_partials(const double* V, long ldv, int nb, const double* partials, int nch, double* hout, double* w, long n,
+                                  double* qpart, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    nb = std::max(nb, 0);
+    if (nb > 8000) return hipErrorInvalidValue;  // h in LDS
+    hipLaunchKernelGGL(gemv_sub_partials, dim3(grid_for(n)), dim3(kBlasThreads), sizeof(double) * (nb + kBlasThreads / 64), s, V, ldv, nb,
+                       partials, nch, hout, w, n, qpart);
+    return hipGetLastError();
+}
+
+hipError_t blas_scale_inv_norm(const double* qpart, int nq, double* qout, double* y, long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_inv_norm, dim3(grid_for(n)), dim3(kBlasThreads), 0, s, qpart, nq, qout, y, n);
     return hipGetLastError();
 }
 
@@ -101,6 +189,6 @@ hipError_t blas_axpby(double a, const double* da, int mode, const double* x, dou
     return hipGetLastError();
 }
 
-size_t blas_scratch_doubles(int nb) { return (size_t)kRedChunks * (size_t)(nb < 1 ? 1 : nb); }
+size_t blas_scratch_doubles(int nb) { return (size_t)kRedChunks * (size_t)(nb < 1 ? 1 : nb) + kMaxGrid; }
 
 }  // namespace imps

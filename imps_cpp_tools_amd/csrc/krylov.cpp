@@ -139,12 +139,16 @@ double dev_norm(hipStream_t s, Work& W, const double* v) {
 // second-pass projections, ||w||^2 before the scaling) stay at dc for the host to collect later,
 // so consecutive steps queue on the stream without a host round trip.
 void cgs2_dev(hipStream_t s, Work& W, int k, double* w, double* dc) {
+    // five launches: inner products (chunk partials), update (sums them), the same again with the
+    // partials of ||w||^2, and the scaling (sums those)
+    const int nch = blas_chunks(W.n);
+    double* p = W.scratch.d();
+    double* qp = p + (size_t)nch * k;
     for (int pass = 0; pass < 2; ++pass) {
-        hip_check(blas_multidot(W.V.d(), W.n, k, w, W.n, dc + pass * k, W.scratch.d(), s), "multidot");
-        hip_check(blas_gemv_sub(W.V.d(), W.n, k, dc + pass * k, w, W.n, s), "gemv");
+        hip_check(blas_multidot_partials(W.V.d(), W.n, k, w, W.n, p, s), "multidot");
+        hip_check(blas_gemv_sub_partials(W.V.d(), W.n, k, p, nch, dc + pass * k, w, W.n, pass ? qp : nullptr, s), "gemv");
     }
-    hip_check(blas_multidot(w, 0, 1, w, W.n, dc + 2 * k, W.scratch.d(), s), "norm");
-    hip_check(blas_axpby(0.0, dc + 2 * k, AXPBY_INV_SQRT_DA, w, 0.0, w, W.n, s), "normalise");
+    hip_check(blas_scale_inv_norm(qp, blas_gemv_grid(W.n), dc + 2 * k, w, W.n, s), "normalise");
 }
 
 // Host side of cgs2_dev: h = the k projection coefficients (both passes summed) from the raw
