@@ -32,6 +32,9 @@ hipError_t blas_multidot_partials(const double* V, long ldv, int nb, const doubl
 // blas_gemv_grid(n) partials of ||w||^2 after the update
 hipError_t blas_gemv_sub_partials(const double* V, long ldv, int nb, const double* partials, int nch, double* hout, double* w, long n,
                                   double* qpart, hipStream_t s);
+// out = x - y + t z, t = the summed partials of one inner product (blas_multidot_partials, nb = 1)
+hipError_t blas_diff_plus_scaled(const double* x, const double* y, const double* z, const double* partials, int nch, double* out, long n,
+                                 hipStream_t s);
 // y /= sqrt(Q), Q = the summed nq partials (stored at qout; y = 0 when Q <= 0)
 hipError_t blas_scale_inv_norm(const double* qpart, int nq, double* qout, double* y, long n, hipStream_t s);
 

@@ -122,6 +122,15 @@ __global__ __launch_bounds__(kBlasThreads) void scale_inv_norm(const double* __r
     for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) y[r] = a != 0.0 ? a * y[r] : 0.0;
 }
 
+// out[r] = x[r] - y[r] + t * z[r], t = the summed nch partials of one inner product (every
+// workgroup sums them in the same order)
+__global__ __launch_bounds__(kBlasThreads) void diff_plus_scaled(const double* __restrict__ x, const double* __restrict__ y,
+                                                                 const double* __restrict__ z, const double* __restrict__ p, int nch,
+                                                                 double* __restrict__ out, long n) {
+    const double t = sum_partials(p, nch, 1, 0);
+    for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) out[r] = (x[r] - y[r]) + t * z[r];
+}
+
 // y[r] = a*x[r] + b*y[r]  (a read from the device scalar da when given, as `mode` says)
 __global__ void axpby_kernel(double a, const double* __restrict__ da, const double* __restrict__ x, double b, double* __restrict__ y,
                              long n, int mode) {
@@ -174,6 +183,13 @@ hipError_t blas_gemv_sub_partials(const double* V, long ldv, int nb, const doubl
 hipError_t blas_scale_inv_norm(const double* qpart, int nq, double* qout, double* y, long n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(scale_inv_norm, dim3(grid_for(n)), dim3(kBlasThreads), 0, s, qpart, nq, qout, y, n);
+    return hipGetLastError();
+}
+
+hipError_t blas_diff_plus_scaled(const double* x, const double* y, const double* z, const double* partials, int nch, double* out, long n,
+                                 hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(diff_plus_scaled, dim3(grid_for(n)), dim3(kBlasThreads), 0, s, x, y, z, partials, nch, out, n);
     return hipGetLastError();
 }
 

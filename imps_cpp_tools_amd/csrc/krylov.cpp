@@ -530,11 +530,9 @@ KrylovStats EnvSolver::solve(const double* x, const double* L, const double* R, 
                 in = o;
                 if (k < ncell - 1) off += bond_elems[k];
             }
-            // tr(v rho_in) stays on the device and scales rho_out there (no host round trip)
-            hip_check(blas_multidot(rhoT.d(), 0, 1, v, n, dd.d(), dd.d() + 4, s), "trace");
-            d2d(s, out, v, n);
-            hip_check(blas_axpby(-1.0, nullptr, AXPBY_DA, tmp.d(), 1.0, out, n, s), "1-T");
-            hip_check(blas_axpby(1.0, dd.d(), AXPBY_DA, rho_out, 1.0, out, n, s), "projector");
+            // tr(v rho_in) stays on the device as chunk partials, summed by the combining kernel
+            hip_check(blas_multidot_partials(rhoT.d(), 0, 1, v, n, dd.d() + 4, s), "trace");
+            hip_check(blas_diff_plus_scaled(v, tmp.d(), rho_out, dd.d() + 4, blas_chunks(n), out, n, s), "1 - T + projector");
         };
         return op;
     };

@@ -242,11 +242,17 @@ class VUMPS:
         return y
 
     def _transpose_blocks(self, x, b):
-        lay = self.blay[b]
-        t = self.torch.empty_like(x)
-        for o, s in zip(lay.offsets, lay.rows):
-            t[o:o + s * s] = x[o:o + s * s].view(s, s).t().reshape(-1)
-        return t
+        """Blockwise transpose on the block-diagonal layout of bond b (one gather)."""
+        k = ("transpose", b)
+        perm = self._units.get(k)
+        if perm is None:
+            lay = self.blay[b]
+            idx = np.empty(lay.nelem, np.int64)
+            for o, s in zip(lay.offsets, lay.rows):
+                idx[o:o + s * s] = o + np.arange(s * s).reshape(s, s).T.reshape(-1)
+            perm = self.torch.from_numpy(idx).cuda()
+            self._units[k] = perm
+        return x[perm]
 
     def _init_environments(self):
         N = self.N
@@ -306,6 +312,29 @@ class VUMPS:
             out[o:o + r * r].reshape(r, r)[np.diag_indices(r)] = sc
         return self._dev(out)
 
+    def _diagonal(self, lay: BlockLayout, key):
+        """(padded gather index, mask, flat diagonal index, sector of each diagonal entry) of the
+        square blocks of `lay`, built once per key: G[index] * mask summed over rows 1 is the
+        per-block trace in a fixed order."""
+        k = ("diag",) + tuple(key)
+        v = self._units.get(k)
+        if v is None:
+            rmax = max([int(r) for r in lay.rows] + [1])
+            pad = np.zeros((len(lay.rows), rmax), np.int64)
+            mask = np.zeros((len(lay.rows), rmax))
+            flat, sec = [], []
+            for q, (o, r) in enumerate(zip(lay.offsets, lay.rows)):
+                d = o + np.arange(r) * (r + 1)
+                pad[q, :r], mask[q, :r] = d, 1.0
+                flat.append(d)
+                sec.append(np.full(r, q))
+            flat = np.concatenate(flat) if flat else np.zeros(0, np.int64)
+            sec = np.concatenate(sec) if sec else np.zeros(0, np.int64)
+            t = self.torch
+            v = (t.from_numpy(pad).cuda(), self._dev(mask), t.from_numpy(flat.astype(np.int64)).cuda(), t.from_numpy(sec.astype(np.int64)).cuda())
+            self._units[k] = v
+        return v
+
     def _unit(self, lay: BlockLayout, key):
         """Identity on the block-diagonal layout `lay`, built once per key."""
         k = ("unit",) + tuple(key)
@@ -325,12 +354,12 @@ class VUMPS:
         kind_g = TM_LEFT_UNITY if direction == LEFT else TM_RIGHT_UNITY
         for it in range(maxit):
             G, Gl = self._prod(kind_g, lay, X, lay, X, key + ("G",))
-            if it == 0:  # per-sector scaling by the Frobenius norm, sqrt(tr G_q)
-                tr = []
-                for o, r in zip(Gl.offsets, Gl.rows):
-                    tr.append(float(G[o:o + r * r].view(r, r).diagonal().sum().item()))
-                scale = [1.0 / np.sqrt(v) if v > 0 else 0.0 for v in tr]
-                S = self._scaled_identity(Gl, scale)
+            if it == 0:  # per-sector scaling by the Frobenius norm, sqrt(tr G_q), on the device
+                pad, mask, flat, sec = self._diagonal(Gl, key)
+                tr = (G[pad] * mask).sum(1)
+                scale = t.where(tr > 0, 1.0 / tr.clamp_min(1e-300).sqrt(), t.zeros_like(tr))
+                S = t.zeros(Gl.nelem, dtype=t.float64, device=G.device)
+                S[flat] = scale[sec]
                 X, _ = (self._prod(MPS_MAT, lay, X, Gl, S, key + ("s",)) if direction == LEFT
                         else self._prod(MAT_MPS, Gl, S, lay, X, key + ("s",)))
                 continue
