@@ -66,12 +66,23 @@ class VUMPS:
     def __init__(self, ctx: Context, model: int, N: int, QN, keys0, sizes0, params, prec: int = FP64, seed: int = 0,
                  eig_tol_max: float = 1e-6, eig_tol_min: float = 5e-16, env_tol: float = 5e-15, env_fac: float = 2.0,
                  krylov: int = 20, eig_maxit: int = 100000, env_krylov: int = 100, env_maxit: int = 20000, polar_start: bool = True,
-                 plans: Optional[Dict] = None):
+                 plans: Optional[Dict] = None, mmax: Optional[int] = None, dm_max: int = 20, expand_pause: int = 5,
+                 expand_thresh: float = 1e-5, lamthresh: float = 1e-12):
         """`plans`: the plan cache of another VUMPS over the same model and bond structure, reused
-        (plans hold structure only; every use sets its operator data)."""
+        (plans hold structure only; every use sets its operator data).
+
+        Bond growth (VUMPS_main.cpp:273-306): with `mmax` set, after each outer iteration every bond
+        whose total dimension is below mmax grows by ExpandFromH with at most min(dm_max, mmax -
+        total) states, when at least `expand_pause` iterations passed since its last growth, the
+        gauge error at the start of the iteration is below `expand_thresh` and some sector's
+        smallest Schmidt value exceeds `lamthresh` (reference defaults: dmmax 20, pause 5,
+        exthresh 1e-5, lamthresh 1e-12)."""
         import torch
         self.torch = torch
         self.ctx, self.prec, self.N = ctx, prec, N
+        self.model, self.QN = model, list(QN)
+        self.mmax, self.dm_max, self.expand_pause, self.expand_thresh, self.lamthresh = mmax, dm_max, expand_pause, expand_thresh, lamthresh
+        self.expand_ct = [0] * N
         self.d, self.nsym = model_d(model), model_nsym(model)
         self.bonds = make_dims(model, N, QN, keys0, sizes0)
         self.blay = [BlockLayout.diag(self.nsym, k, s) for (k, s) in self.bonds]
@@ -380,6 +391,7 @@ class VUMPS:
         F, G = np.zeros(N), np.zeros(N)
         e_cell = 0.0
         it0, env0 = self.eig_iters, self.env_iters
+        exp_prec = float(np.max(self.prec_n))
         for n in range(N):
             left = self._b(n - 1)
             tol = min(max(self.prec_n[n] / 100.0, self.eig_tol_min), self.eig_tol_max)
@@ -427,10 +439,71 @@ class VUMPS:
             self.RE[self._b(n + 2)] = self._right_env(n + 2, etol)
             self.RE[self._b(n + 1)] = (self._tm(RIGHT, self.AR[self._b(n + 1)], self._b(n + 1), self.RE[self._b(n + 2)], self._b(n + 2),
                                                 self._b(n + 1)) + self._optm_right(n + 1))
+        self._maybe_expand(exp_prec)
         torch.cuda.synchronize()
         self.iteration = getattr(self, "iteration", 0) + 1
         return VumpsStats(self.iteration, e_cell / N, list(F), list(G), time.perf_counter() - t0, self.eig_iters - it0,
                           self.env_iters - env0)
+
+    # ------------------------------------------------------------------ bond growth
+    def bond_dims(self) -> List[int]:
+        return [int(np.sum(sz)) for (_, sz) in self.bonds]
+
+    def expand(self, b: int, dm: int) -> int:
+        """ExpandFromH on bond b (between AL[b-1] and AR[b], carrying C[b-1]): at most dm states
+        from the two-site residual H (AL C AR) projected on the null spaces (expand.py); the state
+        is unchanged (C padded with zeros), AR[b-1] and AL[b] get zero columns / rows for the new
+        states, and the environments on bond b are rebuilt from their neighbours as in
+        VUMPS_main.cpp:286-301.  Returns the number of states added."""
+        from . import i2k
+        from .expand import append_columns, append_rows, diag_blocks, expand_from_h, flatten_diag, flatten_mps, grow_bond, mps_blocks
+        N = self.N
+        if N < 2:
+            raise ValueError("bond growth needs a unit cell of at least two sites")
+        b = self._b(b)
+        l, r = self._b(b - 1), b
+        host = lambda t: t.cpu().numpy()  # noqa: E731
+        AL_l = mps_blocks(self.slay[l], host(self.AL[l]))
+        AR_r = mps_blocks(self.slay[r], host(self.AR[r]))
+        AR_l = mps_blocks(self.slay[l], host(self.AR[l]))
+        AL_r = mps_blocks(self.slay[r], host(self.AL[r]))
+        Cb = diag_blocks(self.blay[b], host(self.C[l]))
+        dq = i2k(self.model, N, self.QN)
+        add, new_cols, new_rows, _ = expand_from_h(self.H, dq, AL_l, Cb, AR_r, dm)
+        if not add:
+            return 0
+        keys, sizes = grow_bond(self.bonds[b][0], self.bonds[b][1], add)
+        self.bonds[b] = (keys, sizes)
+        self.blay[b] = BlockLayout.diag(self.nsym, keys, sizes)
+        for n in (l, r):
+            self.slay[n] = mps_structure(self.model, N, self.QN, self.bonds[n], self.bonds[self._b(n + 1)])
+        self.AL[l] = self._dev(flatten_mps(self.slay[l], append_columns(AL_l, new_cols)))
+        self.AR[l] = self._dev(flatten_mps(self.slay[l], AR_l))
+        self.AR[r] = self._dev(flatten_mps(self.slay[r], append_rows(AR_r, new_rows)))
+        self.AL[r] = self._dev(flatten_mps(self.slay[r], AL_r))
+        self.C[l] = self._dev(flatten_diag(self.blay[b], Cb))
+        # structures changed: plans and device index tables are rebuilt on demand
+        self._plans = {}
+        self._units = {}
+        # the environments on bond b from the (unchanged) neighbouring bonds
+        self.LE[b] = self._tm(LEFT, self.AL[l], l, self.LE[l], l, b) + self._optm_left(l - 1)
+        self.RE[b] = self._tm(RIGHT, self.AR[r], r, self.RE[self._b(r + 1)], self._b(r + 1), r) + self._optm_right(r)
+        return int(sum(add.values()))
+
+    def _maybe_expand(self, exp_prec: float):
+        """The growth policy of VUMPS_main.cpp:273-306, applied to every bond after an iteration."""
+        from .expand import schmidt_values
+        if self.mmax is None:
+            return
+        for b in range(self.N):
+            total = int(np.sum(self.bonds[b][1]))
+            lam = schmidt_values(self.blay[b], self.C[self._b(b - 1)].cpu().numpy())
+            minlam = max((float(v.min()) for v in lam.values() if v.size), default=0.0)
+            if self.expand_ct[b] >= self.expand_pause and exp_prec < self.expand_thresh and total < self.mmax and minlam > self.lamthresh:
+                self.expand_ct[b] = 0
+                self.expand(b, min(self.dm_max, self.mmax - total))
+            else:
+                self.expand_ct[b] += 1
 
     def energy(self) -> float:
         b = 0

@@ -75,3 +75,75 @@ def test_random_state_matches_oracle():
             assert g.shape == ref.data.shape
             assert np.abs(g - ref.data).max() <= 1e-12
     torch.cuda.synchronize()
+
+
+def _small_xxz(**kw):
+    """Heisenberg chain from a D = 9 bond (5 sectors), for the bond-growth tests."""
+    from imps_cpp_tools_amd.vumps import VUMPS
+    keys0 = np.array([[2 * (i - 2)] for i in range(5)], np.int32)
+    sizes0 = np.array([1, 2, 3, 2, 1], np.int32)
+    return VUMPS(im.Context(0), im.XXZ, GOLD["N"], GOLD["QN"], keys0, sizes0, GOLD["params"], seed=GOLD["seed"], **kw)
+
+
+def _gauge_error(v):
+    """max_n ||AL[n] C[n] - C[n-1] AR[n]||_max on the device."""
+    err = 0.0
+    for n in range(v.N):
+        a, _ = v._prod(im.MPS_MAT, v.slay[n], v.AL[n], v.blay[v._b(n + 1)], v.C[n], ("chk_alc", n))
+        b, _ = v._prod(im.MAT_MPS, v.blay[n], v.C[v._b(n - 1)], v.slay[n], v.AR[n], ("chk_car", n))
+        err = max(err, float((a - b).abs().max().item()))
+    return err
+
+
+def _isometry_error(v):
+    """max_n of the deviation of sum_s AL^T AL and sum_s AR AR^T from the identity."""
+    import torch
+    err = 0.0
+    for n in range(v.N):
+        for kind, A in ((im.TM_LEFT_UNITY, v.AL[n]), (im.TM_RIGHT_UNITY, v.AR[n])):
+            G, Gl = v._prod(kind, v.slay[n], A, v.slay[n], A, ("chk_iso", kind, n))
+            eye = torch.zeros_like(G)
+            for o, r in zip(Gl.offsets, Gl.rows):
+                eye[int(o):int(o) + int(r) * int(r)].view(int(r), int(r)).diagonal().fill_(1.0)
+            err = max(err, float((G - eye).abs().max().item()))
+    return err
+
+
+def test_expand_keeps_state_and_gauge():
+    """ExpandFromH on one bond (MPSBlockUtilities.hpp:407-608): C is padded with zeros, so the
+    energy, the gauge relation AL C = C AR and the isometries are those of the state before."""
+    v = _small_xxz()
+    for _ in range(4):
+        v.sweep()
+    e0, g0, D0 = v.energy(), _gauge_error(v), v.bond_dims()
+    added = v.expand(1, 6)
+    assert added == 6 and v.bond_dims() == [D0[0], D0[1] + 6]
+    assert abs(v.energy() - e0) <= 1e-12
+    assert abs(_gauge_error(v) - g0) <= 1e-13
+    assert _isometry_error(v) <= 1e-13
+    st = v.sweep()  # the iteration continues on the grown structure
+    assert np.isfinite(st.energy) and st.energy >= GOLD["bethe_ansatz"]
+
+
+def test_vumps_grows_bonds_to_mmax():
+    """The growth policy of VUMPS_main.cpp:273-306 takes both bonds from D = 9 to mmax = 40 and the
+    energy per site approaches the Bethe-ansatz value from above (the critical chain converges
+    slowly in F; the growth and the energy are the point here)."""
+    v = _small_xxz(mmax=40, dm_max=8, expand_pause=2, expand_thresh=1e-2)
+    hist = []
+    for _ in range(80):
+        st = v.sweep()
+        hist.append((st.energy, max(st.F), v.bond_dims()))
+        if min(v.bond_dims()) >= 40 and max(st.F) <= 1e-6:
+            break
+    e, F, dims = hist[-1]
+    print("sweeps", len(hist), "dims", dims, "energy %.12f" % e, "F %.2e" % F, flush=True)
+    print("bond dims by sweep", [h[2] for h in hist[::5]], flush=True)
+    assert dims == [40, 40]
+    assert F <= 1e-5
+    assert GOLD["bethe_ansatz"] <= e <= GOLD["bethe_ansatz"] + 1e-4
+    # energies after the last growth decrease (up to the solver tolerances)
+    last = max(i for i, h in enumerate(hist) if i == 0 or h[2] != hist[i - 1][2])
+    tail = [h[0] for h in hist[last + 2:]]
+    assert not tail or max(np.diff(tail), default=0.0) <= 1e-9
+    assert _isometry_error(v) <= 1e-12
