@@ -342,7 +342,11 @@ imps_status imps_eigsh(imps_plan P, imps_plan P64, double* x, double tol, int ma
         if (P->prec == IMPS_FP64 || !P64) {
             st = lanczos_lowest(s, op_of(P), x, tol, maxit, krylov);
         } else {
-            const double tlo = std::max(tol, P->prec == IMPS_FP32 ? 1e-6 : 1e-4);
+            // The low-precision phase stops at about 100x the relative rounding of its products
+            // (FP32 ~1e-7, BF16X3 ~1e-6); nearer that floor its Ritz vector carries rounding-
+            // correlated error (handing over at 1e-6, FP32 VUMPS at D = 64 stalled at a stationary
+            // point 2e-8 above the FP64 energy), and the FP64 phase finishes the solve.
+            const double tlo = std::max(tol, P->prec == IMPS_FP32 ? 1e-5 : 1e-4);
             KrylovStats a = lanczos_lowest(s, op_of(P), x, tlo, maxit, krylov);
             st = lanczos_lowest(s, op_of(P64), x, tol, std::max(1, maxit - a.iters), krylov);
             st.iters += a.iters;

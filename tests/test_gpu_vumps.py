@@ -6,7 +6,7 @@ VUMPS of example_config_XXZ.cfg at D = 64.  The fixture tests/golden/vumps_xxz_d
 oracle VUMPS energy at gradient norm F <= 1e-10.  At fixed bond structure VUMPS converges to one
 fixed point, so the GPU iteration (FP64, and BF16X3 operators under FP64 refinement) must reach the
 same energy; the energy error of a state with gradient norm F is O(F^2), so at F <= 1e-9 the two
-agree far below the 1e-12 tolerance.
+agree far below the 1e-12 tolerance.  FP32 and BF16X3 operators run under FP64 refinement.
 """
 import json
 import os
@@ -46,7 +46,7 @@ def _run(prec, f_target=1e-9, max_sweeps=600):
     return hist
 
 
-@pytest.mark.parametrize("prec", [im.FP64, im.BF16X3], ids=["fp64", "bf16x3"])
+@pytest.mark.parametrize("prec", [im.FP64, im.FP32, im.BF16X3], ids=["fp64", "fp32", "bf16x3"])
 def test_vumps_xxz_d64_energy(prec):
     hist = _run(prec)
     e, F = hist[-1]
