@@ -505,6 +505,55 @@ class VUMPS:
             else:
                 self.expand_ct[b] += 1
 
+    # ------------------------------------------------------------------ state files
+    def save(self, path: str):
+        """saveIMPS (MPSBlockUtilities.hpp:1212-1266) of the unit cell: Schmidt values, C, AL, AR in
+        the randMPS_LR cell convention (imps_io.py)."""
+        from .expand import schmidt_values
+        from .imps_io import write_imps
+        N = self.N
+        host = [(self.blay[self._b(n + 1)], self.C[n].cpu().numpy()) for n in range(N)]
+        lam = [schmidt_values(lay, c) for lay, c in host]
+        write_imps(path, lam, host, [(self.slay[n], self.AL[n].cpu().numpy()) for n in range(N)],
+                   [(self.slay[n], self.AR[n].cpu().numpy()) for n in range(N)])
+
+    def load(self, path: str):
+        """loadIMPS (MPSBlockUtilities.hpp:1268-1325): replace the state by a file's.  The bonds come
+        from the stored C sectors; the stored blocks are placed in this model's block structure
+        (blocks the file leaves out are zero); environments and plans are rebuilt."""
+        from .expand import diag_blocks, flatten_diag, flatten_mps, mps_blocks
+        from .imps_io import read_imps
+        _, C, AL, AR = read_imps(path, self.nsym)
+        N = self.N
+        if not (len(C) == len(AL) == len(AR) == N):
+            raise ValueError(f"state file holds a {len(AL)}-site cell, this VUMPS has {N} sites")
+        bonds = [None] * N
+        for n in range(N):
+            lay = C[n][0]
+            bonds[self._b(n + 1)] = (np.array(lay.qin, np.int32), np.array(lay.rows, np.int32))
+        blay = [BlockLayout.diag(self.nsym, k, s) for (k, s) in bonds]
+        slay = [mps_structure(self.model, N, self.QN, bonds[n], bonds[self._b(n + 1)]) for n in range(N)]
+
+        def embed(lay_new, lay_old, data):
+            blocks = mps_blocks(lay_old, data)
+            want = {(e, tuple(int(v) for v in q)): (int(r), int(c)) for e in range(lay_new.nlist)
+                    for q, r, c in zip(lay_new.entry(e).qin, lay_new.entry(e).rows, lay_new.entry(e).cols)}
+            for e, ent in enumerate(blocks):
+                for q, (_, m) in ent.items():
+                    if want.get((e, q)) != m.shape:
+                        raise ValueError(f"state file block {q} of physical index {e} does not fit the model's structure")
+            return self._dev(flatten_mps(lay_new, blocks))
+
+        self.bonds, self.blay, self.slay = bonds, blay, slay
+        self.AL = [embed(slay[n], *AL[n]) for n in range(N)]
+        self.AR = [embed(slay[n], *AR[n]) for n in range(N)]
+        self.C = [self._dev(flatten_diag(blay[self._b(n + 1)], diag_blocks(*C[n]))) for n in range(N)]
+        self._plans, self._units = {}, {}
+        self.LE, self.RE = {}, {}
+        self.prec_n = np.ones(N)
+        self.expand_ct = [0] * N
+        self._init_environments()
+
     def energy(self) -> float:
         b = 0
         _, e_cell = self._left_env(b, self.env_tol)
