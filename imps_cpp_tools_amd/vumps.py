@@ -67,7 +67,7 @@ class VUMPS:
                  eig_tol_max: float = 1e-6, eig_tol_min: float = 5e-16, env_tol: float = 5e-15, env_fac: float = 2.0,
                  krylov: int = 20, eig_maxit: int = 100000, env_krylov: int = 100, env_maxit: int = 20000, polar_start: bool = True,
                  plans: Optional[Dict] = None, mmax: Optional[int] = None, dm_max: int = 20, expand_pause: int = 5,
-                 expand_thresh: float = 1e-5, lamthresh: float = 1e-12):
+                 expand_thresh: float = 1e-5, lamthresh: float = 1e-12, concurrent: bool = True):
         """`plans`: the plan cache of another VUMPS over the same model and bond structure, reused
         (plans hold structure only; every use sets its operator data).
 
@@ -76,13 +76,20 @@ class VUMPS:
         total) states, when at least `expand_pause` iterations passed since its last growth, the
         gauge error at the start of the iteration is below `expand_thresh` and some sector's
         smallest Schmidt value exceeds `lamthresh` (reference defaults: dmmax 20, pause 5,
-        exthresh 1e-5, lamthresh 1e-12)."""
+        exthresh 1e-5, lamthresh 1e-12).
+
+        `concurrent`: the three eigensolves of a site step (H_AC, H_C on both bonds) read the same
+        environments and are independent; they run at once, each on its own stream from its own
+        host thread (each solve is a chain of latency-bound Krylov steps, so they overlap).  The
+        results are those of the sequential order."""
         import torch
         self.torch = torch
         self.ctx, self.prec, self.N = ctx, prec, N
         self.model, self.QN = model, list(QN)
         self.mmax, self.dm_max, self.expand_pause, self.expand_thresh, self.lamthresh = mmax, dm_max, expand_pause, expand_thresh, lamthresh
         self.expand_ct = [0] * N
+        self.concurrent = concurrent
+        self._side = None  # (contexts of the two H_C solves, thread pool), made on first use
         self.d, self.nsym = model_d(model), model_nsym(model)
         self.bonds = make_dims(model, N, QN, keys0, sizes0)
         self.blay = [BlockLayout.diag(self.nsym, k, s) for (k, s) in self.bonds]
@@ -287,19 +294,45 @@ class VUMPS:
             p64.set_operator(*ops)
         return p, p64
 
-    def _hc_plans(self, b):
+    def _hc_plans(self, b, side: int = -1):
+        """H_C plans of bond b on the main context, or (side 0 / 1) on the stream of the left /
+        right H_C solve of a concurrent site step."""
         b = self._b(b)
         a = self._b(b - 1)
         HL, HLl = self._redop(self.AL[a], a, LEFT)
-        mk = lambda prec: HCPlan(self.ctx, self.blay[b], self.blay[b], self.blay[b], HLl, self.slay[b], prec)  # noqa: E731
-        p = self._plan(("hc", b, self.prec), lambda: mk(self.prec))
+        ctx = self.ctx if side < 0 else self._side_contexts()[0][side]
+        mk = lambda prec: HCPlan(ctx, self.blay[b], self.blay[b], self.blay[b], HLl, self.slay[b], prec)  # noqa: E731
+        p = self._plan(("hc", b, self.prec, side, id(ctx)), lambda: mk(self.prec))
         ops = (self.LE[b], self.RE[b], HL, self.AR[b])
+        if side >= 0:
+            self.torch.cuda.synchronize()  # the operator data come from the main stream
         p.set_operator(*ops)
         p64 = None
         if self.prec != FP64:
-            p64 = self._plan(("hc", b, FP64), lambda: mk(FP64))
+            p64 = self._plan(("hc", b, FP64, side, id(ctx)), lambda: mk(FP64))
             p64.set_operator(*ops)
+        if side >= 0:
+            # the copies run on the side stream; HL's memory returns to torch's allocator, which
+            # only orders it with the main stream
+            self.torch.cuda.synchronize()
         return p, p64
+
+    def _side_contexts(self):
+        if self._side is None:
+            from concurrent.futures import ThreadPoolExecutor
+            streams = [self.torch.cuda.Stream() for _ in range(2)]  # default priority
+            self._side = ([Context(self.ctx.device, st) for st in streams], streams, ThreadPoolExecutor(max_workers=3))
+        return self._side
+
+    def _eigensolves(self, jobs, tol):
+        """Lowest eigenvectors, in place, for [(plan, x, plan_fp64), ...]; returns the Krylov step counts."""
+        run = lambda j: eigsh(j[0], j[1], tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=j[2], check=False)[1]  # noqa: E731
+        if not self.concurrent:
+            return [run(j) for j in jobs]
+        self.torch.cuda.synchronize()  # inputs made on the main stream
+        its = list(self._side_contexts()[2].map(run, jobs))
+        self.torch.cuda.synchronize()
+        return its
 
     def gradnorm(self, n, AC, hac):
         """GradNormRight: norm_scaled(ApplyTMmixedRight(H_AC(C AR), NR)) = ||HAC - (sum_s HAC_s AR_s^T) AR|| / sqrt(nsec),
@@ -398,16 +431,11 @@ class VUMPS:
             hac, hac64 = self._hac_plans(n)
             AC, _ = self._prod(MAT_MPS, self.blay[n], self.C[left], self.slay[n], self.AR[n], ("CAR", n))
             F[n], G[n] = self.gradnorm(n, AC, hac64 or hac)
-            _, it, _ = eigsh(hac, AC, tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=hac64, check=False)
-            self.eig_iters += it
-            hcl, hcl64 = self._hc_plans(n)
             CL = self.C[left].clone()
-            _, it, _ = eigsh(hcl, CL, tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=hcl64, check=False)
-            self.eig_iters += it
-            hcr, hcr64 = self._hc_plans(n + 1)
             CR = self.C[n].clone()
-            _, it, _ = eigsh(hcr, CR, tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=hcr64, check=False)
-            self.eig_iters += it
+            hcl, hcl64 = self._hc_plans(n, 0 if self.concurrent else -1)
+            hcr, hcr64 = self._hc_plans(n + 1, 1 if self.concurrent else -1)
+            self.eig_iters += sum(self._eigensolves([(hac, AC, hac64), (hcl, CL, hcl64), (hcr, CR, hcr64)], tol))
             CL = CL if float(CL[0].item()) >= 0 else -CL  # sign convention of VUMPS_main.cpp:201-202
             CR = CR if float(CR[0].item()) >= 0 else -CR
             if self.dosvd and self.prec_n[n] < self.svdthresh:
