@@ -212,52 +212,87 @@ class VUMPS:
             h = self._tm(RIGHT, self.AR[s], s, h, self._b(s + 1), s) + self._optm_right(b + k)
         return h
 
-    def _env_plan(self, direction, b, As_layouts_key):
-        """Environment solver of the cell from site b, FP64 at every working precision: the
-        geometric sum's operator 1 - T has the TM gap as its smallest scale, so a low-precision T
-        gains nothing on a single right-hand side (each step is latency-bound) and its rounding,
+    def _left_env_job(self, b, tol, side: int = -1):
+        """LE[b] = InvertE_proj(AL cell from b, GetHL, 1, R, l) with R the cell fixed point (trueLR).
+        The operands are prepared here on the main stream; the returned job runs the fixed point
+        and the geometric sum (on the main stream, or on side stream `side`) and returns (LE[b],
+        Krylov steps).  The environment plans are FP64 at every working precision: the geometric
+        sum's operator 1 - T has the TM gap as its smallest scale, so a low-precision T gains
+        nothing on a single right-hand side (each step is latency-bound) and its rounding,
         amplified by 1 / gap, forces extra refinement rounds."""
-        N = self.N
-        sites = [self._b(b + k) for k in range(N)]
-        return self._plan(("env", direction, b), lambda: EnvPlan(self.ctx, direction, [self.slay[s] for s in sites], self.blay[b], FP64))
-
-    def _left_env(self, b, tol):
-        """LE[b] = InvertE_proj(AL cell from b, GetHL, 1, R, l) with R the cell fixed point (trueLR)."""
         b = self._b(b)
         sites = [self._b(b + k) for k in range(self.N)]
-        p = self._env_plan(LEFT, b, None)
+        ctx = self.ctx if side < 0 else self._side_contexts()[0][side]
+        p = self._plan(("env", LEFT, b, id(ctx)), lambda: EnvPlan(ctx, LEFT, [self.slay[s] for s in sites], self.blay[b], FP64))
         p.set_mps([self.AL[s] for s in sites])
         Cb = self.C[self._b(b - 1)]
         R, _ = self._prod(MPS_MATT, self.blay[b], Cb, self.blay[b], Cb, ("CCt", b))  # C C^T
         # fixed point of T_R over the cell: the right-direction chain of the same AL tensors
-        pr = self._plan(("envR_AL", b), lambda: EnvPlan(self.ctx, RIGHT, [self.slay[s] for s in sites], self.blay[b], FP64))
+        pr = self._plan(("envR_AL", b, id(ctx)), lambda: EnvPlan(ctx, RIGHT, [self.slay[s] for s in sites], self.blay[b], FP64))
         pr.set_mps([self.AL[s] for s in sites])
-        pr.fixed_point(R, tol=1e-14, maxit=self.env_maxit, check=False)
         HAL = self._get_hl(b)
         one = self._identity(b)
         y = self._empty(self.blay[b].nelem)
         y0 = self.LE.get(b)
-        it, rr = p.solve(HAL, one, R, y, y0=y0, tol=tol, maxit=self.env_maxit, krylov=self.env_krylov, check=False)
-        self.env_iters += it
-        e_cell = float(self.torch.dot(HAL, self._transpose_blocks(R, b)).item())
-        return y, e_cell
 
-    def _right_env(self, b, tol):
+        def job():
+            pr.fixed_point(R, tol=1e-14, maxit=self.env_maxit, check=False)  # R becomes the fixed point
+            it, rr = p.solve(HAL, one, R, y, y0=y0, tol=tol, maxit=self.env_maxit, krylov=self.env_krylov, check=False)
+            return y, it
+        return job, (HAL, R, b)
+
+    def _right_env_job(self, b, tol, side: int = -1):
+        """RE[b] = InvertE_proj(AR cell from b, GetHR, L, 1, r), L = the cell fixed point; as
+        _left_env_job."""
         b = self._b(b)
         sites = [self._b(b + k) for k in range(self.N)]
-        p = self._env_plan(RIGHT, b, None)
+        ctx = self.ctx if side < 0 else self._side_contexts()[0][side]
+        p = self._plan(("env", RIGHT, b, id(ctx)), lambda: EnvPlan(ctx, RIGHT, [self.slay[s] for s in sites], self.blay[b], FP64))
         p.set_mps([self.AR[s] for s in sites])
         Cb = self.C[self._b(b - 1)]
         L, _ = self._prod(MATT_MPS, self.blay[b], Cb, self.blay[b], Cb, ("CtC", b))  # C^T C
-        pl = self._plan(("envL_AR", b), lambda: EnvPlan(self.ctx, LEFT, [self.slay[s] for s in sites], self.blay[b], FP64))
+        pl = self._plan(("envL_AR", b, id(ctx)), lambda: EnvPlan(ctx, LEFT, [self.slay[s] for s in sites], self.blay[b], FP64))
         pl.set_mps([self.AR[s] for s in sites])
-        pl.fixed_point(L, tol=1e-14, maxit=self.env_maxit, check=False)
         HAR = self._get_hr(b)
         one = self._identity(b)
         y = self._empty(self.blay[b].nelem)
-        it, rr = p.solve(HAR, L, one, y, y0=self.RE.get(b), tol=tol, maxit=self.env_maxit, krylov=self.env_krylov, check=False)
+        y0 = self.RE.get(b)
+
+        def job():
+            pl.fixed_point(L, tol=1e-14, maxit=self.env_maxit, check=False)
+            it, rr = p.solve(HAR, L, one, y, y0=y0, tol=tol, maxit=self.env_maxit, krylov=self.env_krylov, check=False)
+            return y, it
+        return job
+
+    def _left_env(self, b, tol):
+        job, (HAL, R, bb) = self._left_env_job(b, tol)
+        y, it = job()
+        self.env_iters += it
+        e_cell = float(self.torch.dot(HAL, self._transpose_blocks(R, bb)).item())
+        return y, e_cell
+
+    def _right_env(self, b, tol):
+        y, it = self._right_env_job(b, tol)()
         self.env_iters += it
         return y
+
+    def _envs(self, n, tol):
+        """(LE[n], e_cell, RE[n+2]): the left and the right geometric sums of a site step, which are
+        independent, at once on two streams when `concurrent`."""
+        if not self.concurrent:
+            LE, e = self._left_env(n, tol)
+            return LE, e, self._right_env(n + 2, tol)
+        lj, (HAL, R, bb) = self._left_env_job(n, tol)
+        rj = self._right_env_job(n + 2, tol, side=0)
+        self.torch.cuda.synchronize()  # operands made on the main stream
+        pool = self._side_contexts()[2]
+        fr = pool.submit(rj)
+        LE, itl = lj()
+        RE, itr = fr.result()
+        self.torch.cuda.synchronize()
+        self.env_iters += itl + itr
+        e_cell = float(self.torch.dot(HAL, self._transpose_blocks(R, bb)).item())
+        return LE, e_cell, RE
 
     def _transpose_blocks(self, x, b):
         """Blockwise transpose on the block-diagonal layout of bond b (one gather)."""
@@ -460,11 +495,11 @@ class VUMPS:
             self.C[left], self.C[n] = CL, CR
             # environments (VUMPS_main.cpp:247-305)
             etol = max(tol / self.env_fac, self.env_tol)
-            self.LE[n], e_cell = self._left_env(n, etol)
+            self.LE[n], e_cell, RE_next = self._envs(n, etol)
             self.LE[self._b(n + 1)] = self._tm(LEFT, self.AL[n], n, self.LE[n], n, self._b(n + 1)) + self._optm_left(n - 1)
             self.LE[self._b(n + 2)] = (self._tm(LEFT, self.AL[self._b(n + 1)], self._b(n + 1), self.LE[self._b(n + 1)], self._b(n + 1),
                                                 self._b(n + 2)) + self._optm_left(n))
-            self.RE[self._b(n + 2)] = self._right_env(n + 2, etol)
+            self.RE[self._b(n + 2)] = RE_next
             self.RE[self._b(n + 1)] = (self._tm(RIGHT, self.AR[self._b(n + 1)], self._b(n + 1), self.RE[self._b(n + 2)], self._b(n + 2),
                                                 self._b(n + 1)) + self._optm_right(n + 1))
         self._maybe_expand(exp_prec)
