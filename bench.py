@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="bound on the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vumps", action="store_true")
+    ap.add_argument("--no-xxz", action="store_true", help="skip the D = 64 Heisenberg VUMPS runs (keep the D = 1024 Hubbard one)")
     ap.add_argument("--no-exc", action="store_true")
     ap.add_argument("--no-tm", action="store_true")
     ap.add_argument("--exc-momenta", type=int, default=4, help="momenta of the excitation sample (of 32)")
@@ -203,7 +204,7 @@ def main():
         keys0 = np.array([[2 * (i - 4)] for i in range(9)], np.int32)
         sizes0 = np.array([1, 3, 7, 12, 18, 12, 7, 3, 1], np.int32)
         vres = {}
-        for name in [k for k in ("fp64", "fp32", "bf16x3") if k in results]:
+        for name in [k for k in ("fp64", "fp32", "bf16x3") if k in results and not args.no_xxz]:
             v = VUMPS(ctx, im.XXZ, 2, [0], keys0, sizes0, [1.0, -1.0, 0.0], prec=precs[name], seed=1001)
             t0 = time.perf_counter()
             sweeps, st = 0, None
