@@ -3,8 +3,9 @@
 //
 // Determinism: every reduction is two-stage with a fixed order.  Stage one: workgroup (c, i) reduces
 // row chunk c of basis vector i (thread strides, then a fixed wave64 shuffle tree and the four wave
-// sums in order) into a partial; stage two sums the chunk partials of each vector in chunk order.
-// Results are bitwise reproducible for a given problem size.
+// sums in order) into a partial; stage two sums the chunk partials of each vector, either in chunk
+// order (multidot_final) or, inside the kernel that consumes them, by a fixed lane-stride and
+// shuffle-tree order (sum_partials_lds).  Results are bitwise reproducible for a given problem size.
 #include <hip/hip_runtime.h>
 
 #include "blas.hpp"
@@ -62,12 +63,21 @@ __global__ __launch_bounds__(kBlasThreads) void gemv_sub(const double* __restric
     }
 }
 
-// Summed chunk partials of inner product i (chunk order); every consumer sums them the same way.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int nch, int nb, int i) {
-    double t = 0;
-#pragma unroll 4
-    for (int c = 0; c < nch; ++c) t += p[(long)c * nb + i];
-    return t;
+// hs[i] = sum over c < nch of p[c*nb + i] for i < nb, in a fixed order (lane l sums c = l, l + 64,
+// ... in turn, then the wave64 shuffle tree; wave w takes i = w, w + 4, ...), identical in every
+// workgroup; workgroup 0 also stores the sums at hout (when given).  Ends with a barrier.
+__device__ __forceinline__ void sum_partials_lds(const double* __restrict__ p, int nch, int nb, double* hs, double* __restrict__ hout) {
+    const int lane = threadIdx.x & 63;
+    for (int i = threadIdx.x >> 6; i < nb; i += kBlasThreads / 64) {
+        double t = 0;
+        for (int c = lane; c < nch; c += 64) t += p[(long)c * nb + i];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+        if (lane == 0) {
+            hs[i] = t;
+            if (hout && blockIdx.x == 0) hout[i] = t;
+        }
+    }
+    __syncthreads();
 }
 
 // w[r] -= sum_i V[i*ldv + r] * h[i], h[i] summed from the multidot chunk partials p by every
@@ -77,12 +87,7 @@ __global__ __launch_bounds__(kBlasThreads) void gemv_sub_partials(const double* 
                                                                   int nch, double* __restrict__ hout, double* __restrict__ w, long n,
                                                                   double* __restrict__ qpart) {
     extern __shared__ double hs[];
-    for (int i = threadIdx.x; i < nb; i += kBlasThreads) {
-        const double t = sum_partials(p, nch, nb, i);
-        hs[i] = t;
-        if (blockIdx.x == 0) hout[i] = t;
-    }
-    __syncthreads();
+    sum_partials_lds(p, nch, nb, hs, hout);
     double q = 0;
     for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) {
         double s = 0;
@@ -127,7 +132,9 @@ __global__ __launch_bounds__(kBlasThreads) void scale_inv_norm(const double* __r
 __global__ __launch_bounds__(kBlasThreads) void diff_plus_scaled(const double* __restrict__ x, const double* __restrict__ y,
                                                                  const double* __restrict__ z, const double* __restrict__ p, int nch,
                                                                  double* __restrict__ out, long n) {
-    const double t = sum_partials(p, nch, 1, 0);
+    __shared__ double t1;
+    sum_partials_lds(p, nch, 1, &t1, nullptr);
+    const double t = t1;
     for (long r = blockIdx.x * (long)blockDim.x + threadIdx.x; r < n; r += (long)gridDim.x * blockDim.x) out[r] = (x[r] - y[r]) + t * z[r];
 }
 
