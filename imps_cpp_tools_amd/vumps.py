@@ -359,6 +359,18 @@ class VUMPS:
             self._side = ([Context(self.ctx.device, st) for st in streams], streams, ThreadPoolExecutor(max_workers=3))
         return self._side
 
+    def close(self):
+        """Release the side streams' thread pool (also on garbage collection)."""
+        if getattr(self, "_side", None) is not None:
+            self._side[2].shutdown(wait=True)
+            self._side = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def _eigensolves(self, jobs, tol):
         """Lowest eigenvectors, in place, for [(plan, x, plan_fp64), ...]; returns the Krylov step counts."""
         run = lambda j: eigsh(j[0], j[1], tol=tol, maxit=self.eig_maxit, krylov=self.krylov, plan_fp64=j[2], check=False)[1]  # noqa: E731
