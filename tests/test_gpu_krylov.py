@@ -99,6 +99,27 @@ def test_env_solve_parity(ctx, prec, name, direction):
     assert relerr(y.cpu().numpy(), yref.data) < 1e-9, relerr(y.cpu().numpy(), yref.data)
 
 
+@pytest.mark.parametrize("name", ["xxz", "fhub"])
+@pytest.mark.parametrize("direction", [0, 1])
+def test_env_solve_long_krylov_space(ctx, name, direction):
+    """GMRES with a Krylov space of up to 300 vectors (the size the VUMPS driver uses): the inner
+    products over hundreds of basis vectors and their in-kernel partial sums, against the oracle."""
+    c = cell(name)
+    rng = np.random.default_rng(37)
+    As = c.AL if direction == 0 else c.AR
+    L, R = (identity(c), fixed_point(c, As, 1)) if direction == 0 else (fixed_point(c, As, 0), identity(c))
+    x = random_like(c.diag(0), rng, sym=True)
+    yref, flag = orc.invert_e_proj(direction, As, x, L, R, tol=1e-13)
+    assert flag > 0
+    p = im.EnvPlan(ctx, direction, [lay(a) for a in As], lay(x), im.FP64)
+    p.set_mps([dev(a.data) for a in As])
+    y = torch.zeros(x.nelem, dtype=torch.float64, device="cuda")
+    it, rr = p.solve(dev(x.data), dev(L.data), dev(R.data), y, tol=1e-14, krylov=300)
+    print(name, direction, "n", x.nelem, "steps", it, flush=True)
+    assert rr < 1e-13
+    assert relerr(y.cpu().numpy(), yref.data) < 1e-9, relerr(y.cpu().numpy(), yref.data)
+
+
 def dense_operator(apply, n):
     M = np.zeros((n, n))
     for i in range(n):

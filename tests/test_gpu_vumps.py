@@ -179,3 +179,31 @@ def test_concurrent_eigensolves_match_sequential():
         hist[conc] = [(st.energy, tuple(st.F), st.eig_iters) for st in (v.sweep() for _ in range(6))]
         hist[conc].append(tuple(float(c.abs().sum().item()) for c in v.C))
     assert hist[True] == hist[False]
+
+
+def _lieb_wu(U):
+    """Ground-state energy per site of the half-filled Hubbard chain (Lieb-Wu):
+    -4 int_0^inf J0(w) J1(w) / (w (1 + exp(w U / 2))) dw, for H = -t sum c^+c + U sum n_up n_dn, t = 1."""
+    from scipy import integrate, special
+    f = lambda w: special.j0(w) * special.j1(w) / (w * (1.0 + np.exp(min(w * U / 2.0, 700.0))))  # noqa: E731
+    edges = np.linspace(0.0, 400.0, 4001)
+    return -4.0 * sum(integrate.quad(f, a, b)[0] for a, b in zip(edges[:-1], edges[1:]))
+
+
+def test_hubbard_d1024_grows_to_lieb_wu():
+    """Fermi-Hubbard chain, t = 1, U = 4, half filling (QN = [2, 0] on the two-site cell): VUMPS from
+    D = 64 grows both bonds to D = 1024 by ExpandFromH; the energy per site approaches the Lieb-Wu
+    value from above.  The model's interaction U (n_up - 1/2)(n_dn - 1/2) shifts it by -U/4 at half
+    filling (Models.cpp:543-572)."""
+    from imps_cpp_tools_amd import workloads as wl
+    from imps_cpp_tools_amd.vumps import VUMPS
+    cfg = wl.HUBBARD_1024
+    k0, s0 = wl.fhub_profile(5, 8, 1, 64)
+    v = VUMPS(im.Context(0), cfg["model"], cfg["N"], cfg["QN"], k0, s0, cfg["params"], seed=1003, krylov=30, env_krylov=300,
+              mmax=1024, dm_max=256, expand_pause=1, expand_thresh=1.0)
+    for _ in range(14):
+        st = v.sweep()
+    exact = _lieb_wu(cfg["params"][1]) - cfg["params"][1] / 4.0
+    print("dims", v.bond_dims(), "E %.12f exact %.12f diff %.2e F %.1e" % (st.energy, exact, st.energy - exact, max(st.F)), flush=True)
+    assert v.bond_dims() == [1024, 1024]
+    assert exact - 1e-9 <= st.energy <= exact + 1e-6
