@@ -16,8 +16,11 @@
  *    (VUMPS_helpers.hpp:69,118; EigsBlockFunctions.hpp:55,73).
  *  - Every function returns an imps_status; no C++ exception crosses this boundary.  On error,
  *    imps_last_error() returns a thread-local message.
- *  - A plan is not thread-safe.  Work is ordered on the context's HIP stream.  The library never
- *    retains caller pointers beyond a call, except the operator data copied by *_set_* calls.
+ *  - A plan is not thread-safe.  Work is ordered on the context's HIP stream.  Different plans may
+ *    be driven from different host threads at once (Krylov scratch and error messages are per
+ *    thread); give each thread's plans their own context and stream to overlap the solves.  The
+ *    library never retains caller pointers beyond a call; *_set_* calls copy the operator data
+ *    on the plan's stream, so their sources must stay valid until that stream reaches the copy.
  *  - Precision (imps_prec) selects the arithmetic of the contractions.  Inputs and outputs stay FP64.
  */
 #ifndef IMPS_CAPI_H
