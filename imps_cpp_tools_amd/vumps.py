@@ -572,9 +572,11 @@ class VUMPS:
             return
         for b in range(self.N):
             total = int(np.sum(self.bonds[b][1]))
-            lam = schmidt_values(self.blay[b], self.C[self._b(b - 1)].cpu().numpy())
-            minlam = max((float(v.min()) for v in lam.values() if v.size), default=0.0)
-            if self.expand_ct[b] >= self.expand_pause and exp_prec < self.expand_thresh and total < self.mmax and minlam > self.lamthresh:
+            grow = self.expand_ct[b] >= self.expand_pause and exp_prec < self.expand_thresh and total < self.mmax
+            if grow:  # the Schmidt values (a host SVD per sector) only when the cheap conditions hold
+                lam = schmidt_values(self.blay[b], self.C[self._b(b - 1)].cpu().numpy())
+                grow = max((float(v.min()) for v in lam.values() if v.size), default=0.0) > self.lamthresh
+            if grow:
                 self.expand_ct[b] = 0
                 self.expand(b, min(self.dm_max, self.mmax - total))
             else:
