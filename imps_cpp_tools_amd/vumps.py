@@ -71,12 +71,12 @@ class VUMPS:
         """`plans`: the plan cache of another VUMPS over the same model and bond structure, reused
         (plans hold structure only; every use sets its operator data).
 
-        Bond growth (VUMPS_main.cpp:273-306): with `mmax` set, after each outer iteration every bond
-        whose total dimension is below mmax grows by ExpandFromH with at most min(dm_max, mmax -
-        total) states, when at least `expand_pause` iterations passed since its last growth, the
-        gauge error at the start of the iteration is below `expand_thresh` and some sector's
-        smallest Schmidt value exceeds `lamthresh` (reference defaults: dmmax 20, pause 5,
-        exthresh 1e-5, lamthresh 1e-12).
+        Bond growth (VUMPS_main.cpp:273-306): with `mmax` set (one value, or one per bond), after
+        each outer iteration every bond whose total dimension is below mmax grows by ExpandFromH
+        with at most min(dm_max, mmax - total) states, when at least `expand_pause` iterations
+        passed since its last growth, the gauge error at the start of the iteration is below
+        `expand_thresh` and some sector's smallest Schmidt value exceeds `lamthresh` (reference
+        defaults: dmmax 20, pause 5, exthresh 1e-5, lamthresh 1e-12).
 
         `concurrent`: the three eigensolves of a site step (H_AC, H_C on both bonds) read the same
         environments and are independent; they run at once, each on its own stream from its own
@@ -584,13 +584,14 @@ class VUMPS:
             return
         for b in range(self.N):
             total = int(np.sum(self.bonds[b][1]))
-            grow = self.expand_ct[b] >= self.expand_pause and exp_prec < self.expand_thresh and total < self.mmax
+            mmax = int(self.mmax[b]) if isinstance(self.mmax, (list, tuple, np.ndarray)) else int(self.mmax)
+            grow = self.expand_ct[b] >= self.expand_pause and exp_prec < self.expand_thresh and total < mmax
             if grow:  # the Schmidt values (a host SVD per sector) only when the cheap conditions hold
                 lam = schmidt_values(self.blay[b], self.C[self._b(b - 1)].cpu().numpy())
                 grow = max((float(v.min()) for v in lam.values() if v.size), default=0.0) > self.lamthresh
             if grow:
                 self.expand_ct[b] = 0
-                self.expand(b, min(self.dm_max, self.mmax - total))
+                self.expand(b, min(self.dm_max, mmax - total))
             else:
                 self.expand_ct[b] += 1
 

@@ -207,3 +207,22 @@ def test_hubbard_d1024_grows_to_lieb_wu():
     print("dims", v.bond_dims(), "E %.12f exact %.12f diff %.2e F %.1e" % (st.energy, exact, st.energy - exact, max(st.F)), flush=True)
     assert v.bond_dims() == [1024, 1024]
     assert exact - 1e-9 <= st.energy <= exact + 1e-6
+
+
+def test_vumps_from_config_file(tmp_path):
+    """The configuration-file front end runs VUMPS_main's loop: Heisenberg chain from m0 = [1,2,1],
+    growth to mmax = 24, stop at thresh, save; the saved state reloads with the same energy."""
+    from imps_cpp_tools_amd import vumps_main as vm
+    cfg_file = tmp_path / "xxz.cfg"
+    cfg_file.write_text("--model=XXZ\n--Delta=-1\n--N=2\n--QN=[0]\n--m0=[1,2,1]\n--mmax=[24]\n--dmmax=8\n"
+                        "--expand_pause=2\n--exthresh=1e-2\n--thresh=1e-7\n--maxit=80\n--save=1\n")
+    cfg = vm.settings(vm.parse([str(cfg_file), f"--folder={tmp_path / 'data'}"]))
+    v, hist = vm.run(cfg, log=lambda *a: None)
+    print("iterations", len(hist), "D", v.bond_dims(), "E %.12f F %.1e" % (hist[-1].energy, max(hist[-1].F)), flush=True)
+    assert v.bond_dims() == [24, 24]
+    assert GOLD["bethe_ansatz"] <= hist[-1].energy <= GOLD["bethe_ansatz"] + 1e-3
+    files = list((tmp_path / "data").glob("UMPS_XXZ_*_N2_chi24.bin"))
+    assert len(files) == 1
+    w = _small_xxz()
+    w.load(str(files[0]))
+    assert abs(w.energy() - v.energy()) <= 1e-12
