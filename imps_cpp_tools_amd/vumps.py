@@ -595,6 +595,31 @@ class VUMPS:
             else:
                 self.expand_ct[b] += 1
 
+    # ------------------------------------------------------------------ observables
+    # diagonal one-site operators of the models (Models.cpp:57-75 XXZ, :546-560 Fermi-Hubbard;
+    # local states of the Hubbard chain: empty, down, up, double)
+    DIAG_OPS = {0: {"sz": [0.5, -0.5]},
+                1: {"nup": [0.0, 0.0, 1.0, 1.0], "ndo": [0.0, 1.0, 0.0, 1.0], "sz": [0.0, -0.5, 0.5, 0.0],
+                    "n": [0.0, 1.0, 1.0, 2.0]}}
+
+    def observables(self, names) -> Dict[str, np.ndarray]:
+        """MeasureObservables (MPSBlockUtilities.hpp:1017-1073) for one-site operators that conserve
+        the charges (diagonal in the local basis): <O>_n = sum_s O_ss ||AC[n]_s||^2 / ||AC[n]||^2 with
+        AC[n] = C[n-1] AR[n], for every site n of the cell.  Charge-changing operators have zero
+        expectation value in these symmetric states and are not offered."""
+        ops = self.DIAG_OPS[self.model]
+        out = {}
+        for name in names:
+            if name not in ops:
+                raise ValueError(f"operator {name} not defined (charge-conserving one-site operators: {', '.join(ops)})")
+            vals = np.zeros(self.N)
+            for n in range(self.N):
+                AC, _ = self._prod(MAT_MPS, self.blay[n], self.C[self._b(n - 1)], self.slay[n], self.AR[n], ("obsAC", n))
+                w = np.array([float(AC[a:b].square().sum().item()) for a, b in (self.slay[n].entry_range(e) for e in range(self.d))])
+                vals[n] = float(np.dot(ops[name], w) / w.sum())
+            out[name] = vals
+        return out
+
     # ------------------------------------------------------------------ state files
     def save(self, path: str):
         """saveIMPS (MPSBlockUtilities.hpp:1212-1266) of the unit cell: Schmidt values, C, AL, AR in

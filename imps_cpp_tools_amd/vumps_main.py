@@ -9,7 +9,8 @@ settings replacing earlier ones.  Vectors are written `[a,b,c]`.
 
 Keys (VUMPS_main.cpp:51-69, Models.cpp:940-1025): model (XXZ | FHUB), N (>= 2), QN, m0 (initial
 dims per sector, GetDimMap), mmax (one value or one per bond), thresh (stop when max F is below),
-dmmax, exthresh, lamthresh, expand_pause, InvETol, InvEfac, save, folder; XXZ: J (1), Delta, hz (0);
+dmmax, exthresh, lamthresh, expand_pause, InvETol, InvEfac, save, folder, obs (charge-conserving
+one-site operators: XXZ sz; FHUB n, nup, ndo, sz); XXZ: J (1), Delta, hz (0);
 FHUB: t (1), U, V (0), mu (0), hz (0).  Extra keys of this runner: maxit (outer iterations, 1000),
 prec (fp64 | fp32 | bf16x3), seed.
 """
@@ -18,8 +19,6 @@ from __future__ import annotations
 import os
 import sys
 from typing import Dict, Iterable, List, Optional
-
-import numpy as np
 
 MODELS = {"XXZ": 0, "FHUB": 1}
 
@@ -94,7 +93,8 @@ def settings(values: Dict[str, str]) -> Dict:
         expand_pause=_get(values, "expand_pause", int, 5), InvETol=_get(values, "InvETol", float, 5e-15),
         InvEfac=_get(values, "InvEfac", float, 2.0), save=_get(values, "save", _bool, False),
         folder=_get(values, "folder", str, "data"), maxit=_get(values, "maxit", int, 1000),
-        prec=_get(values, "prec", str, "fp64"), seed=_get(values, "seed", int, 0))
+        prec=_get(values, "prec", str, "fp64"), seed=_get(values, "seed", int, 0),
+        obs=_get(values, "obs", lambda s: _vector(s, str.strip), []))
 
 
 def run(cfg: Dict, ctx=None, log=print):
@@ -116,6 +116,9 @@ def run(cfg: Dict, ctx=None, log=print):
         log(f"{it + 1}: E = {st.energy:.14f}  max F = {max(st.F):.3e}  D = {v.bond_dims()}  {st.seconds:.3f} s")
         if max(st.F) < cfg["thresh"]:
             break
+    if cfg["obs"]:
+        for name, vals in v.observables(cfg["obs"]).items():
+            log(f"{name}: {' '.join(f'{x:.12f}' for x in vals)}  mean: {vals.mean():.12f}")
     if cfg["save"]:
         os.makedirs(cfg["folder"], exist_ok=True)
         name = "_".join([f"UMPS_{cfg['model']}"] + [f"{k}{p:g}" for k, p in zip(("J", "Delta", "hz") if cfg["model"] == "XXZ"

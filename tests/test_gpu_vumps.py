@@ -207,6 +207,16 @@ def test_hubbard_d1024_grows_to_lieb_wu():
     print("dims", v.bond_dims(), "E %.12f exact %.12f diff %.2e F %.1e" % (st.energy, exact, st.energy - exact, max(st.F)), flush=True)
     assert v.bond_dims() == [1024, 1024]
     assert exact - 1e-9 <= st.energy <= exact + 1e-6
+    # the cell's charges fix the densities: 2 particles and Sz = 0 on the two sites.  Each site's
+    # value comes from its own AC = C AR, whose left environment C C^T is the fixed point only up to
+    # the gauge error max ||AC - AL C||, ||AC - C AR|| of the iteration (prec), so the sums hold to
+    # that order
+    obs = v.observables(["n", "sz", "nup", "ndo"])
+    tol = max(1e-10, 10.0 * float(np.max(v.prec_n)))
+    print("densities", obs, "gauge error", float(np.max(v.prec_n)), flush=True)
+    assert abs(obs["n"].sum() - 2.0) <= tol and abs(obs["sz"].sum()) <= tol
+    np.testing.assert_allclose(obs["nup"] + obs["ndo"], obs["n"], atol=1e-12)
+    np.testing.assert_allclose(0.5 * (obs["nup"] - obs["ndo"]), obs["sz"], atol=1e-12)
 
 
 def test_vumps_from_config_file(tmp_path):
@@ -221,6 +231,10 @@ def test_vumps_from_config_file(tmp_path):
     print("iterations", len(hist), "D", v.bond_dims(), "E %.12f F %.1e" % (hist[-1].energy, max(hist[-1].F)), flush=True)
     assert v.bond_dims() == [24, 24]
     assert GOLD["bethe_ansatz"] <= hist[-1].energy <= GOLD["bethe_ansatz"] + 1e-3
+    sz = v.observables(["sz"])["sz"]
+    assert abs(sz.sum()) <= max(1e-10, 10.0 * float(np.max(v.prec_n)))  # QN = [0]: no total Sz (to the gauge error)
+    with pytest.raises(ValueError):
+        v.observables(["sx"])
     files = list((tmp_path / "data").glob("UMPS_XXZ_*_N2_chi24.bin"))
     assert len(files) == 1
     w = _small_xxz()

@@ -53,6 +53,7 @@ def test_settings_defaults_and_vectors():
     assert s["save"] is False
     x = vm.settings(vm.parse_lines(["--model=XXZ", "--Delta=-1", "--N=2", "--QN=[0]", "--m0=[2,4,2]", "--mmax=[16,32]", "--hz=0.5"]))
     assert x["params"] == [1.0, -1.0, 0.5] and x["mmax"] == [16, 32]
+    assert vm.settings(vm.parse_lines(CFG.splitlines() + ["--obs=[n,sz]"]))["obs"] == ["n", "sz"]
 
 
 @pytest.mark.parametrize("bad", [["--model=QIM"], ["--model=XXZ", "--N=2", "--QN=[0]", "--m0=[1]", "--mmax=[8]"],
