@@ -2,6 +2,7 @@
 #include "krylov.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <deque>
 #include <memory>
 #include <cmath>
@@ -150,6 +151,62 @@ void cgs2_dev(hipStream_t s, Work& W, int k, double* w, double* dc) {
     }
     hip_check(blas_scale_inv_norm(qp, blas_gemv_grid(W.n), dc + 2 * k, w, W.n, s), "normalise");
 }
+
+// A Krylov cycle's launch sequence as a HIP graph: captured on the first run() with a given key,
+// replayed afterwards.  Graphs need a capturable stream (not the legacy default stream, not one
+// already being captured); run() returns false when the sequence was not run (the caller then
+// issues it directly).  Capture is thread-local, so other host threads may keep allocating.
+// Opt-in (IMPS_KRYLOV_GRAPH=1): on the D = 1024 Hubbard VUMPS iteration, capturing per solve and
+// replaying the restart cycles measured no faster than issuing the launches directly.
+class CycleGraph {
+  public:
+    explicit CycleGraph(hipStream_t s) : s_(s) {
+        const char* e = std::getenv("IMPS_KRYLOV_GRAPH");
+        enabled_ = e && e[0] == '1' && s != nullptr;
+        if (enabled_) {
+            hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+            enabled_ = hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusNone;
+        }
+    }
+    ~CycleGraph() { reset(); }
+    template <class F>
+    bool run(int key, F&& issue) {
+        if (!enabled_) return false;
+        if (!exec_ || key != key_) {
+            reset();
+            hipGraph_t g = nullptr;
+            hip_check(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                issue();
+            } catch (...) {
+                (void)hipStreamEndCapture(s_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(s_, &g), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                exec_ = nullptr;
+                enabled_ = false;
+                return false;
+            }
+            key_ = key;
+        }
+        hip_check(hipGraphLaunch(exec_, s_), "hipGraphLaunch");
+        return true;
+    }
+
+  private:
+    void reset() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        exec_ = nullptr;
+    }
+    hipStream_t s_;
+    bool enabled_ = false;
+    hipGraphExec_t exec_ = nullptr;
+    int key_ = -1;
+};
 
 // Host side of cgs2_dev: h = the k projection coefficients (both passes summed) from the raw
 // values t; returns ||w|| before the scaling.
@@ -348,14 +405,28 @@ KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol,
     hip_check(blas_axpby(1.0 / nx, nullptr, AXPBY_DA, x, 0.0, W.Vi(0), n, s), "scale");
     std::vector<double> T((size_t)m * m, 0.0), h, w, Z, raw;
     int j0 = 0;  // basis vectors already present (and their T block) at the start of a cycle
+    // From the first restart on, every cycle issues the same launches on the same buffers (steps
+    // j0 = nk .. m-1): it is captured once into a graph and replayed.
+    CycleGraph graph(s);
     while (true) {
         int jend = j0;  // every cycle takes at least one step (j0 <= m - 2)
-        for (int j = j0; j < m; ++j) {
-            A.apply(W.Vi(j), W.Vi(j + 1));
-            ++st.iters;
-            cgs2_dev(s, W, j + 1, W.Vi(j + 1), W.Ci(j));
-            jend = j + 1;
-            if (st.iters >= maxit) break;
+        auto steps = [&](int upto) {
+            for (int j = j0; j < upto; ++j) {
+                A.apply(W.Vi(j), W.Vi(j + 1));
+                cgs2_dev(s, W, j + 1, W.Vi(j + 1), W.Ci(j));
+            }
+        };
+        if (j0 > 0 && maxit - st.iters >= m - j0 && graph.run(j0, [&] { steps(m); })) {
+            st.iters += m - j0;
+            jend = m;
+        } else {
+            for (int j = j0; j < m; ++j) {
+                A.apply(W.Vi(j), W.Vi(j + 1));
+                ++st.iters;
+                cgs2_dev(s, W, j + 1, W.Vi(j + 1), W.Ci(j));
+                jend = j + 1;
+                if (st.iters >= maxit) break;
+            }
         }
         raw.resize((size_t)(jend - j0) * W.cstride);
         d2h(s, raw.data(), W.Ci(j0), raw.size());

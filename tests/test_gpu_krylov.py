@@ -227,3 +227,47 @@ def test_gradnorm_matches_oracle():
         HR = orc.reduced_op(o.H, o.d, o.AR[(n + 1) % g["N"]], 1)
         F_ref = orc.gradnorm(1, o.AR[n], o.C[left], o.LE[n], o.RE[(n + 1) % g["N"]], HL, HR)
         assert abs(F_gpu - F_ref) <= 1e-9 * F_ref, (n, F_gpu, F_ref)
+
+
+@pytest.mark.parametrize("prec", [im.FP64, im.BF16X3], ids=["fp64", "bf16x3"])
+def test_eigsh_graph_cycles_match_eager(prec, monkeypatch):
+    """Lanczos restart cycles replayed from a captured HIP graph (a context on its own stream)
+    give bitwise the results of issuing the launches directly (IMPS_KRYLOV_GRAPH=0), and the dense
+    eigenvalue; a small Krylov space forces many restarts, hence many replays."""
+    c = cell("fhub")
+    H = orc.model_ham(c.model, [1.0, 4.0, 0.0, 0.0, 0.0])
+    rng = np.random.default_rng(33)
+    HL = orc.reduced_op(H, c.d, c.AL[1], 0)
+    HR = orc.reduced_op(H, c.d, c.AR[1], 1)
+    EHL = random_like(c.diag(0), rng, sym=True)
+    EHR = random_like(c.diag(1), rng, sym=True)
+    x0 = random_like(c.struct[0], rng)
+
+    def ap(v):
+        t = random_like(c.struct[0], rng)
+        t.data = v
+        return orc.hac(EHL, EHR, HL, HR, t).data
+
+    e_dense = np.linalg.eigvalsh(dense_operator(ap, x0.nelem))[0]
+    sctx = im.Context(0, torch.cuda.Stream())
+    args = (lay(x0), lay(EHL), lay(EHR), lay(HL), lay(HR))
+    ops = (dev(EHL.data), dev(EHR.data), dev(HL.data), dev(HR.data))
+    p = im.HACPlan(sctx, *args, prec)
+    p.set_operator(*ops)
+    p64 = None
+    if prec != im.FP64:
+        p64 = im.HACPlan(sctx, *args, im.FP64)
+        p64.set_operator(*ops)
+    got = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("IMPS_KRYLOV_GRAPH", flag)
+        x = dev(x0.data)
+        torch.cuda.synchronize()
+        e, it, res = im.eigsh(p, x, tol=1e-12, krylov=12, plan_fp64=p64)
+        torch.cuda.synchronize()
+        got[flag] = (e, it, x.cpu().numpy())
+    print("iterations", got["1"][1], flush=True)
+    assert got["1"][1] == got["0"][1] and got["1"][1] > 3 * 12
+    assert got["1"][0] == got["0"][0]
+    np.testing.assert_array_equal(got["1"][2], got["0"][2])
+    assert abs(got["1"][0] - e_dense) < 1e-11 * abs(e_dense)
