@@ -44,6 +44,15 @@ def committed_traffic(kernel: str, nvec: int):
     return None
 
 
+def settle():
+    """Collect what earlier runs left: left to the cyclic garbage collector, their plans' device
+    frees (each a device synchronisation) would land inside a later timed region."""
+    import gc
+    import torch
+    gc.collect()
+    torch.cuda.synchronize()
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -201,7 +210,7 @@ def main():
         # VUMPS on torch's default stream: its iterations measured ~20% faster there than with the
         # main context on a separate stream (the side streams of the concurrent solves are separate
         # either way)
-        torch.cuda.synchronize()
+        settle()
         torch.cuda.set_stream(torch.cuda.default_stream())
         vctx = im.Context(0)
         prev_tune = os.environ.get("IMPS_AUTOTUNE")
@@ -222,12 +231,15 @@ def main():
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             vres[name] = dict(sweeps=sweeps, s_per_sweep=dt / sweeps, seconds_to_F=dt, F=max(st.F), energy=st.energy)
+            v.close()
+            del v
+            settle()
             log(f"vumps xxz D=64 {name}: {sweeps} sweeps, {dt / sweeps * 1e3:.1f} ms/sweep, F {max(st.F):.2e}, E {st.energy:.15f}")
         out["vumps_xxz_d64"] = vres
         # VUMPS outer-iteration time on the D = 1024 Hubbard chain, as the reference reaches it: from a
         # D = 64 bond (25 sectors, seeded randMPS_LR state) the bonds grow by ExpandFromH (at most 256
         # states per bond and iteration) to D = 1024; after one more iteration on the final structure
-        # (plans built), the mean wall time of the next three iterations is the iteration time.
+        # (plans built), the median wall time of the next five iterations is the iteration time.
         k64, s64 = wl.fhub_profile(5, 8, 1, 64)
         hres = {}
         for name in [k for k in ("fp64", "fp32", "bf16x3") if k in results]:
@@ -238,18 +250,24 @@ def main():
                 v.sweep()
                 grow += 1
             v.sweep()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            sts = [v.sweep() for _ in range(3)]
-            torch.cuda.synchronize()
-            dt = (time.perf_counter() - t0) / 3
+            settle()
+            sts, its = [], []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                sts.append(v.sweep())
+                torch.cuda.synchronize()
+                its.append(time.perf_counter() - t0)
+            dt = float(np.median(its))
             st = sts[-1]
-            hres[name] = dict(s_per_iter=dt, growth_iters=grow, bond_dims=v.bond_dims(), sectors=[len(b[1]) for b in v.bonds],
-                              energy=st.energy, F=max(st.F), eig_iters_per_iter=sum(x.eig_iters for x in sts) / 3,
-                              env_iters_per_iter=sum(x.env_iters for x in sts) / 3)
+            hres[name] = dict(s_per_iter=dt, iter_s=its, growth_iters=grow, bond_dims=v.bond_dims(), sectors=[len(b[1]) for b in v.bonds],
+                              energy=st.energy, F=max(st.F), eig_iters_per_iter=sum(x.eig_iters for x in sts) / len(sts),
+                              env_iters_per_iter=sum(x.env_iters for x in sts) / len(sts))
             log(f"vumps hubbard D=1024 {name}: grown in {grow} iterations to {v.bond_dims()} ({hres[name]['sectors']} sectors), "
-                f"{dt:.3f} s/iteration ({hres[name]['eig_iters_per_iter']:.0f} Lanczos, {hres[name]['env_iters_per_iter']:.0f} GMRES "
+                f"{dt:.3f} s/iteration (median of {' '.join(f'{t:.3f}' for t in its)}; {hres[name]['eig_iters_per_iter']:.0f} Lanczos, {hres[name]['env_iters_per_iter']:.0f} GMRES "
                 f"steps), E {st.energy:.12f}, F {max(st.F):.2e}")
+            v.close()
+            del v
+            settle()
         out["vumps_hubbard_d1024"] = hres
         torch.cuda.synchronize()
         torch.cuda.set_stream(stream)
