@@ -37,20 +37,46 @@ from . import (BF16X3, FP32, FP64, LEFT, RIGHT, MAT_MPS, MATT_MPS, MPS_MAT, MPS_
                make_dims, model_d, model_ham, model_nsym, mps_structure)
 
 
-# The two side streams of the concurrent site steps, one pair per device for the whole process.
-# HIP spreads streams over a few hardware queues in creation order; streams taken afresh from torch's
-# round-robin pool for every VUMPS object land on different queues from one object to the next, and
-# where one shares a queue with the main stream the concurrent solves serialise (measured: up to 40%
-# longer D = 1024 iterations for the second or third object in a process).  Reusing one pair gives
-# every object the first object's placement.
-_SIDE_STREAMS: Dict[int, list] = {}
+# The two side streams of the concurrent site steps, one pair per device and main stream for the
+# whole process.  HIP multiplexes streams over a few hardware queues, and work of streams that share
+# a queue runs in order: a side stream on the main stream's queue (or both on one queue) serialises
+# the concurrent solves (measured: D = 1024 iterations 0.15 s instead of 0.11 s).  Which queue a
+# stream of torch's pool lands on is not exposed, so the pair is chosen by measurement: a bounded
+# spin kernel on the main stream and on each candidate together must take about as long as one alone.
+_SIDE_STREAMS: Dict[tuple, list] = {}
+
+
+def _span(torch, streams, cycles) -> float:
+    """Wall time of one spin kernel of `cycles` clock cycles on each stream, issued together."""
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for st in streams:
+        with torch.cuda.stream(st):
+            torch.cuda._sleep(cycles)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t
 
 
 def _side_streams(torch, device: int) -> list:
-    if device not in _SIDE_STREAMS:
+    main = torch.cuda.current_stream(device)
+    key = (device, main.cuda_stream)
+    if key not in _SIDE_STREAMS:
         with torch.cuda.device(device):
-            _SIDE_STREAMS[device] = [torch.cuda.Stream() for _ in range(2)]  # default priority
-    return _SIDE_STREAMS[device]
+            cycles = 1 << 18
+            one = _span(torch, [main], cycles)  # also loads the kernel
+            while one < 2e-3 and cycles < 1 << 30:
+                cycles *= 4
+                one = _span(torch, [main], cycles)
+            picked, drawn = [], []
+            for _ in range(16):
+                st = torch.cuda.Stream()  # default priority
+                drawn.append(st)
+                if _span(torch, [main] + picked + [st], cycles) < 1.5 * one:
+                    picked.append(st)
+                    if len(picked) == 2:
+                        break
+            _SIDE_STREAMS[key] = (picked + [st for st in drawn if st not in picked])[:2]
+    return _SIDE_STREAMS[key]
 
 
 def lower_sqrt_factor(r: np.ndarray, lay: BlockLayout) -> np.ndarray:
