@@ -147,17 +147,31 @@ class BlockLayout:
     def nlist(self) -> int:
         return len(self.nblk)
 
+    def _sized(self):
+        """(block sizes, element count, block offsets), computed once per rows / cols arrays
+        (read-only arrays: layouts are shared)."""
+        src = (id(self.rows), id(self.cols))
+        c = self.__dict__.get("_size_cache")
+        if c is None or c[0] != src:
+            sz = self.rows.astype(np.int64) * self.cols.astype(np.int64)
+            off = np.concatenate([[0], np.cumsum(sz)[:-1]]).astype(np.int64)
+            sz.setflags(write=False)
+            off.setflags(write=False)
+            c = (src, sz, int(sz.sum()), off)
+            self._size_cache = c
+        return c
+
     @property
     def sizes(self) -> np.ndarray:
-        return self.rows.astype(np.int64) * self.cols.astype(np.int64)
+        return self._sized()[1]
 
     @property
     def nelem(self) -> int:
-        return int(self.sizes.sum())
+        return self._sized()[2]
 
     @property
     def offsets(self) -> np.ndarray:
-        return np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(np.int64)
+        return self._sized()[3]
 
     def entry(self, e: int) -> "BlockLayout":
         s = int(self.nblk[:e].sum())

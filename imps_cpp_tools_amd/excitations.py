@@ -131,23 +131,33 @@ class BlockOps:
             raise ValueError("product of two complex objects")
         p = self._plan(("prod", kind, _lkey(X1.lay), _lkey(X2.lay)), lambda: ProdPlan(self.ctx, kind, X1.lay, X2.lay))
         cplx = X1.cplx or X2.cplx
-        y = self._zeros(p.out_layout.nelem, cplx)
-        if p.out_layout.nelem and X1.lay.nelem and X2.lay.nelem:
-            if X1.cplx:
-                p.apply(X1.data[0], X2.data, y[0])
-                p.apply(X1.data[1], X2.data, y[1])
-            elif X2.cplx:
-                p.apply(X1.data, X2.data[0], y[0])
-                p.apply(X1.data, X2.data[1], y[1])
-            else:
-                p.apply(X1.data, X2.data, y)
+        ny = p.out_layout.nelem
+        if not (ny and X1.lay.nelem and X2.lay.nelem):
+            return DObj(p.out_layout, self._zeros(ny, cplx))
+        y = self.torch.empty((2, ny) if cplx else (ny,), dtype=self.torch.float64, device=self.dev)  # fully overwritten
+        if X1.cplx:
+            p.apply(X1.data[0], X2.data, y[0])
+            p.apply(X1.data[1], X2.data, y[1])
+        elif X2.cplx:
+            p.apply(X1.data, X2.data[0], y[0])
+            p.apply(X1.data, X2.data[1], y[1])
+        else:
+            p.apply(X1.data, X2.data, y)
         return DObj(p.out_layout, y)
 
     def scale(self, x: DObj, a: complex) -> DObj:
+        """a x as a complex object (x itself when x is complex and a = 1: results are never
+        written in place)."""
         ar, ai = float(np.real(a)), float(np.imag(a))
         if x.cplx:
+            if ai == 0.0:
+                return x if ar == 1.0 else DObj(x.lay, x.data * ar)
             re, im = x.data[0], x.data[1]
             return DObj(x.lay, self.torch.stack((ar * re - ai * im, ar * im + ai * re)))
+        if ai == 0.0:
+            out = self._zeros(x.lay.nelem, True)
+            out[0] = x.data if ar == 1.0 else x.data * ar
+            return DObj(x.lay, out)
         return DObj(x.lay, self.torch.stack((ar * x.data, ai * x.data)))
 
     def _union(self, la: BlockLayout, lb: BlockLayout):
@@ -177,10 +187,13 @@ class BlockOps:
 
     def add(self, a: DObj, b: DObj, alpha=1.0, beta=1.0) -> DObj:
         """alpha a + beta b on the union of the sector sets (BlockMat::operator+=)."""
+        sa, sb = self.scale(a, alpha).data, self.scale(b, beta).data
+        if _lkey(a.lay) == _lkey(b.lay):
+            return DObj(a.lay, sa + sb)
         lu, ia, ib = self._union(a.lay, b.lay)
         out = self._zeros(lu.nelem, True)
-        out[:, ia] += self.scale(a, alpha).data
-        out[:, ib] += self.scale(b, beta).data
+        out.index_add_(1, ia, sa)
+        out.index_add_(1, ib, sb)
         return DObj(lu, out)
 
     def onto(self, x: DObj, lay: BlockLayout) -> DObj:
