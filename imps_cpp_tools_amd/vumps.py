@@ -35,48 +35,7 @@ import numpy as np
 from . import (BF16X3, FP32, FP64, LEFT, RIGHT, MAT_MPS, MATT_MPS, MPS_MAT, MPS_MATT, MPS_REDOP, REDOP_MPS, TM_LEFT_UNITY,
                TM_RIGHT_UNITY, BlockLayout, Context, EnvPlan, HACPlan, HCPlan, ProdPlan, QRPlan, RedOpPlan, TMPlan, eigsh,
                make_dims, model_d, model_ham, model_nsym, mps_structure)
-
-
-# The two side streams of the concurrent site steps, one pair per device and main stream for the
-# whole process.  HIP multiplexes streams over a few hardware queues, and work of streams that share
-# a queue runs in order: a side stream on the main stream's queue (or both on one queue) serialises
-# the concurrent solves (measured: D = 1024 iterations 0.15 s instead of 0.11 s).  Which queue a
-# stream of torch's pool lands on is not exposed, so the pair is chosen by measurement: a bounded
-# spin kernel on the main stream and on each candidate together must take about as long as one alone.
-_SIDE_STREAMS: Dict[tuple, list] = {}
-
-
-def _span(torch, streams, cycles) -> float:
-    """Wall time of one spin kernel of `cycles` clock cycles on each stream, issued together."""
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for st in streams:
-        with torch.cuda.stream(st):
-            torch.cuda._sleep(cycles)
-    torch.cuda.synchronize()
-    return time.perf_counter() - t
-
-
-def _side_streams(torch, device: int) -> list:
-    main = torch.cuda.current_stream(device)
-    key = (device, main.cuda_stream)
-    if key not in _SIDE_STREAMS:
-        with torch.cuda.device(device):
-            cycles = 1 << 18
-            one = _span(torch, [main], cycles)  # also loads the kernel
-            while one < 2e-3 and cycles < 1 << 30:
-                cycles *= 4
-                one = _span(torch, [main], cycles)
-            picked, drawn = [], []
-            for _ in range(16):
-                st = torch.cuda.Stream()  # default priority
-                drawn.append(st)
-                if _span(torch, [main] + picked + [st], cycles) < 1.5 * one:
-                    picked.append(st)
-                    if len(picked) == 2:
-                        break
-            _SIDE_STREAMS[key] = (picked + [st for st in drawn if st not in picked])[:2]
-    return _SIDE_STREAMS[key]
+from .streams import side_streams
 
 
 def lower_sqrt_factor(r: np.ndarray, lay: BlockLayout) -> np.ndarray:
@@ -397,7 +356,7 @@ class VUMPS:
     def _side_contexts(self):
         if self._side is None:
             from concurrent.futures import ThreadPoolExecutor
-            streams = _side_streams(self.torch, self.ctx.device)
+            streams = side_streams(self.torch, self.ctx.device)
             self._side = ([Context(self.ctx.device, st) for st in streams], streams, ThreadPoolExecutor(max_workers=3))
         return self._side
 
