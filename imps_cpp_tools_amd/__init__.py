@@ -91,6 +91,7 @@ def lib():
             "imps_env_fixed_point": (I, [P, V, D, I, ip, dp]),
             "imps_prod_plan": (I, [P, ctypes.POINTER(P), I, LP, LP]),
             "imps_prod_apply": (I, [P, V, V, V]),
+            "imps_prod_apply_n": (I, [P, V, V, V, I]),
             "imps_qr_plan": (I, [P, ctypes.POINTER(P), LP, I]),
             "imps_qr_apply": (I, [P, V, V, V]),
             "imps_model_nsym": (I, [I]),
@@ -528,6 +529,10 @@ class ProdPlan(Plan):
 
     def apply(self, x1, x2, y):
         _check(lib().imps_prod_apply(self.h, _ptr(x1), _ptr(x2), _ptr(y)))
+
+    def apply_n(self, x1, x2, y, nvec: int):
+        """nvec first operands back to back in x1 (outputs back to back in y), one x2."""
+        _check(lib().imps_prod_apply_n(self.h, _ptr(x1), _ptr(x2), _ptr(y), int(nvec)))
 
 
 class QRPlan(Plan):

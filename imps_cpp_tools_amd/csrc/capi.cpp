@@ -371,10 +371,14 @@ imps_status imps_prod_plan(imps_context ctx, imps_plan* plan, int kind, const im
     });
 }
 imps_status imps_prod_apply(imps_plan P, const double* x1, const double* x2, double* y) {
+    return imps_prod_apply_n(P, x1, x2, y, 1);
+}
+
+imps_status imps_prod_apply_n(imps_plan P, const double* x1, const double* x2, double* y, int nvec) {
     return guarded([&] {
         check_kind(P, K_PROD);
         if (!x2 && P->prog.op_elems > 0) throw InvalidArg("x2 is NULL");
-        run(P, x1, y, 1, x2);
+        run(P, x1, y, nvec, x2);
         return IMPS_OK;
     });
 }

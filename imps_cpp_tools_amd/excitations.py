@@ -21,7 +21,6 @@ with AL C = C AR, LM = C^T and RM = C on bond 0 (eigenvalue 1, tr(LM RM) = 1).
 from __future__ import annotations
 
 import math
-import threading
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -98,16 +97,9 @@ class BlockOps:
         self.H0 = None if H is None else np.array(H, np.float64) - E0 * np.eye(d * d)
         self._plans: Dict = {}
         self._maps: Dict = {}
-        self._tls = threading.local()  # .ctx: the context of a side thread's launches
 
     # ---- helpers -----------------------------------------------------------------------------
-    def _context(self) -> Context:
-        return getattr(self._tls, "ctx", None) or self.ctx
-
     def _plan(self, key, make):
-        ctx = self._context()
-        if ctx is not self.ctx:
-            key = key + (id(ctx),)  # plans launch on their context's stream
         p = self._plans.get(key)
         if p is None:
             p = make()
@@ -128,7 +120,7 @@ class BlockOps:
         return t
 
     def _redop(self, A: DObj, direction):
-        p = self._plan(("redop", _lkey(A.lay), direction), lambda: RedOpPlan(self._context(), self.H0, self.d, A.lay, direction))
+        p = self._plan(("redop", _lkey(A.lay), direction), lambda: RedOpPlan(self.ctx, self.H0, self.d, A.lay, direction))
         y = self._zeros(p.out_layout.nelem, False)
         p.apply(A.data, y)
         return DObj(p.out_layout, y)
@@ -137,15 +129,14 @@ class BlockOps:
         """Block product through the library (grouped GEMM); complex operands as two launches."""
         if X1.cplx and X2.cplx:
             raise ValueError("product of two complex objects")
-        p = self._plan(("prod", kind, _lkey(X1.lay), _lkey(X2.lay)), lambda: ProdPlan(self._context(), kind, X1.lay, X2.lay))
+        p = self._plan(("prod", kind, _lkey(X1.lay), _lkey(X2.lay)), lambda: ProdPlan(self.ctx, kind, X1.lay, X2.lay))
         cplx = X1.cplx or X2.cplx
         ny = p.out_layout.nelem
         if not (ny and X1.lay.nelem and X2.lay.nelem):
             return DObj(p.out_layout, self._zeros(ny, cplx))
         y = self.torch.empty((2, ny) if cplx else (ny,), dtype=self.torch.float64, device=self.dev)  # fully overwritten
         if X1.cplx:
-            p.apply(X1.data[0], X2.data, y[0])
-            p.apply(X1.data[1], X2.data, y[1])
+            p.apply_n(X1.data.contiguous(), X2.data, y, 2)  # real and imaginary parts in one launch
         elif X2.cplx:
             p.apply(X1.data, X2.data[0], y[0])
             p.apply(X1.data, X2.data[1], y[1])
@@ -205,7 +196,10 @@ class BlockOps:
         return DObj(lu, out)
 
     def onto(self, x: DObj, lay: BlockLayout) -> DObj:
-        """x restricted to / zero-extended onto the sector set `lay`."""
+        """x restricted to / zero-extended onto the sector set `lay` (x itself when its structure
+        is `lay`: results are never written in place)."""
+        if _lkey(x.lay) == _lkey(lay):
+            return x
         key = ("onto", _lkey(x.lay), _lkey(lay))
         m = self._maps.get(key)
         if m is None:
@@ -517,13 +511,8 @@ class ExcitationEngine(BlockOps):
     caller.  All work is FP64."""
 
     def __init__(self, ctx: Context, AL, ALl, AR, ARl, NL, NLl, C, Cl, H: np.ndarray, d: int, E0: float, K, inv_tol: float = 1e-14,
-                 concurrent: bool = False, recycle: bool = False):
+                 recycle: bool = False):
         super().__init__(ctx, ALl[0].nsym, d, H, E0, inv_tol)
-        # concurrent: the two geometric sums of an apply on two streams and host threads.  At D = 512
-        # Hubbard (both sums ~29 GMRES steps) it measured slower (30 vs 22 ms per apply): the steps
-        # are host-bound and the two threads contend for the interpreter
-        self.concurrent = concurrent
-        self._side_state = None
         # recycle: start each geometric sum from the previous apply's solution, as
         # Excitations_main.cpp:248,274 does through ApplyHeff's pEBR / pEHBL (otherwise from zero).
         # Inside the eigensolver successive inputs are orthogonal Krylov vectors, so the previous
@@ -619,44 +608,6 @@ class ExcitationEngine(BlockOps):
         return sum(l.nelem for l in self.xdims)
 
     # ---- ApplyHeff ---------------------------------------------------------------------------
-    def _on_side(self, fn):
-        """Start fn() in the side thread, its launches on a side stream ordered after the main
-        stream's work so far; returns a future of (result, event recorded at its end)."""
-        t = self.torch
-        if self._side_state is None:
-            from concurrent.futures import ThreadPoolExecutor
-            from .streams import side_streams
-            st = side_streams(t, self.ctx.device)[0]
-            self._side_state = (Context(self.ctx.device, st), st, ThreadPoolExecutor(max_workers=1))
-        ctx, st, pool = self._side_state
-        ready = t.cuda.Event()
-        ready.record(t.cuda.current_stream())
-
-        def run():
-            self._tls.ctx = ctx
-            try:
-                with t.cuda.stream(st):
-                    st.wait_event(ready)
-                    out = fn()
-                    done = t.cuda.Event()
-                    done.record(st)
-                return out, done
-            finally:
-                self._tls.ctx = None
-        return pool.submit(run)
-
-    def close(self):
-        """Release the side thread (also on garbage collection)."""
-        if getattr(self, "_side_state", None) is not None:
-            self._side_state[2].shutdown(wait=True)
-            self._side_state = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
-
     def _left_sum(self, Bin: List[DObj], ck: complex, proj: bool) -> List[DObj]:
         """The left geometric sum of ApplyHeff: HBL (the Hamiltonian terms left of B) summed
         through (1 - e^{-ik} T_L)^-1 (Excitations_helpers.hpp:108-142)."""
@@ -682,9 +633,7 @@ class ExcitationEngine(BlockOps):
         return HBLtot
 
     def apply(self, X: List[DObj], p: float) -> List[DObj]:
-        """H_eff(k) X at momentum p (unit-cell phase p pi N, Excitations_main.cpp:258-259).  The
-        two geometric sums are independent; with `concurrent` the right one runs in a side thread
-        on a side stream while this thread runs the left one (same results as in sequence)."""
+        """H_eff(k) X at momentum p (unit-cell phase p pi N, Excitations_main.cpp:258-259)."""
         N, d = self.N, self.d
         pact = p * math.pi * N
         kfac = complex(math.cos(pact), math.sin(pact))
@@ -696,25 +645,15 @@ class ExcitationEngine(BlockOps):
         ABR = [self.tmr_unity(Bin[-1], AR[-1])]
         for n in range(N - 1, 0, -1):
             ABR.insert(0, self.tmr_unity(self.add(Bin[n - 1], self.mps_mat(AL[n - 1], ABR[0])), AR[n - 1]))
-
-        def right_sum():
-            y0 = self._ebr if self.recycle and self._ebr is not None and _lkey(self._ebr.lay) == _lkey(ABR[0].lay) else None
-            EBR, _ = self.invert_e(AL, AR, ABR[0], RIGHT, kfac, self.LM if proj else None, self.RM if proj else None, y0=y0)
-            self._ebr = EBR
-            tot = [self.scale(EBR, kfac)]
-            for n in range(N - 1, 0, -1):
-                EBR = self.tmr(AL[n], AR[n], EBR)
-                tot.insert(0, self.add(ABR[n], EBR, 1.0, kfac))
-            return tot[-1:] + tot[:-1]
-        fut = self._on_side(right_sum) if self.concurrent else None
-        try:
-            HBLtot = self._left_sum(Bin, ck, proj)
-        finally:
-            if fut is not None:
-                ABRtot, done = fut.result()
-                self.torch.cuda.current_stream().wait_event(done)
-        if fut is None:
-            ABRtot = right_sum()
+        y0 = self._ebr if self.recycle and self._ebr is not None and _lkey(self._ebr.lay) == _lkey(ABR[0].lay) else None
+        EBR, _ = self.invert_e(AL, AR, ABR[0], RIGHT, kfac, self.LM if proj else None, self.RM if proj else None, y0=y0)
+        self._ebr = EBR
+        ABRtot = [self.scale(EBR, kfac)]
+        for n in range(N - 1, 0, -1):
+            EBR = self.tmr(AL[n], AR[n], EBR)
+            ABRtot.insert(0, self.add(ABR[n], EBR, 1.0, kfac))
+        ABRtot = ABRtot[-1:] + ABRtot[:-1]
+        HBLtot = self._left_sum(Bin, ck, proj)
         Bout = []
         for n in range(N):
             b = self.add(self.mat_mps(self.HLtot[P(n - 1)], Bin[n]), self.mps_mat(Bin[n], self.HRtot[P(n + 1)]))

@@ -5,8 +5,7 @@ import time
 from typing import Dict
 
 # Two side streams per device and main stream for the whole process, for solves that overlap the
-# main stream's work (the concurrent VUMPS site step, the two geometric sums of an excitation
-# apply).  HIP multiplexes streams over a few hardware queues, and work of streams that share a
+# main stream's work (the concurrent VUMPS site step).  HIP multiplexes streams over a few hardware queues, and work of streams that share a
 # queue runs in order: a side stream on the main stream's queue (or both on one queue) serialises
 # the concurrent solves (measured: D = 1024 VUMPS iterations 0.15 s instead of 0.11 s).  Which
 # queue a stream of torch's pool lands on is not exposed, so the pair is chosen by measurement: a

@@ -191,6 +191,11 @@ imps_status imps_eigsh(imps_plan plan, imps_plan plan_fp64, double* x, double to
 imps_status imps_prod_plan(imps_context ctx, imps_plan* plan, int kind, const imps_layout* X1,
                            const imps_layout* X2);
 imps_status imps_prod_apply(imps_plan plan, const double* x1, const double* x2, double* y);
+/* The same product for nvec first operands stored back to back (x1[v*len_x1 + i], outputs
+ * y[v*len_y + i]) with one second operand x2: the real and imaginary parts of a complex first
+ * operand in one launch, as the reference's complex BlockMat products need them
+ * (Excitations_helpers.hpp:20-228 with VTX = Complex). */
+imps_status imps_prod_apply_n(imps_plan plan, const double* x1, const double* x2, double* y, int nvec);
 
 /* ---- per-sector QR (A18) -------------------------------------------------------------------
  * qr(R, MPS, dir) (MPSBlockUtilities.hpp:181-244) with qr_pos (helpers.hpp:181-195): the dense

@@ -95,6 +95,12 @@ def test_products(ctx, name):
         y = torch.full((ref.nelem,), float("nan"), dtype=torch.float64, device="cuda")
         p.apply(dev(x1.data), dev(x2.data), y)
         assert relerr(y.cpu().numpy(), ref.data) < 1e-13, kind
+        # two first operands back to back (x1 and -x1 / 2, whose product is -ref / 2) in one launch
+        y2 = torch.full((2, ref.nelem), float("nan"), dtype=torch.float64, device="cuda")
+        p.apply_n(dev(np.stack([x1.data, -0.5 * x1.data])), dev(x2.data), y2, 2)
+        got = y2.cpu().numpy()
+        assert relerr(got[0], ref.data) < 1e-13, kind
+        assert relerr(got[1], -0.5 * ref.data) < 1e-13, kind
 
 
 def _entry(ob, e):
