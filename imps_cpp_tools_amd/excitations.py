@@ -317,7 +317,9 @@ class BlockOps:
         rnorm = t.linalg.vector_norm(r).item()
         cap = min(kry + 1, 64)
         V = t.empty((cap, dim), dtype=b.dtype, device=b.device)
+        Vc = t.empty_like(V)  # conjugated basis: the inner products read it without conjugating V per step
         V[0] = r / rnorm
+        Vc[0] = V[0].conj()
         Hm = np.zeros((kry + 1, kry + 1), complex)
         c = np.zeros(kry + 1, complex)
         s = np.zeros(kry + 1, complex)
@@ -326,11 +328,11 @@ class BlockOps:
         kdim, flag = 0, -1
         for j in range(kry):
             w = afun(V[j])
-            Vj = V[:j + 1]
-            h1 = t.mv(Vj.conj(), w)
-            w = w - t.mv(Vj.transpose(0, 1), h1)
-            h2 = t.mv(Vj.conj(), w)
-            w = w - t.mv(Vj.transpose(0, 1), h2)
+            Vj, Vcj = V[:j + 1], Vc[:j + 1]
+            h1 = t.mv(Vcj, w)
+            w = t.addmv(w, Vj.transpose(0, 1), h1, alpha=-1)
+            h2 = t.mv(Vcj, w)
+            w = t.addmv(w, Vj.transpose(0, 1), h2, alpha=-1)
             hn = t.linalg.vector_norm(w)
             coef = t.cat((h1 + h2, hn.to(h1.dtype).reshape(1))).cpu().numpy()
             Hm[:j + 1, j] = coef[:j + 1]
@@ -347,7 +349,11 @@ class BlockOps:
                 grown = t.empty((min(kry + 1, 2 * V.shape[0]), dim), dtype=b.dtype, device=b.device)
                 grown[:V.shape[0]] = V
                 V = grown
+                grown = t.empty_like(V)
+                grown[:Vc.shape[0]] = Vc
+                Vc = grown
             V[j + 1] = w / Hm[j + 1, j]
+            Vc[j + 1] = V[j + 1].conj()
             habs = abs(Hm[j, j])
             phase = Hm[j, j] / habs if habs > 0 else 1.0
             s[j] = phase * np.conj(Hm[j + 1, j]) / beta

@@ -29,6 +29,9 @@ def side_streams(torch, device: int) -> list:
     """The two side streams for torch's current stream on `device` (chosen on the first call)."""
     main = torch.cuda.current_stream(device)
     key = (device, main.cuda_stream)
+    if key not in _SIDE_STREAMS and not hasattr(torch.cuda, "_sleep"):
+        with torch.cuda.device(device):  # no spin kernel to measure with: two pool streams
+            _SIDE_STREAMS[key] = [torch.cuda.Stream() for _ in range(2)]
     if key not in _SIDE_STREAMS:
         with torch.cuda.device(device):
             cycles = 1 << 18
