@@ -159,8 +159,9 @@ class BlockOps:
             return DObj(x.lay, out)
         return DObj(x.lay, self.torch.stack((ar * x.data, ai * x.data)))
 
-    def _union(self, la: BlockLayout, lb: BlockLayout):
-        key = ("union", _lkey(la), _lkey(lb))
+    def _union_host(self, la: BlockLayout, lb: BlockLayout):
+        """(union layout, positions of la's elements in it, positions of lb's) on the host."""
+        key = ("union_host", _lkey(la), _lkey(lb))
         m = self._maps.get(key)
         if m is None:
             if len(la.nblk) != len(lb.nblk):
@@ -179,10 +180,68 @@ class BlockOps:
                 for k, (qo, r, c, o) in src.items():
                     out[o:o + r * c] = iu[k][3] + np.arange(r * c)
                 return out
-            t = self.torch
-            m = (lu, t.from_numpy(positions(ia_)).to(self.dev), t.from_numpy(positions(ib_)).to(self.dev))
+            m = (lu, positions(ia_), positions(ib_))
             self._maps[key] = m
         return m
+
+    def _union(self, la: BlockLayout, lb: BlockLayout):
+        key = ("union", _lkey(la), _lkey(lb))
+        m = self._maps.get(key)
+        if m is None:
+            lu, pa, pb = self._union_host(la, lb)
+            t = self.torch
+            m = (lu, t.from_numpy(pa).to(self.dev), t.from_numpy(pb).to(self.dev))
+            self._maps[key] = m
+        return m
+
+    def _combine_rows(self, key, src, parts, rows) -> DObj:
+        """concat over rows of combine(terms): the same layout as the chain of adds (sector sets
+        united in term order), computed as one gather of the source elements, a scaling and a sum
+        in term order per output element.  src: the sources' data side by side ((2, n) or (n,));
+        parts[p] = (layout, element offset in src); rows[r] = [(coefficient, part), ...]."""
+        t = self.torch
+        m = self._maps.get(key)
+        if m is None:
+            out_lays, trip, base = [], [], 0
+            for terms in rows:
+                if not terms:
+                    out_lays.append(BlockLayout(self.nsym, [0], np.zeros((0, self.nsym)), np.zeros((0, self.nsym)), [], []))
+                    continue
+                c0, p0 = terms[0]
+                lay = parts[p0][0]
+                placed = [(np.arange(lay.nelem), p0, c0)]
+                for c, p in terms[1:]:
+                    lay, pa, pb = self._union_host(lay, parts[p][0])
+                    placed = [(pa[pos], q, cq) for pos, q, cq in placed]
+                    placed.append((pb, p, c))
+                out_lays.append(lay)
+                for pos, q, cq in placed:
+                    trip.append((base + pos, parts[q][1] + np.arange(parts[q][0].nelem), np.full(len(pos), float(cq))))
+                base += lay.nelem
+            lay = BlockLayout.concat(out_lays)
+            if trip:
+                outs = np.concatenate([a for a, _, _ in trip])
+                ins = np.concatenate([b for _, b, _ in trip])
+                cs = np.concatenate([c for _, _, c in trip])
+                order = np.argsort(outs, kind="stable")  # term order within each output element
+                outs, ins, cs = outs[order], ins[order], cs[order]
+                start = np.searchsorted(outs, np.arange(base))
+                rank = np.arange(len(outs)) - start[outs]
+                width = int(rank.max()) + 1
+                idx = np.zeros((base, width), np.int64)
+                coef = np.zeros((base, width))
+                idx[outs, rank] = ins
+                coef[outs, rank] = cs
+                m = (lay, t.from_numpy(idx).to(self.dev), t.from_numpy(coef).to(self.dev))
+            else:
+                m = (lay, None, None)
+            self._maps[key] = m
+        lay, idx, coef = m
+        if idx is None:
+            return DObj(lay, self._zeros(0, True))
+        if src.dim() == 1:
+            src = t.stack((src, t.zeros_like(src)))
+        return DObj(lay, (src[:, idx] * coef).sum(-1))
 
     def add(self, a: DObj, b: DObj, alpha=1.0, beta=1.0) -> DObj:
         """alpha a + beta b on the union of the sector sets (BlockMat::operator+=)."""
@@ -261,44 +320,39 @@ class BlockOps:
     def apply_operator(self, AA: DObj, H: np.ndarray) -> DObj:
         """out[row] = sum_col H(row, col) AA[col] (ApplyOperator, MPSBlockUtilities.hpp:18-34)."""
         D = H.shape[0]
-        ents = [self.entry(AA, e) for e in range(D)]
-        outs = []
-        for row in range(D):
-            terms = [(H[row, col], ents[col]) for col in range(D) if H[row, col] != 0.0]
-            outs.append(self.combine(terms) if terms else DObj(BlockLayout(self.nsym, [0], np.zeros((0, self.nsym)), np.zeros((0, self.nsym)),
-                                                                              [], []), self._zeros(0, AA.cplx)))
-        return self.concat(outs)
+        parts = [(AA.lay.entry(e), AA.lay.entry_range(e)[0]) for e in range(D)]
+        rows = [[(H[row, col], col) for col in range(D) if H[row, col] != 0.0] for row in range(D)]
+        return self._combine_rows(("aop", _lkey(AA.lay), H.tobytes()), AA.data, parts, rows)
 
     def diag_part(self, x: DObj) -> DObj:
         idx = {k: v for k, v in _index(x.lay).items() if k[1] == v[0]}
         return self.onto(x, _layout_from_index(x.lay.nsym, len(x.lay.nblk), idx))
 
+    def _grid(self, name, T: List[DObj], coef) -> DObj:
+        """Row-major d x d grid of combinations of the entries T[k][i]: entry (a, b) is
+        sum_{k,i} coef(a, b, k, i) T[k][i] over the nonzero coefficients, in (k, i) order."""
+        d = self.d
+        parts, off = [], 0
+        for Tk in T:
+            for i in range(d):
+                parts.append((Tk.lay.entry(i), off + Tk.lay.entry_range(i)[0]))
+            off += Tk.lay.nelem
+        rows = [[(coef(a, b, k, i), k * d + i) for k in range(d) for i in range(d) if coef(a, b, k, i) != 0.0]
+                for a in range(d) for b in range(d)]
+        src = self.torch.cat([Tk.data for Tk in T], dim=-1)
+        return self._combine_rows((name, tuple(_lkey(Tk.lay) for Tk in T), self.H0.tobytes()), src, parts, rows)
+
     def redop_left_mixed(self, Abra: DObj, Bket: DObj) -> DObj:
         """M[s][j] = sum_{k,i} H(k d + s, i d + j) Abra[k]^T Bket[i] (d x d grid, row-major)."""
         d, H = self.d, self.H0
         T = [self.prod(MATT_MPS, self.entry(Abra, k), Bket) for k in range(d)]  # T[k][i] = Abra[k]^T Bket[i]
-        Te = [[self.entry(T[k], i) for i in range(d)] for k in range(d)]
-        grid = []
-        for s in range(d):
-            for j in range(d):
-                terms = [(H[k * d + s, i * d + j], Te[k][i]) for k in range(d) for i in range(d) if H[k * d + s, i * d + j] != 0.0]
-                grid.append(self.combine(terms) if terms else self._empty_entry(True))
-        return self.concat(grid)
+        return self._grid("rlm", T, lambda s, j, k, i: H[k * d + s, i * d + j])
 
     def redop_right_mixed(self, Aket: DObj, Bbra: DObj) -> DObj:
         """M[s][i] = sum_{j,k} H(s d + k, i d + j) Aket[j] Bbra[k]^T (d x d grid, row-major)."""
         d, H = self.d, self.H0
         U = [self.prod(MPS_MATT, Aket, self.entry(Bbra, k)) for k in range(d)]  # U[k][j] = Aket[j] Bbra[k]^T
-        Ue = [[self.entry(U[k], j) for j in range(d)] for k in range(d)]
-        grid = []
-        for s in range(d):
-            for i in range(d):
-                terms = [(H[s * d + k, i * d + j], Ue[k][j]) for k in range(d) for j in range(d) if H[s * d + k, i * d + j] != 0.0]
-                grid.append(self.combine(terms) if terms else self._empty_entry(True))
-        return self.concat(grid)
-
-    def _empty_entry(self, cplx):
-        return DObj(BlockLayout(self.nsym, [0], np.zeros((0, self.nsym)), np.zeros((0, self.nsym)), [], []), self._zeros(0, cplx))
+        return self._grid("rrm", U, lambda s, i, k, j: H[s * d + k, i * d + j])
 
     # ---- geometric sums ------------------------------------------------------------------------
     def _gmres(self, afun, b, x0, tol):
