@@ -387,8 +387,10 @@ class BlockOps:
             w = t.addmv(w, Vj.transpose(0, 1), h1, alpha=-1)
             h2 = t.mv(Vcj, w)
             w = t.addmv(w, Vj.transpose(0, 1), h2, alpha=-1)
-            hn = t.linalg.vector_norm(w)
-            coef = t.cat((h1 + h2, hn.to(h1.dtype).reshape(1))).cpu().numpy()
+            cbuf = t.empty(j + 2, dtype=b.dtype, device=b.device)  # coefficients and ||w||, one transfer
+            t.add(h1, h2, out=cbuf[:j + 1])
+            cbuf[j + 1] = t.linalg.vector_norm(w)
+            coef = cbuf.cpu().numpy()
             Hm[:j + 1, j] = coef[:j + 1]
             Hm[j + 1, j] = coef[j + 1].real
             for i in range(j):
@@ -406,8 +408,8 @@ class BlockOps:
                 grown = t.empty_like(V)
                 grown[:Vc.shape[0]] = Vc
                 Vc = grown
-            V[j + 1] = w / Hm[j + 1, j]
-            Vc[j + 1] = V[j + 1].conj()
+            t.div(w, complex(Hm[j + 1, j]), out=V[j + 1])
+            t.conj_physical(V[j + 1], out=Vc[j + 1])
             habs = abs(Hm[j, j])
             phase = Hm[j, j] / habs if habs > 0 else 1.0
             s[j] = phase * np.conj(Hm[j + 1, j]) / beta
@@ -461,6 +463,7 @@ class BlockOps:
 
     def invert_e(self, As, Bs, x: DObj, direction, fac, L: Optional[DObj] = None, R: Optional[DObj] = None, y0=None):
         """InvertE_fac (L, R None) or InvertE_proj_fac of the mixed unit-cell TM (As, Bs)."""
+        t = self.torch
         lay = x.lay
         T = (lambda v: self._chain_left(As, Bs, v)) if direction == LEFT else (lambda v: self._chain_right(As, Bs, v))
         proj = L is not None and R is not None
@@ -477,7 +480,7 @@ class BlockOps:
         def afun(z):
             xin = self._p(z, lay)
             tx = self._c(self.onto(T(xin), lay))
-            out = z - fac * tx
+            out = t.sub(z, tx, alpha=fac)  # z - fac T z
             if proj:
                 tr = self._tr(xin, rin) if direction == LEFT else self._tr(rin, xin)
                 out = out + tr * routc
