@@ -101,6 +101,8 @@ def test_products(ctx, name):
         got = y2.cpu().numpy()
         assert relerr(got[0], ref.data) < 1e-13, kind
         assert relerr(got[1], -0.5 * ref.data) < 1e-13, kind
+        with pytest.raises(im.ImpsError):  # no vectors: rejected on the host before any launch
+            p.apply_n(dev(x1.data), dev(x2.data), y, 0)
 
 
 def _entry(ob, e):
