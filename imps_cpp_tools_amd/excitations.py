@@ -435,31 +435,40 @@ class BlockOps:
         return DObj(lay, self.torch.stack((z.real, z.imag)).contiguous())
 
     def _tr(self, a: DObj, b: DObj) -> complex:
-        """trace(a * b) summed over sectors (a real or complex, b real or complex)."""
-        ia, ib = _index(a.lay), _index(b.lay)
-        tot = 0j
-        for (e, q), (qo, r, c, o) in ia.items():
-            v = ib.get((e, qo))
-            if v is None:
-                continue
-            qo2, r2, c2, o2 = v
-            if qo2 != q:
-                continue
-            A = a.data[..., o:o + r * c]
-            B = b.data[..., o2:o2 + r2 * c2]
-            Am = (A.view(2, c, r).transpose(1, 2) if a.cplx else A.view(c, r).t())
-            Bm = (B.view(2, c2, r2).transpose(1, 2) if b.cplx else B.view(c2, r2).t())
-            # tr(A B) = sum_ij A_ij B_ji
-            if a.cplx and b.cplx:
-                tot += complex((Am[0] * Bm[0].t()).sum().item() - (Am[1] * Bm[1].t()).sum().item(),
-                               (Am[0] * Bm[1].t()).sum().item() + (Am[1] * Bm[0].t()).sum().item())
-            elif a.cplx:
-                tot += complex((Am[0] * Bm.t()).sum().item(), (Am[1] * Bm.t()).sum().item())
-            elif b.cplx:
-                tot += complex((Am * Bm[0].t()).sum().item(), (Am * Bm[1].t()).sum().item())
-            else:
-                tot += (Am * Bm.t()).sum().item()
-        return tot
+        """trace(a * b) summed over sectors (a real or complex, b real or complex): sum over the
+        matching blocks of A_ij B_ji, as one gather of both operands (cached element pairs) and
+        one host transfer."""
+        t = self.torch
+        key = ("tr", _lkey(a.lay), _lkey(b.lay))
+        m = self._maps.get(key)
+        if m is None:
+            ia, ib = _index(a.lay), _index(b.lay)
+            xa, xb = [], []
+            for (e, q), (qo, r, c, o) in ia.items():
+                v = ib.get((e, qo))
+                if v is None or v[0] != q:
+                    continue
+                _, r2, c2, o2 = v
+                if (r2, c2) != (c, r):
+                    raise ValueError("trace of blocks whose shapes do not match")
+                i, j = np.meshgrid(np.arange(r), np.arange(c), indexing="ij")
+                xa.append((o + i + j * r).ravel())   # A_ij, column-major
+                xb.append((o2 + j + i * r2).ravel())  # B_ji
+            cat = lambda parts: t.from_numpy(np.concatenate(parts) if parts else np.zeros(0, np.int64)).to(self.dev)  # noqa: E731
+            m = (cat(xa), cat(xb))
+            self._maps[key] = m
+        ja, jb = m
+        A, B = a.data[..., ja], b.data[..., jb]
+        if a.cplx and b.cplx:
+            re, im = (A[0] * B[0] - A[1] * B[1]).sum(), (A[0] * B[1] + A[1] * B[0]).sum()
+        elif a.cplx:
+            re, im = (A[0] * B).sum(), (A[1] * B).sum()
+        elif b.cplx:
+            re, im = (A * B[0]).sum(), (A * B[1]).sum()
+        else:
+            re, im = (A * B).sum(), t.zeros((), dtype=A.dtype, device=A.device)
+        v = t.stack((re, im)).cpu().numpy()
+        return complex(float(v[0]), float(v[1]))
 
     def invert_e(self, As, Bs, x: DObj, direction, fac, L: Optional[DObj] = None, R: Optional[DObj] = None, y0=None):
         """InvertE_fac (L, R None) or InvertE_proj_fac of the mixed unit-cell TM (As, Bs)."""

@@ -74,3 +74,33 @@ def test_apply_operator_dense_sector():
     blocks = AA.data.numpy().reshape(2, 4, 6)
     want = np.einsum("rc,pcx->prx", H, blocks).reshape(2, -1)
     np.testing.assert_allclose(got.data.numpy(), want, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("cplx", [(False, False), (True, False), (False, True), (True, True)])
+def test_sector_trace(cplx):
+    """_tr(a, b) = sum over sectors of trace(A_q B_q'), B's block ingoing where A's goes out and
+    back; sectors without a partner contribute nothing."""
+    rng = np.random.default_rng(9)
+    qa = [0, 2, 4]          # A: q -> q + 1, B: q + 1 -> q for q in 0, 2 (4 has no partner)
+    shp = {0: (2, 3), 2: (3, 1), 4: (2, 2)}
+    la = BlockLayout(1, [3], [[q] for q in qa], [[q + 1] for q in qa], [shp[q][0] for q in qa], [shp[q][1] for q in qa])
+    lb = BlockLayout(1, [2], [[1], [3]], [[0], [2]], [shp[0][1], shp[2][1]], [shp[0][0], shp[2][0]])
+
+    def obj(lay, c):
+        n = lay.nelem
+        return DObj(lay, torch.from_numpy(rng.standard_normal((2, n)) if c else rng.standard_normal(n)))
+    a, b = obj(la, cplx[0]), obj(lb, cplx[1])
+    ops = _ops()
+
+    def blocks(x, lay):
+        d = x.data.numpy()
+        z = d[0] + 1j * d[1] if x.cplx else d.astype(complex)
+        out, o = {}, 0
+        for q, r, c in zip(lay.qin[:, 0], lay.rows, lay.cols):
+            out[int(q)] = z[o:o + r * c].reshape(r, c, order="F")
+            o += r * c
+        return out
+    A, B = blocks(a, la), blocks(b, lb)
+    want = sum(np.trace(A[q] @ B[q + 1]) for q in (0, 2))
+    got = ops._tr(a, b)
+    assert abs(got - want) <= 1e-13 * max(1.0, abs(want))
