@@ -292,11 +292,18 @@ def mps_structure(model: int, N: int, QN: Sequence[int], left, right) -> BlockLa
 
 # ----------------------------------------------------------------------------------------------
 # device context and plans
-def _ptr(t) -> Optional[int]:
+def _ptr(t, n: Optional[int] = None, device: Optional[int] = None, what: str = "buffer") -> Optional[int]:
+    """Device address of a contiguous float64 CUDA tensor.  With n, the tensor must hold at least n
+    elements; with device, it must live on that GPU: the library reads and writes exactly the
+    extents its plans derive, so a short or foreign buffer is rejected here, before any launch."""
     if t is None:
         return None
     if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8 and t.is_contiguous()):
-        raise ValueError("device buffers must be contiguous float64 CUDA tensors")
+        raise ValueError(f"{what}: device buffers must be contiguous float64 CUDA tensors")
+    if n is not None and t.numel() < n:
+        raise ValueError(f"{what}: {t.numel()} elements, the plan needs {n}")
+    if device is not None and t.device.index != device:
+        raise ValueError(f"{what}: tensor on cuda:{t.device.index}, the plan's context is on cuda:{device}")
     return t.data_ptr()
 
 
@@ -337,8 +344,12 @@ class Context:
 class Plan:
     kind = ""
 
-    def __init__(self, handle):
+    def __init__(self, handle, ctx: Optional["Context"] = None):
         self.h = handle
+        self.device = ctx.device if ctx is not None else None
+
+    def _p(self, t, n: int, what: str):
+        return _ptr(t, n, self.device, f"{self.kind} {what}")
 
     def _out_layout(self, nsym: int) -> BlockLayout:
         nl, nb, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
@@ -403,15 +414,16 @@ class TMPlan(Plan):
         cB = B.c() if B is not None else None
         _check(lib().imps_tm_plan(_ctx_handle(ctx), ctypes.byref(h), direction, ctypes.byref(cA),
                                   ctypes.byref(cB) if cB is not None else None, ctypes.byref(cx), ctypes.byref(cy), prec))
-        super().__init__(h)
+        super().__init__(h, ctx)
         self.mixed = B is not None
         self.xlen, self.ylen = x.nelem, y.nelem
+        self.alen, self.blen = A.nelem, (B.nelem if B is not None else 0)
 
     def set_mps(self, A, B=None):
-        _check(lib().imps_tm_set_mps(self.h, _ptr(A), _ptr(B)))
+        _check(lib().imps_tm_set_mps(self.h, self._p(A, self.alen, "A"), self._p(B, self.blen, "B")))
 
     def apply(self, x, y, nvec: int = 1):
-        _check(lib().imps_tm_apply(self.h, _ptr(x), _ptr(y), nvec))
+        _check(lib().imps_tm_apply(self.h, self._p(x, nvec * self.xlen, "x"), self._p(y, nvec * self.ylen, "y"), nvec))
 
 
 class HACPlan(Plan):
@@ -423,14 +435,17 @@ class HACPlan(Plan):
         h = ctypes.c_void_p()
         c = [l.c() for l in (AC, EHL, EHR, HL, HR)]
         _check(lib().imps_hac_plan(_ctx_handle(ctx), ctypes.byref(h), *[ctypes.byref(x) for x in c], prec))
-        super().__init__(h)
+        super().__init__(h, ctx)
         self.n = AC.nelem
+        self.oplen = (EHL.nelem, EHR.nelem, HL.nelem, HR.nelem)
 
     def set_operator(self, EHL, EHR, HL, HR):
-        _check(lib().imps_hac_set_operator(self.h, _ptr(EHL), _ptr(EHR), _ptr(HL), _ptr(HR)))
+        a, b, c, d = self.oplen
+        _check(lib().imps_hac_set_operator(self.h, self._p(EHL, a, "EHL"), self._p(EHR, b, "EHR"), self._p(HL, c, "HL"),
+                                           self._p(HR, d, "HR")))
 
     def apply(self, x, y, nvec: int = 1):
-        _check(lib().imps_hac_apply(self.h, _ptr(x), _ptr(y), nvec))
+        _check(lib().imps_hac_apply(self.h, self._p(x, nvec * self.n, "x"), self._p(y, nvec * self.n, "y"), nvec))
 
 
 class HCPlan(Plan):
@@ -442,14 +457,17 @@ class HCPlan(Plan):
         h = ctypes.c_void_p()
         c = [l.c() for l in (C, EHL, EHR, HL, AR)]
         _check(lib().imps_hc_plan(_ctx_handle(ctx), ctypes.byref(h), *[ctypes.byref(x) for x in c], prec))
-        super().__init__(h)
+        super().__init__(h, ctx)
         self.n = C.nelem
+        self.oplen = (EHL.nelem, EHR.nelem, HL.nelem, AR.nelem)
 
     def set_operator(self, EHL, EHR, HL, AR):
-        _check(lib().imps_hc_set_operator(self.h, _ptr(EHL), _ptr(EHR), _ptr(HL), _ptr(AR)))
+        a, b, c, d = self.oplen
+        _check(lib().imps_hc_set_operator(self.h, self._p(EHL, a, "EHL"), self._p(EHR, b, "EHR"), self._p(HL, c, "HL"),
+                                          self._p(AR, d, "AR")))
 
     def apply(self, x, y, nvec: int = 1):
-        _check(lib().imps_hc_apply(self.h, _ptr(x), _ptr(y), nvec))
+        _check(lib().imps_hc_apply(self.h, self._p(x, nvec * self.n, "x"), self._p(y, nvec * self.n, "y"), nvec))
 
 
 class RedOpPlan(Plan):
@@ -461,11 +479,12 @@ class RedOpPlan(Plan):
         Hf = np.asfortranarray(H, dtype=np.float64).reshape(-1, order="F").copy()
         cA = A.c()
         _check(lib().imps_redop_plan(_ctx_handle(ctx), ctypes.byref(h), _dptr(Hf), d, ctypes.byref(cA), direction, prec))
-        super().__init__(h)
+        super().__init__(h, ctx)
         self.out_layout = self._out_layout(A.nsym)
+        self.alen = A.nelem
 
     def apply(self, A, y):
-        _check(lib().imps_redop_apply(self.h, _ptr(A), _ptr(y)))
+        _check(lib().imps_redop_apply(self.h, self._p(A, self.alen, "A"), self._p(y, self.out_layout.nelem, "y")))
 
 
 class EnvPlan(Plan):
@@ -478,26 +497,30 @@ class EnvPlan(Plan):
         self._A = list(A)
         cx = x.c()
         _check(lib().imps_env_plan(_ctx_handle(ctx), ctypes.byref(h), direction, len(A), arr, ctypes.byref(cx), prec))
-        super().__init__(h)
+        super().__init__(h, ctx)
         self.n = x.nelem
 
     def set_mps(self, tensors):
-        arr = (ctypes.c_void_p * len(tensors))(*[_ptr(t) for t in tensors])
+        if len(tensors) != len(self._A):
+            raise ValueError(f"env: {len(tensors)} tensors for a cell of {len(self._A)}")
+        arr = (ctypes.c_void_p * len(tensors))(*[self._p(t, a.nelem, f"A[{k}]") for k, (t, a) in enumerate(zip(tensors, self._A))])
         _check(lib().imps_env_set_mps(self.h, arr))
 
     def tm_apply(self, x, y):
-        _check(lib().imps_env_tm_apply(self.h, _ptr(x), _ptr(y)))
+        _check(lib().imps_env_tm_apply(self.h, self._p(x, self.n, "x"), self._p(y, self.n, "y")))
 
     def fixed_point(self, x, tol: float = 1e-14, maxit: int = 0, check: bool = True):
         it, dl = ctypes.c_int(), ctypes.c_double()
-        st = lib().imps_env_fixed_point(self.h, _ptr(x), tol, maxit, ctypes.byref(it), ctypes.byref(dl))
+        st = lib().imps_env_fixed_point(self.h, self._p(x, self.n, "x"), tol, maxit, ctypes.byref(it), ctypes.byref(dl))
         if check:
             _check(st)
         return it.value, dl.value
 
     def solve(self, x, L, R, y, y0=None, tol: float = 1e-14, maxit: int = 0, krylov: int = 40, check: bool = True):
         it, rr = ctypes.c_int(), ctypes.c_double()
-        st = lib().imps_env_solve(self.h, _ptr(x), _ptr(L), _ptr(R), _ptr(y0), _ptr(y), tol, maxit, krylov, ctypes.byref(it), ctypes.byref(rr))
+        n = self.n
+        st = lib().imps_env_solve(self.h, self._p(x, n, "x"), self._p(L, n, "L"), self._p(R, n, "R"), self._p(y0, n, "y0"),
+                                  self._p(y, n, "y"), tol, maxit, krylov, ctypes.byref(it), ctypes.byref(rr))
         if check:
             _check(st)
         return it.value, rr.value
@@ -506,7 +529,7 @@ class EnvPlan(Plan):
 def eigsh(plan: Plan, x, tol: float = 1e-12, maxit: int = 0, krylov: int = 20, plan_fp64: Optional[Plan] = None, check: bool = True):
     """Lowest eigenpair of an H_AC / H_C plan (eigs_rs "SA", nev = 1); x: start vector in, eigenvector out."""
     ev, it, rr = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
-    st = lib().imps_eigsh(plan.h, plan_fp64.h if plan_fp64 is not None else None, _ptr(x), tol, maxit, krylov,
+    st = lib().imps_eigsh(plan.h, plan_fp64.h if plan_fp64 is not None else None, plan._p(x, plan.n, "x"), tol, maxit, krylov,
                           ctypes.byref(ev), ctypes.byref(it), ctypes.byref(rr))
     if check:
         _check(st)
@@ -524,15 +547,19 @@ class ProdPlan(Plan):
         h = ctypes.c_void_p()
         c1, c2 = X1.c(), X2.c()
         _check(lib().imps_prod_plan(_ctx_handle(ctx), ctypes.byref(h), kind, ctypes.byref(c1), ctypes.byref(c2)))
-        super().__init__(h)
+        super().__init__(h, ctx)
         self.out_layout = self._out_layout(X1.nsym)
+        self.len1, self.len2 = X1.nelem, X2.nelem
 
     def apply(self, x1, x2, y):
-        _check(lib().imps_prod_apply(self.h, _ptr(x1), _ptr(x2), _ptr(y)))
+        _check(lib().imps_prod_apply(self.h, self._p(x1, self.len1, "x1"), self._p(x2, self.len2, "x2"),
+                                     self._p(y, self.out_layout.nelem, "y")))
 
     def apply_n(self, x1, x2, y, nvec: int):
         """nvec first operands back to back in x1 (outputs back to back in y), one x2."""
-        _check(lib().imps_prod_apply_n(self.h, _ptr(x1), _ptr(x2), _ptr(y), int(nvec)))
+        nvec = int(nvec)
+        _check(lib().imps_prod_apply_n(self.h, self._p(x1, nvec * self.len1, "x1"), self._p(x2, self.len2, "x2"),
+                                       self._p(y, nvec * self.out_layout.nelem, "y"), nvec))
 
 
 class QRPlan(Plan):
@@ -543,8 +570,10 @@ class QRPlan(Plan):
         h = ctypes.c_void_p()
         cA = A.c()
         _check(lib().imps_qr_plan(_ctx_handle(ctx), ctypes.byref(h), ctypes.byref(cA), direction))
-        super().__init__(h)
+        super().__init__(h, ctx)
         self.out_layout = self._out_layout(A.nsym)
+        self.alen = A.nelem
 
     def apply(self, A, Q, R):
-        _check(lib().imps_qr_apply(self.h, _ptr(A), _ptr(Q), _ptr(R)))
+        _check(lib().imps_qr_apply(self.h, self._p(A, self.alen, "A"), self._p(Q, self.alen, "Q"),
+                                   self._p(R, self.out_layout.nelem, "R")))

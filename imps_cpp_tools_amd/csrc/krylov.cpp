@@ -7,6 +7,7 @@
 #include <memory>
 #include <cmath>
 #include <map>
+#include <mutex>
 
 #include "blas.hpp"
 
@@ -15,14 +16,19 @@ namespace imps {
 namespace {
 
 // Pinned host staging for device-to-host reads (one buffer per host thread, grown on demand and
-// kept for the life of the process): the copy is a direct DMA instead of a staged pageable copy.
+// freed when the thread exits): the copy is a direct DMA instead of a staged pageable copy.
 struct PinnedStage {
     double* p = nullptr;
     size_t cap = 0;
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage&) = delete;
+    PinnedStage& operator=(const PinnedStage&) = delete;
+    ~PinnedStage() { if (p) (void)hipHostFree(p); }
     double* get(size_t n) {
         if (n > cap) {
             if (p) hip_check(hipHostFree(p), "hipHostFree");
             p = nullptr;
+            cap = 0;
             const size_t c = std::max<size_t>(n, 4096);
             hip_check(hipHostMalloc((void**)&p, c * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
             cap = c;
@@ -48,21 +54,26 @@ void d2d(hipStream_t s, double* d, const double* src, size_t n) {
     if (n) hip_check(hipMemcpyAsync(d, src, n * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
 }
 
-// Device scratch of the Krylov drivers, reused across calls: a pool per host thread and device;
-// a call takes free buffers (the smallest that fits, else grows one) and gives them back when it
-// ends, so repeated solves do not allocate.  The pools live for the process (never freed, which
-// keeps the HIP runtime's teardown order out of the picture).
+// Device scratch of the Krylov drivers, reused across calls: one pool per device, shared by all
+// host threads (a mutex guards it).  A call takes free buffers (the smallest that fits, else grows
+// the largest free one) and gives them back when it ends, so repeated solves do not allocate and a
+// thread that exits leaves its buffers to the next caller.  The pool holds at most as many buffers
+// as were in use at once; they stay allocated for the process (never freed, which keeps the HIP
+// runtime's teardown order out of the picture).
 class BufPool {
   public:
     static BufPool& get() {
-        thread_local std::map<int, BufPool*> pools;
+        static std::mutex m;
+        static std::map<int, BufPool*> pools;
         int dev = 0;
         hip_check(hipGetDevice(&dev), "hipGetDevice");
+        std::lock_guard<std::mutex> g(m);
         BufPool*& p = pools[dev];
         if (!p) p = new BufPool();
         return *p;
     }
     DevBuf* take(size_t bytes) {
+        std::lock_guard<std::mutex> g(mu_);
         Slot* fit = nullptr;      // smallest free buffer that holds `bytes`
         Slot* largest = nullptr;  // else the largest free one, grown
         for (auto& s : slots_) {
@@ -76,12 +87,26 @@ class BufPool {
             best = &slots_.back();
         }
         best->used = true;
-        best->buf->ensure(std::max<size_t>(bytes, 8));
+        try {
+            best->buf->ensure(std::max<size_t>(bytes, 8));
+        } catch (...) {
+            best->used = false;
+            throw;
+        }
         return best->buf.get();
     }
     void give(DevBuf* b) {
+        std::lock_guard<std::mutex> g(mu_);
         for (auto& s : slots_)
             if (s.buf.get() == b) s.used = false;
+    }
+    // buffers and bytes held (introspection for tests)
+    void stats(int* nbuf, size_t* bytes) {
+        std::lock_guard<std::mutex> g(mu_);
+        size_t t = 0;
+        for (auto& s : slots_) t += s.buf->bytes;
+        *nbuf = (int)slots_.size();
+        *bytes = t;
     }
 
   private:
@@ -89,6 +114,7 @@ class BufPool {
         std::unique_ptr<DevBuf> buf;
         bool used;
     };
+    std::mutex mu_;
     std::deque<Slot> slots_;  // stable addresses
 };
 

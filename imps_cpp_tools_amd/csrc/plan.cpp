@@ -11,7 +11,7 @@ imps_plan_s::~imps_plan_s() {
 namespace imps {
 
 void ctx_release(imps_context ctx) {
-    if (!ctx || --ctx->refs > 0) return;
+    if (!ctx || ctx->refs.fetch_sub(1) > 1) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);

@@ -1,6 +1,7 @@
 // Plans: a compiled grouped-GEMM Program, its device copy and the operator store.
 #pragma once
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <vector>
@@ -14,7 +15,7 @@ struct imps_context_s {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    int refs = 1;  // the caller's handle + one per live plan; freed at zero
+    std::atomic<int> refs{1};  // the caller's handle + one per live plan; freed at zero (plans may be made and dropped on several threads)
 };
 
 namespace imps {
@@ -30,7 +31,7 @@ struct CtxHold {
     CtxHold() = default;
     CtxHold(const CtxHold&) = delete;
     CtxHold& operator=(const CtxHold&) = delete;
-    void hold(imps_context ctx) { c = ctx; if (c) ++c->refs; }
+    void hold(imps_context ctx) { c = ctx; if (c) c->refs.fetch_add(1); }
     ~CtxHold() { if (c) ctx_release(c); }
 };
 
