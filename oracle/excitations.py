@@ -30,6 +30,12 @@ TMs, MPS products, ApplyOperator, operator TMs) against the separately pinned C+
 explicit geometric series sum_m (fac T)^m x for InvertE_fac; the reduction of InvertE_proj_fac at
 fac = 1 to the C++ oracle's InvertE_proj; Hermiticity of the assembled H_eff(k) and positivity of
 its spectrum above the ground state; the fixed-point eigenvalue 1 of the dense TM spectra.
+tests/test_excitations_pins.py pins apply_heff itself: closed-form one-particle bands at bond
+dimension 1 (XXZ magnon, Hubbard hole and doublon); its quadratic form <X|H_eff(k)|X> against the
+second derivative of the energy density of the perturbed uniform state, computed from dense
+symmetry-free transfer matrices, at K = 0 (projected and not) and K = +-2 for k in {0, pi/2, pi}
+per cell; the reference's trace checks of HeffConstants and the LRoverlaps closed form (C^T, C);
+and it checks that these pins fail when any single term of apply_heff is left out (`omit`).
 """
 from __future__ import annotations
 
@@ -372,8 +378,10 @@ def invert_e_proj_fac(As, Bs, x: BM, L: BM, R: BM, direction: int, fac, tol=1e-1
 
 
 # ---- excitation set-up -----------------------------------------------------------------------
-def heff_constants(AL, AR, L: BM, R: BM, H, tol=1e-14):
-    """HLtot, HRtot (HeffConstants, Excitations_helpers.hpp:260-347); H must have E0 subtracted."""
+def heff_bond_sums(AL, AR, H):
+    """The unit-cell bond-energy sums HL[n], HR[n] of HeffConstants before the environment solves
+    (Excitations_helpers.hpp:281-303).  With E0 subtracted from H, tr(HL[N-1] R) and tr(L HR[0])
+    vanish: the reference's own check that the bond energies average to E0 (:287-291, :299-303)."""
     N = len(AL)
     HL = [op_tml_diag(H, mps_mps(AL[-1], AL[0]))]
     for n in range(1, N):
@@ -381,6 +389,13 @@ def heff_constants(AL, AR, L: BM, R: BM, H, tol=1e-14):
     HR = [op_tmr_diag(H, mps_mps(AR[-1], AR[0]))]
     for n in range(N - 2, -1, -1):
         HR.insert(0, bm_add(op_tmr_diag(H, mps_mps(AR[n], AR[n + 1])), diag_part(tmr(AR[n], AR[n], HR[0]))))
+    return HL, HR
+
+
+def heff_constants(AL, AR, L: BM, R: BM, H, tol=1e-14):
+    """HLtot, HRtot (HeffConstants, Excitations_helpers.hpp:260-347); H must have E0 subtracted."""
+    N = len(AL)
+    HL, HR = heff_bond_sums(AL, AR, H)
     EHL = [None] * N
     EHR = [None] * N
     EHL[N - 1], _ = invert_e_proj_fac(AL, AL, HL[-1], eye(R), R, 0, 1.0, tol)
@@ -484,9 +499,24 @@ def lr_overlaps(AL, AR, K: Key):
 
 
 # ---- ApplyHeff (Excitations_helpers.hpp:20-228) -----------------------------------------------
-def apply_heff(X: List[BM], kfac, AL, AR, NL, LM: Optional[BM], RM: Optional[BM], H, HLtot, HRtot, tol=1e-14):
+HEFF_TERMS = ("ABR_chain", "ABRtot_prop", "HBL_env", "HBL_op_left", "HBL_op_right", "HBL_chain", "HBLtot_prop",
+              "on_HL", "on_HR", "on_HAB", "on_HBA", "left_HBLtot", "left_wrap", "left_inner",
+              "right_HL", "right_HAA", "right_HAB")
+
+
+def apply_heff(X: List[BM], kfac, AL, AR, NL, LM: Optional[BM], RM: Optional[BM], H, HLtot, HRtot, tol=1e-14, omit=()):
     """Effective Hamiltonian of the momentum-k excitation ansatz on the parameters X[n]
-    (B[n] = NL[n] X[n]); returns X_out in the sector structure of X."""
+    (B[n] = NL[n] X[n]); returns X_out in the sector structure of X.
+
+    `omit` (test infrastructure): names from HEFF_TERMS whose contribution is left out, so that a
+    test can check that its pins notice the loss of any single term."""
+    for t in omit:
+        if t not in HEFF_TERMS:
+            raise ValueError(f"unknown ApplyHeff term {t}")
+    keep = lambda t: t not in omit  # noqa: E731
+    zero = lambda bm: {q: (qo, 0.0 * m) for q, (qo, m) in bm.items()}  # noqa: E731
+    use = lambda t, bm: bm if keep(t) else zero(bm)  # noqa: E731
+    usem = lambda t, A: A if keep(t) else [zero(a) for a in A]  # noqa: E731
     N = len(X)
     proj = LM is not None and RM is not None
     ck = np.conj(kfac)
@@ -495,7 +525,7 @@ def apply_heff(X: List[BM], kfac, AL, AR, NL, LM: Optional[BM], RM: Optional[BM]
     # terms where Bin is right of Bout
     ABR = [tmr_unity(Bin[-1], AR[-1])]
     for n in range(N - 1, 0, -1):
-        ABR.insert(0, tmr_unity(mps_add(Bin[n - 1], mps_mat(AL[n - 1], ABR[0])), AR[n - 1]))
+        ABR.insert(0, tmr_unity(mps_add(Bin[n - 1], usem("ABR_chain", mps_mat(AL[n - 1], ABR[0]))), AR[n - 1]))
     if proj:
         EBR, _ = invert_e_proj_fac(AL, AR, ABR[0], LM, RM, 1, kfac, tol)
     else:
@@ -503,14 +533,17 @@ def apply_heff(X: List[BM], kfac, AL, AR, NL, LM: Optional[BM], RM: Optional[BM]
     ABRtot = [bm_scale(EBR, kfac)]
     for n in range(N - 1, 0, -1):
         EBR = tmr(AL[n], AR[n], EBR)
-        ABRtot.insert(0, bm_add(ABR[n], bm_scale(EBR, kfac)))
+        ABRtot.insert(0, bm_add(ABR[n], use("ABRtot_prop", bm_scale(EBR, kfac))))
     ABRtot = ABRtot[-1:] + ABRtot[:-1]  # shift(ABRtot, 1): last element to the front
     # terms where Bin is left of Bout (H left of or on Bin)
-    HBL = [bm_add(tml_unity(mat_mps(HLtot[-1], Bin[0]), AL[0]),
-                  op_tml_gen(H, mps_add(mps_mps(AL[-1], Bin[0]), mps_scale(mps_mps(Bin[-1], AR[0]), ck)), mps_mps(AL[-1], AL[0])))]
+    HBL = [bm_add(use("HBL_env", tml_unity(mat_mps(HLtot[-1], Bin[0]), AL[0])),
+                  op_tml_gen(H, mps_add(usem("HBL_op_left", mps_mps(AL[-1], Bin[0])),
+                                        usem("HBL_op_right", mps_scale(mps_mps(Bin[-1], AR[0]), ck))), mps_mps(AL[-1], AL[0])))]
     for n in range(1, N):
-        HBL.append(bm_add(tml_unity(mps_add(mat_mps(HBL[-1], AR[n]), mat_mps(HLtot[n - 1], Bin[n])), AL[n]),
-                          op_tml_gen(H, mps_add(mps_mps(AL[n - 1], Bin[n]), mps_mps(Bin[n - 1], AR[n])), mps_mps(AL[n - 1], AL[n]))))
+        HBL.append(bm_add(tml_unity(mps_add(usem("HBL_chain", mat_mps(HBL[-1], AR[n])), usem("HBL_env", mat_mps(HLtot[n - 1], Bin[n]))),
+                                    AL[n]),
+                          op_tml_gen(H, mps_add(usem("HBL_op_left", mps_mps(AL[n - 1], Bin[n])),
+                                                usem("HBL_op_right", mps_mps(Bin[n - 1], AR[n]))), mps_mps(AL[n - 1], AL[n]))))
     LMt = bm_t(LM) if proj else None
     RMt = bm_t(RM) if proj else None
     if proj:
@@ -520,42 +553,42 @@ def apply_heff(X: List[BM], kfac, AL, AR, NL, LM: Optional[BM], RM: Optional[BM]
     HBLtot = [bm_scale(EHBL, ck)]
     for n in range(N - 1):
         EHBL = tml(AR[n], AL[n], EHBL)
-        HBLtot.append(bm_add(HBL[n], bm_scale(EHBL, ck)))
+        HBLtot.append(bm_add(HBL[n], use("HBLtot_prop", bm_scale(EHBL, ck))))
     HBLtot = HBLtot[1:] + HBLtot[:1]  # shift(HBLtot, -1): first element to the back
     d = len(AL[0])
     # Bin and Bout on top of each other
-    Bout = [mps_add(mat_mps(HLtot[P(n - 1)], Bin[n]), mps_mat(Bin[n], HRtot[P(n + 1)])) for n in range(N)]
+    Bout = [mps_add(usem("on_HL", mat_mps(HLtot[P(n - 1)], Bin[n])), usem("on_HR", mps_mat(Bin[n], HRtot[P(n + 1)]))) for n in range(N)]
     for n in range(N):
         HAB = apply_operator(mps_mps(AL[P(n - 1)], Bin[n]), H)
         HBA = apply_operator(mps_mps(Bin[n], AR[P(n + 1)]), H)
         for s in range(d):
             for k in range(d):
-                Bout[n][s] = bm_add(Bout[n][s], bm_mul(bm_t(AL[P(n - 1)][k]), HAB[k * d + s]))
-                Bout[n][s] = bm_add(Bout[n][s], bm_mul(HBA[s * d + k], bm_t(AR[P(n + 1)][k])))
+                Bout[n][s] = bm_add(Bout[n][s], use("on_HAB", bm_mul(bm_t(AL[P(n - 1)][k]), HAB[k * d + s])))
+                Bout[n][s] = bm_add(Bout[n][s], use("on_HBA", bm_mul(HBA[s * d + k], bm_t(AR[P(n + 1)][k]))))
     # Bin left of Bout
-    Bout[0] = mps_add(Bout[0], mat_mps(HBLtot[-1], AR[0]))
+    Bout[0] = mps_add(Bout[0], usem("left_HBLtot", mat_mps(HBLtot[-1], AR[0])))
     HBA = apply_operator(mps_mps(Bin[-1], AR[0]), H)
     for s in range(d):
         for k in range(d):
-            Bout[0][s] = bm_add(Bout[0][s], bm_scale(bm_mul(bm_t(AL[-1][k]), HBA[k * d + s]), ck))
+            Bout[0][s] = bm_add(Bout[0][s], use("left_wrap", bm_scale(bm_mul(bm_t(AL[-1][k]), HBA[k * d + s]), ck)))
     for n in range(1, N):
-        Bout[n] = mps_add(Bout[n], mat_mps(HBLtot[n - 1], AR[n]))
+        Bout[n] = mps_add(Bout[n], usem("left_HBLtot", mat_mps(HBLtot[n - 1], AR[n])))
         HBA = apply_operator(mps_mps(Bin[n - 1], AR[n]), H)
         for s in range(d):
             for k in range(d):
-                Bout[n][s] = bm_add(Bout[n][s], bm_mul(bm_t(AL[n - 1][k]), HBA[k * d + s]))
+                Bout[n][s] = bm_add(Bout[n][s], use("left_inner", bm_mul(bm_t(AL[n - 1][k]), HBA[k * d + s])))
     # Bin right of Bout
     for n in range(N):
         Atmp = mps_add(Bin[P(n + 1)], mps_mat(AL[P(n + 1)], ABRtot[P(n + 2)]))
         if n == N - 1:
             Atmp = mps_scale(Atmp, kfac)
-        Bout[n] = mps_add(Bout[n], mat_mps(HLtot[P(n - 1)], mps_mat(AL[n], ABRtot[P(n + 1)])))
+        Bout[n] = mps_add(Bout[n], usem("right_HL", mat_mps(HLtot[P(n - 1)], mps_mat(AL[n], ABRtot[P(n + 1)]))))
         HAA = apply_operator(mps_mps(AL[P(n - 1)], mps_mat(AL[n], ABRtot[P(n + 1)])), H)
         HAB = apply_operator(mps_mps(AL[n], Atmp), H)
         for s in range(d):
             for k in range(d):
-                Bout[n][s] = bm_add(Bout[n][s], bm_mul(bm_t(AL[P(n - 1)][k]), HAA[k * d + s]))
-                Bout[n][s] = bm_add(Bout[n][s], bm_mul(HAB[s * d + k], bm_t(AR[P(n + 1)][k])))
+                Bout[n][s] = bm_add(Bout[n][s], use("right_HAA", bm_mul(bm_t(AL[P(n - 1)][k]), HAA[k * d + s])))
+                Bout[n][s] = bm_add(Bout[n][s], use("right_HAB", bm_mul(HAB[s * d + k], bm_t(AR[P(n + 1)][k]))))
     # X_out = NL^T B_out, kept on the structure of X
     out = []
     for n in range(N):

@@ -15,13 +15,16 @@ from oracle import pyoracle as orc
 
 
 @functools.lru_cache(maxsize=None)
-def state():
+def state(delta: float = -1.5, sizes: tuple = (1, 2, 3, 4, 3, 2, 1), seed: int = 7):
+    """The default is the D = 16 state above; other (delta, sizes, seed) give further XXZ states
+    built the same way (bond-0 charges 2 (i - len(sizes) // 2))."""
     from oracle_vumps import OracleVUMPS
-    keys0 = np.array([[2 * (i - 3)] for i in range(7)], np.int32)
-    sizes0 = np.array([1, 2, 3, 4, 3, 2, 1], np.int32)
-    v = OracleVUMPS(0, 2, [0], keys0, sizes0, [1.0, -1.5, 0.0], seed=7)
+    ns = len(sizes)
+    keys0 = np.array([[2 * (i - ns // 2)] for i in range(ns)], np.int32)
+    sizes0 = np.array(sizes, np.int32)
+    v = OracleVUMPS(0, 2, [0], keys0, sizes0, [1.0, delta, 0.0], seed=seed)
     e, F = 0.0, [1.0]
-    for _ in range(200):
+    for _ in range(400):
         e, F = v.sweep()
         if max(F) < 1e-12:
             break
