@@ -94,6 +94,11 @@ def lib():
             "imps_prod_apply_n": (I, [P, V, V, V, I]),
             "imps_qr_plan": (I, [P, ctypes.POINTER(P), LP, I]),
             "imps_qr_apply": (I, [P, V, V, V]),
+            "imps_heff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I]),
+            "imps_heff_set_state": (I, [P, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), D]),
+            "imps_heff_check": (I, [P, dp, dp, ip]),
+            "imps_heff_apply": (I, [P, dp, V, V, ip, dp]),
+            "imps_heff_eigs": (I, [P, dp, I, D, I, I, dp, V, dp, ip]),
             "imps_model_nsym": (I, [I]),
             "imps_model_d": (I, [I]),
             "imps_model_i2k": (I, [I, I, ip, ip]),
@@ -577,3 +582,78 @@ class QRPlan(Plan):
     def apply(self, A, Q, R):
         _check(lib().imps_qr_apply(self.h, self._p(A, self.alen, "A"), self._p(Q, self.alen, "Q"),
                                    self._p(R, self.out_layout.nelem, "R")))
+
+
+class HeffPlan(Plan):
+    """Excitation effective Hamiltonian H_eff(k) for nmom momenta at once (ApplyHeff with its
+    constants, Excitations_helpers.hpp:20-347): imps_heff_plan / set_state / apply / eigs.
+
+    AL, AR, NL: per-site MPS layouts; C: per-site bond layouts (C[n] right of site n); H: dense
+    d^2 x d^2 two-site Hamiltonian; E0: ground-state energy per site (subtracted from H); K: charge.
+    Excitation vectors are complex128 CUDA tensors of shape (nmom, mtot) in XDims order."""
+    kind = "heff"
+
+    def __init__(self, ctx: Context, AL: Sequence[BlockLayout], AR: Sequence[BlockLayout], NL: Sequence[BlockLayout],
+                 C: Sequence[BlockLayout], H: np.ndarray, E0: float, K, nmom: int = 1):
+        h = ctypes.c_void_p()
+        N = len(AL)
+        arr = [(_CLayout * N)(*[x.c() for x in L]) for L in (AL, AR, NL, C)]
+        self._keep_layouts = (list(AL), list(AR), list(NL), list(C))
+        Hf = np.asfortranarray(H, dtype=np.float64).reshape(-1, order="F").copy()
+        Kq = np.ascontiguousarray(np.atleast_1d(K), np.int32)
+        _check(lib().imps_heff_plan(ctx.h, ctypes.byref(h), N, AL[0].nlist, *arr, _dptr(Hf), float(E0), _iptr(Kq), int(nmom)))
+        super().__init__(h, ctx)
+        self.N, self.nmom = N, int(nmom)
+        self.xdims = self._out_layout(AL[0].nsym)
+        self.mtot = self.xdims.nelem
+
+    def set_state(self, AL, AR, NL, C, inv_tol: float = 1e-14):
+        lays = self._keep_layouts
+
+        def arr(ts, ls, what):
+            if len(ts) != self.N:
+                raise ValueError(f"heff {what}: {len(ts)} tensors for a cell of {self.N}")
+            return (ctypes.c_void_p * self.N)(*[self._p(t, l.nelem, f"{what}[{k}]") for k, (t, l) in enumerate(zip(ts, ls))])
+        _check(lib().imps_heff_set_state(self.h, arr(AL, lays[0], "AL"), arr(AR, lays[1], "AR"), arr(NL, lays[2], "NL"),
+                                         arr(C, lays[3], "C"), float(inv_tol)))
+
+    def check(self):
+        """(tr(HL R), tr(L HR), environment-solver applications) of HeffConstants' check."""
+        a, b, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(lib().imps_heff_check(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(it)))
+        return a.value, b.value, it.value
+
+    def _cp(self, t, n, what):
+        import torch
+        if t.dtype != torch.complex128:
+            raise ValueError(f"heff {what}: complex128 tensors are needed")
+        return _ptr(torch.view_as_real(t), 2 * n, self.device, f"heff {what}")
+
+    def _moms(self, p):
+        pv = np.ascontiguousarray(np.atleast_1d(p), np.float64)
+        if pv.size != self.nmom:
+            raise ValueError(f"heff: {pv.size} momenta for a plan of {self.nmom}")
+        return pv
+
+    def apply(self, p, x, y):
+        """y[t] = H_eff(p[t]) x[t]; returns (batched GMRES steps, worst GMRES relative residual)."""
+        pv = self._moms(p)
+        n = self.nmom * self.mtot
+        st, rr = ctypes.c_int(), ctypes.c_double()
+        _check(lib().imps_heff_apply(self.h, _dptr(pv), self._cp(x, n, "x"), self._cp(y, n, "y"), ctypes.byref(st), ctypes.byref(rr)))
+        return st.value, rr.value
+
+    def eigs(self, p, nev: int = 4, tol: float = 1e-10, maxit: int = 0, krylov: int = 0, vectors: bool = False, check: bool = True):
+        """Lowest nev eigenpairs per momentum: (energies (nmom, nev), vectors (nmom, nev, mtot) or
+        None, residual norms (nmom, nev), batched applications)."""
+        import torch
+        pv = self._moms(p)
+        ev = np.zeros((self.nmom, nev))
+        rs = np.zeros((self.nmom, nev))
+        vec = torch.empty((self.nmom, nev, self.mtot), dtype=torch.complex128, device=f"cuda:{self.device}") if vectors else None
+        na = ctypes.c_int()
+        st = lib().imps_heff_eigs(self.h, _dptr(pv), int(nev), float(tol), int(maxit), int(krylov), _dptr(ev),
+                                  self._cp(vec, self.nmom * nev * self.mtot, "vectors") if vectors else None, _dptr(rs), ctypes.byref(na))
+        if check:
+            _check(st)
+        return ev, vec, rs, na.value

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/imps_capi.h"
+#include "heff.hpp"
 #include "krylov.hpp"
 #include "plan.hpp"
 
@@ -439,6 +440,86 @@ imps_status imps_env_fixed_point(imps_plan P, double* x, double tol, int maxit, 
         if (iters) *iters = it;
         if (delta) *delta = dl;
         if (dl >= tol) throw NotConverged("fixed point: last change " + std::to_string(dl));
+        return IMPS_OK;
+    });
+}
+
+
+// ---- excitations ---------------------------------------------------------------------------------
+imps_status imps_heff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, const imps_layout* AL, const imps_layout* AR,
+                           const imps_layout* NL, const imps_layout* C, const double* H, double E0, const int* K, int nmom) {
+    return guarded([&] {
+        if (!plan || !AL || !AR || !NL || !C || !K) throw InvalidArg("plan, layouts or K is NULL");
+        if (ncell < 1 || ncell > 64) throw InvalidArg("ncell out of range");
+        std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_HEFF, IMPS_FP64));
+        std::vector<Layout> la, lr, ln, lc;
+        for (int k = 0; k < ncell; ++k) {
+            la.push_back(layout_from_c(&AL[k], "AL"));
+            lr.push_back(layout_from_c(&AR[k], "AR"));
+            ln.push_back(layout_from_c(&NL[k], "NL"));
+            lc.push_back(layout_from_c(&C[k], "C"));
+        }
+        if (ctx) hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        P->heff = std::make_shared<HeffEngine>();
+        P->heff->build(ctx, ncell, d, la, lr, ln, lc, H, E0, K, nmom);
+        P->out_layout = P->heff->Xin.lay;
+        *plan = P.release();
+        return IMPS_OK;
+    });
+}
+
+imps_status imps_heff_set_state(imps_plan P, const double* const* AL, const double* const* AR, const double* const* NL,
+                                const double* const* C, double inv_tol) {
+    return guarded([&] {
+        check_kind(P, K_HEFF);
+        if (!AL || !AR || !NL || !C) throw InvalidArg("tensor arrays are NULL");
+        const int N = P->heff->N;
+        P->heff->set_state(std::vector<const double*>(AL, AL + N), std::vector<const double*>(AR, AR + N),
+                           std::vector<const double*>(NL, NL + N), std::vector<const double*>(C, C + N), inv_tol);
+        P->op_set = true;
+        return IMPS_OK;
+    });
+}
+
+imps_status imps_heff_apply(imps_plan P, const double* p, const double* x, double* y, int* gmres_steps, double* gmres_relres) {
+    return guarded([&] {
+        check_kind(P, K_HEFF);
+        P->heff->stats = HeffStats();
+        int steps = 0;
+        double worst = 0;
+        // the engine records the statistics of each geometric sum; report their sum and worst residual
+        P->heff->apply(p, x, y);
+        steps = P->heff->stats_steps_total;
+        worst = P->heff->stats_relres_worst;
+        hip_check(hipStreamSynchronize(P->ctx->stream), "hipStreamSynchronize");
+        if (gmres_steps) *gmres_steps = steps;
+        if (gmres_relres) *gmres_relres = worst;
+        return IMPS_OK;
+    });
+}
+
+imps_status imps_heff_eigs(imps_plan P, const double* p, int nev, double tol, int maxit, int krylov, double* evals, double* evecs,
+                           double* resid, int* napply) {
+    return guarded([&] {
+        check_kind(P, K_HEFF);
+        int conv = 0;
+        const int n = P->heff->eigs(p, nev, tol, maxit, krylov, evals, evecs, resid, &conv);
+        if (napply) *napply = n;
+        if (!conv) {
+            g_err = "heff eigs: not converged within maxit applications";
+            return IMPS_NOT_CONVERGED;
+        }
+        return IMPS_OK;
+    });
+}
+
+imps_status imps_heff_check(imps_plan P, double* trHL_R, double* trL_HR, int* env_iters) {
+    return guarded([&] {
+        check_kind(P, K_HEFF);
+        if (!P->heff->state_set) throw InvalidArg("heff: state not set");
+        if (trHL_R) *trHL_R = P->heff->check_trL;
+        if (trL_HR) *trL_HR = P->heff->check_trR;
+        if (env_iters) *env_iters = P->heff->env_iters;
         return IMPS_OK;
     });
 }

@@ -1,6 +1,8 @@
 // Plan construction helpers and execution of grouped-GEMM programs.
 #include "plan.hpp"
 
+#include "heff.hpp"
+
 #include <cstdlib>
 #include <cstring>
 

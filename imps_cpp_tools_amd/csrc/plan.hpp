@@ -39,9 +39,10 @@ struct CtxHold {
 
 namespace imps {
 
-enum Kind { K_TM = 0, K_HAC = 1, K_HC = 2, K_REDOP = 3, K_ENV = 4, K_PROD = 5, K_QR = 6 };
+enum Kind { K_TM = 0, K_HAC = 1, K_HC = 2, K_REDOP = 3, K_ENV = 4, K_PROD = 5, K_QR = 6, K_HEFF = 7 };
 
-struct EnvSolver;  // krylov.cpp
+struct EnvSolver;   // krylov.cpp
+struct HeffEngine;  // heff.cpp
 
 }  // namespace imps
 
@@ -71,6 +72,7 @@ struct imps_plan_s {
     std::vector<hipEvent_t> tev;
     size_t tev_used = 0;
     std::shared_ptr<imps::EnvSolver> env;  // K_ENV
+    std::shared_ptr<imps::HeffEngine> heff;  // K_HEFF
     imps::QrPlanData qr;                   // K_QR
     int qr_dir = 0;
     size_t elem_bytes() const { return prec == IMPS_FP64 ? 8 : 4; }

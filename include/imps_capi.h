@@ -205,6 +205,50 @@ imps_status imps_prod_apply_n(imps_plan plan, const double* x1, const double* x2
 imps_status imps_qr_plan(imps_context ctx, imps_plan* plan, const imps_layout* A, int dir);
 imps_status imps_qr_apply(imps_plan plan, const double* A, double* Q, double* R);
 
+/* ---- quasiparticle excitations (A21, A14 complex, A17 complex) ----------------------------
+ * The momentum-k effective Hamiltonian of the excitation ansatz B[n] = NL[n] X[n] for a ground
+ * state in mixed gauge, for nmom momenta at once.
+ *
+ * imps_heff_plan: the structure.  AL, AR, NL: ncell MPS layouts each (nlist = d; NL[n] the left
+ * null space of AL[n], Nullspace(AL[n], l), MPSBlockUtilities.hpp:247-340); C: ncell BlockDiagMat
+ * layouts, C[n] on the bond right of site n (C[ncell-1] on bond 0); H: dense d^2 x d^2 column-major
+ * two-site Hamiltonian; E0: the ground-state energy per site, subtracted from H as
+ * Excitations_main.cpp:191 does; K: excitation charge (nsym ints).  The parameter space X is XDims
+ * (Excitations_helpers.hpp:639-678): query it with imps_plan_out_layout (nlist = ncell, entry n =
+ * the blocks of X[n]; mtot = nelem).  Fails with IMPS_INVALID_ARG when XDims is empty ("no
+ * excitations for this quantum number", Excitations_main.cpp:198).
+ * imps_heff_set_state: copies the tensors (device FP64, Vectorize order) and computes the constants
+ * of HeffConstants (Excitations_helpers.hpp:260-347: HLtot, HRtot through two environment solves
+ * at tolerance inv_tol), the reduced operators and the dominant eigenmatrices of the mixed transfer
+ * matrix in closed form, LM = C^T and RM = C on bond 0 (LRoverlaps, :915-946, for AL C = C AR).
+ * imps_heff_check: the reference's check of HeffConstants (:287-303), tr(HL R) and tr(L HR), which
+ * vanish when E0 is the state's energy; and the environment solver's operator applications. */
+imps_status imps_heff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, const imps_layout* AL, const imps_layout* AR,
+                           const imps_layout* NL, const imps_layout* C, const double* H, double E0, const int* K, int nmom);
+imps_status imps_heff_set_state(imps_plan plan, const double* const* AL, const double* const* AR, const double* const* NL,
+                                const double* const* C, double inv_tol);
+imps_status imps_heff_check(imps_plan plan, double* trHL_R, double* trL_HR, int* env_iters);
+/* Replaces the raw callback ApplyHeff(in, out, xdims, mtot, kfac, AL, AR, NL, LM, RM, H, HLtot, HRtot,
+ * InvETol, ...) (Excitations_helpers.hpp:230-257) for nmom momenta at once: y_t = H_eff(p_t) x_t.
+ * p[t]: momentum in units of pi per site; the unit-cell phase is kfac = e^{i p pi ncell}
+ * (Excitations_main.cpp:258-259).  The two geometric sums (InvertE_fac, and InvertE_proj_fac when
+ * K = 0 and |p pi ncell| < 10 inv_tol, Excitations_main.cpp:264-268) are solved for all momenta
+ * by one batched complex GMRES each, to ||r|| < inv_tol ||b|| (IterativeSolvers.hpp:119).
+ * x, y: device arrays of nmom * mtot interleaved complex doubles (std::complex<double>, the
+ * reference's VTX = Complex), momentum-major; y is fully overwritten.  gmres_steps (optional):
+ * batched GMRES operator applications of the call; gmres_relres: the worst final relative residual. */
+imps_status imps_heff_apply(imps_plan plan, const double* p, const double* x, double* y, int* gmres_steps, double* gmres_relres);
+/* Replaces eigs_n(Hfun, mtot, vals, vecs, nev, "SR", tol) (eigs.h:54-55, eigs.cpp:6-168) as called
+ * per momentum by Excitations_main.cpp:270-279, for all nmom momenta at once.  H_eff(k) is
+ * Hermitian: a thick-restart Lanczos with full CGS2 re-orthogonalisation (krylov basis vectors,
+ * default max(20, 2 nev + 1), the reference's ncv), converged when every wanted Ritz pair has
+ * ||H y - theta y|| <= tol max(1, |theta|).  evals: [nmom][nev] ascending; evecs (optional): device
+ * [nmom][nev][mtot] interleaved complex, unit norm; resid (optional, host): [nmom][nev] residual
+ * norms; napply: batched H_eff applications.  IMPS_NOT_CONVERGED when maxit applications do not
+ * suffice (results written). */
+imps_status imps_heff_eigs(imps_plan plan, const double* p, int nev, double tol, int maxit, int krylov, double* evals, double* evecs,
+                           double* resid, int* napply);
+
 /* ---- bookkeeping (A1/A2) -------------------------------------------------------------------
  * model: 0 = XXZ spin-1/2 (one U(1)), 1 = Fermi-Hubbard in (N,M) representation (U(1)xU(1)).
  * imps_model_i2k: GetI2K (Models.cpp:262-295, :699-733); writes d*nsym charges, returns d or -1.
