@@ -94,6 +94,9 @@ def lib():
             "imps_prod_apply_n": (I, [P, V, V, V, I]),
             "imps_qr_plan": (I, [P, ctypes.POINTER(P), LP, I]),
             "imps_qr_apply": (I, [P, V, V, V]),
+            "imps_tm_vjp": (I, [P, V, V, V, V, V, V, V]),
+            "imps_hac_vjp": (I, [P, V, V, V, V, V, V, V, V, V, V, V]),
+            "imps_hc_vjp": (I, [P, V, V, V, V, V, V, V, V, V, V, V]),
             "imps_heff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I]),
             "imps_heff_set_state": (I, [P, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), D]),
             "imps_heff_check": (I, [P, dp, dp, ip]),
@@ -430,6 +433,12 @@ class TMPlan(Plan):
     def apply(self, x, y, nvec: int = 1):
         _check(lib().imps_tm_apply(self.h, self._p(x, nvec * self.xlen, "x"), self._p(y, nvec * self.ylen, "y"), nvec))
 
+    def vjp(self, A, B, x, ybar, xbar=None, Abar=None, Bbar=None):
+        """Adjoint of the apply (imps_tm_vjp, FP64): writes the given outputs."""
+        _check(lib().imps_tm_vjp(self.h, self._p(A, self.alen, "A"), self._p(B, self.blen, "B") if self.mixed else None,
+                                 self._p(x, self.xlen, "x"), self._p(ybar, self.ylen, "ybar"), self._p(xbar, self.xlen, "xbar"),
+                                 self._p(Abar, self.alen, "Abar"), self._p(Bbar, self.blen, "Bbar") if self.mixed else None))
+
 
 class HACPlan(Plan):
     """Two-site H_AC apply (VUMPS_helpers.hpp:38-88)."""
@@ -452,6 +461,13 @@ class HACPlan(Plan):
     def apply(self, x, y, nvec: int = 1):
         _check(lib().imps_hac_apply(self.h, self._p(x, nvec * self.n, "x"), self._p(y, nvec * self.n, "y"), nvec))
 
+    def vjp(self, ops, x, ybar, xbar=None, opbars=(None, None, None, None)):
+        """Adjoint of the apply (imps_hac_vjp, FP64): ops = (EHL, EHR, HL, HR), opbars their gradients."""
+        names = ("EHL", "EHR", "HL", "HR")
+        o = [self._p(t, n, w) for t, n, w in zip(ops, self.oplen, names)]
+        g = [self._p(t, n, w + "bar") for t, n, w in zip(opbars, self.oplen, names)]
+        _check(lib().imps_hac_vjp(self.h, *o, self._p(x, self.n, "x"), self._p(ybar, self.n, "ybar"), self._p(xbar, self.n, "xbar"), *g))
+
 
 class HCPlan(Plan):
     """Bond H_C apply (VUMPS_helpers.hpp:91-133) in reduced form with HL = ReducedOp(H, AL, l)."""
@@ -473,6 +489,13 @@ class HCPlan(Plan):
 
     def apply(self, x, y, nvec: int = 1):
         _check(lib().imps_hc_apply(self.h, self._p(x, nvec * self.n, "x"), self._p(y, nvec * self.n, "y"), nvec))
+
+    def vjp(self, ops, x, ybar, xbar=None, opbars=(None, None, None, None)):
+        """Adjoint of the apply (imps_hc_vjp, FP64): ops = (EHL, EHR, HL, AR), opbars their gradients."""
+        names = ("EHL", "EHR", "HL", "AR")
+        o = [self._p(t, n, w) for t, n, w in zip(ops, self.oplen, names)]
+        g = [self._p(t, n, w + "bar") for t, n, w in zip(opbars, self.oplen, names)]
+        _check(lib().imps_hc_vjp(self.h, *o, self._p(x, self.n, "x"), self._p(ybar, self.n, "ybar"), self._p(xbar, self.n, "xbar"), *g))
 
 
 class RedOpPlan(Plan):

@@ -14,8 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libimps_hip.so")
 
 SOURCES = ["gemm_kernels.hip", "panel_kernels.hip", "blas_kernels.hip", "qr_kernels.hip", "cplx_kernels.hip", "plan.cpp", "krylov.cpp",
-           "heff.cpp", "capi.cpp", "models.cpp"]
-HEADERS = ["engine.hpp", "panel.hpp", "layout.hpp", "blas.hpp", "common.hpp", "plan.hpp", "krylov.hpp", "qr.hpp", "cplx.hpp", "heff.hpp"]
+           "heff.cpp", "adjoint.cpp", "capi.cpp", "models.cpp"]
+HEADERS = ["engine.hpp", "panel.hpp", "layout.hpp", "blas.hpp", "common.hpp", "plan.hpp", "krylov.hpp", "qr.hpp", "cplx.hpp", "heff.hpp", "adjoint.hpp"]
 
 # -packed-fp32-ops: keep the compiler from forming packed FP32 (v_pk_*_f32) instructions.
 HIPCC_FLAGS = [

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/imps_capi.h"
+#include "adjoint.hpp"
 #include "heff.hpp"
 #include "krylov.hpp"
 #include "plan.hpp"
@@ -205,6 +206,9 @@ imps_status imps_tm_plan(imps_context ctx, imps_plan* plan, int dir, const imps_
         P->prog = build_tm(dir, LA, B ? &LB : nullptr, LX, LY);
         P->op_parts = {LA.nelem};
         if (B) P->op_parts.push_back(LB.nelem);
+        P->lays = {LA, B ? LB : LA, LX, LY};
+        P->tm_dir = dir;
+        P->tm_mixed = B != nullptr;
         return finish_plan(plan, P);
     });
 }
@@ -234,6 +238,7 @@ imps_status imps_hac_plan(imps_context ctx, imps_plan* plan, const imps_layout* 
                l3 = layout_from_c(HL, "HL"), l4 = layout_from_c(HR, "HR");
         P->prog = build_hac(la, l1, l2, l3, l4);
         P->op_parts = {l1.nelem, l2.nelem, l3.nelem, l4.nelem};
+        P->lays = {la, l1, l2, l3, l4};
         return finish_plan(plan, P);
     });
 }
@@ -254,6 +259,7 @@ imps_status imps_hc_plan(imps_context ctx, imps_plan* plan, const imps_layout* C
                l3 = layout_from_c(HL, "HL"), l4 = layout_from_c(AR, "AR");
         P->prog = build_hc(lc, l1, l2, l3, l4);
         P->op_parts = {l1.nelem, l2.nelem, l3.nelem, l4.nelem};
+        P->lays = {lc, l1, l2, l3, l4};
         return finish_plan(plan, P);
     });
 }
@@ -520,6 +526,37 @@ imps_status imps_heff_check(imps_plan P, double* trHL_R, double* trL_HR, int* en
         if (trHL_R) *trHL_R = P->heff->check_trL;
         if (trL_HR) *trL_HR = P->heff->check_trR;
         if (env_iters) *env_iters = P->heff->env_iters;
+        return IMPS_OK;
+    });
+}
+
+
+// ---- adjoints ----------------------------------------------------------------------------------
+imps_status imps_tm_vjp(imps_plan P, const double* A, const double* B, const double* x, const double* ybar, double* xbar, double* Abar,
+                        double* Bbar) {
+    return guarded([&] {
+        check_kind(P, K_TM);
+        if (P->tm_mixed && !B) throw InvalidArg("plan was built for a mixed TM: B is required");
+        if (!P->tm_mixed && (B || Bbar)) throw InvalidArg("regular TM: B and Bbar must be NULL (Abar holds both roles)");
+        vjp_run(P, {A, P->tm_mixed ? B : A}, x, ybar, {xbar, Abar, Bbar});
+        return IMPS_OK;
+    });
+}
+
+imps_status imps_hac_vjp(imps_plan P, const double* EHL, const double* EHR, const double* HL, const double* HR, const double* x,
+                         const double* ybar, double* xbar, double* EHLbar, double* EHRbar, double* HLbar, double* HRbar) {
+    return guarded([&] {
+        check_kind(P, K_HAC);
+        vjp_run(P, {EHL, EHR, HL, HR}, x, ybar, {xbar, EHLbar, EHRbar, HLbar, HRbar});
+        return IMPS_OK;
+    });
+}
+
+imps_status imps_hc_vjp(imps_plan P, const double* EHL, const double* EHR, const double* HL, const double* AR, const double* x,
+                        const double* ybar, double* xbar, double* EHLbar, double* EHRbar, double* HLbar, double* ARbar) {
+    return guarded([&] {
+        check_kind(P, K_HC);
+        vjp_run(P, {EHL, EHR, HL, AR}, x, ybar, {xbar, EHLbar, EHRbar, HLbar, ARbar});
         return IMPS_OK;
     });
 }

@@ -7,6 +7,8 @@
 #include <cstring>
 #include <map>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 
 namespace imps {
@@ -131,18 +133,27 @@ void herm_eig(int n, std::vector<cplx> A, std::vector<double>& w, std::vector<cp
     Z.swap(Zs);
 }
 
+// IMPS_VERBOSE=1: progress lines of the excitation solvers on stderr
+bool verbose() {
+    static const bool v = [] {
+        const char* e = std::getenv("IMPS_VERBOSE");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 }  // namespace
 
 // ================================================================================================
 // recording
-long HeffEngine::alloc(long n, bool cplx) {
+long BlockTape::alloc(long n, bool cplx) {
     const long off = arena_elems;
     arena_elems += (cplx ? 2L * nmom : 1L) * std::max<long>(n, 0);
     arena_elems = (arena_elems + 31) / 32 * 32;  // 256-byte aligned objects
     return off;
 }
 
-XObj HeffEngine::make(const Layout& lay, bool cplx) {
+XObj BlockTape::make(const Layout& lay, bool cplx) {
     XObj o;
     o.lay = lay;
     o.cplx = cplx;
@@ -151,7 +162,7 @@ XObj HeffEngine::make(const Layout& lay, bool cplx) {
     return o;
 }
 
-XObj HeffEngine::view_entry(const XObj& x, int e) const {
+XObj BlockTape::view_entry(const XObj& x, int e) const {
     XObj v;
     v.lay = entry_layout(x.lay, e);
     v.off = x.off + entry_start(x.lay, e);
@@ -160,14 +171,14 @@ XObj HeffEngine::view_entry(const XObj& x, int e) const {
     return v;
 }
 
-int HeffEngine::coef(cplx v, int pow) {
+int BlockTape::coef(cplx v, int pow) {
     for (size_t i = 0; i < coefs.size(); ++i)
         if (coefs[i].v == v && coefs[i].pow == pow) return (int)i;
     coefs.push_back(CoefSpec{v, pow});
     return (int)coefs.size() - 1;
 }
 
-XObj HeffEngine::prod(int kind, const XObj& a, const XObj& b) {
+XObj BlockTape::prod(int kind, const XObj& a, const XObj& b) {
     if (a.cplx && b.cplx) throw InvalidArg("heff: product of two complex objects");
     ProdOp op;
     Layout out;
@@ -183,7 +194,7 @@ XObj HeffEngine::prod(int kind, const XObj& a, const XObj& b) {
     return y;
 }
 
-XObj HeffEngine::redop(const XObj& A, int dir) {
+XObj BlockTape::redop(const XObj& A, int dir) {
     ProdOp op;
     Layout out;
     op.prog = build_redop(H0.data(), d, A.lay, dir, out);
@@ -198,7 +209,7 @@ XObj HeffEngine::redop(const XObj& A, int dir) {
     return y;
 }
 
-XObj HeffEngine::comb(const std::vector<std::vector<Src>>& rows, const Layout* fixed, const XObj* dest, bool force_cplx) {
+XObj BlockTape::comb(const std::vector<std::vector<Src>>& rows, const Layout* fixed, const XObj* dest, bool force_cplx) {
     const size_t nl = fixed ? fixed->l.size() : rows.size();
     if (rows.size() != nl) throw InvalidArg("heff: combination rows do not match the output entries");
     bool cp = force_cplx;
@@ -265,7 +276,7 @@ XObj HeffEngine::comb(const std::vector<std::vector<Src>>& rows, const Layout* f
     return y;
 }
 
-XObj HeffEngine::add(const XObj& a, const XObj& b, cplx ca, cplx cb, int powb) {
+XObj BlockTape::add(const XObj& a, const XObj& b, cplx ca, cplx cb, int powb) {
     if (a.lay.l.size() != b.lay.l.size()) throw InvalidArg("heff: sum of objects with different list lengths");
     std::vector<std::vector<Src>> rows(a.lay.l.size());
     const int c1 = coef(ca, 0), c2 = coef(cb, powb);
@@ -273,7 +284,7 @@ XObj HeffEngine::add(const XObj& a, const XObj& b, cplx ca, cplx cb, int powb) {
     return comb(rows);
 }
 
-XObj HeffEngine::scale(const XObj& a, cplx c, int pow) {
+XObj BlockTape::scale(const XObj& a, cplx c, int pow) {
     if (pow == 0 && c == cplx(1.0) && a.cplx) return a;
     std::vector<std::vector<Src>> rows(a.lay.l.size());
     const int k = coef(c, pow);
@@ -281,7 +292,7 @@ XObj HeffEngine::scale(const XObj& a, cplx c, int pow) {
     return comb(rows, nullptr, nullptr, true);
 }
 
-XObj HeffEngine::onto(const XObj& a, const Layout& lay, const XObj* dest) {
+XObj BlockTape::onto(const XObj& a, const Layout& lay, const XObj* dest) {
     if (!dest && same_layout(a.lay, lay)) return a;
     if (a.lay.l.size() != lay.l.size()) throw InvalidArg("heff: restriction onto a layout with another list length");
     std::vector<std::vector<Src>> rows(lay.l.size());
@@ -290,7 +301,7 @@ XObj HeffEngine::onto(const XObj& a, const Layout& lay, const XObj* dest) {
     return comb(rows, &lay, dest, dest ? dest->cplx : false);
 }
 
-XObj HeffEngine::diag_part(const XObj& a) {
+XObj BlockTape::diag_part(const XObj& a) {
     std::vector<std::map<Key, Blk>> ents(a.lay.l.size());
     for (size_t e = 0; e < ents.size(); ++e)
         for (const Blk& b : a.lay.l[e].b)
@@ -298,7 +309,7 @@ XObj HeffEngine::diag_part(const XObj& a) {
     return onto(a, from_maps(nsym, ents));
 }
 
-XObj HeffEngine::apply_operator(const XObj& AA) {
+XObj BlockTape::apply_operator(const XObj& AA) {
     const int D = d * d;
     if ((int)AA.lay.l.size() != D) throw InvalidArg("heff: ApplyOperator needs a two-site object");
     std::vector<std::vector<Src>> rows(D);
@@ -399,6 +410,7 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     if (!H) throw InvalidArg("heff: H is NULL");
     ctx = c;
     N = ncell;
+    ncell_phase = ncell;
     d = dphys;
     nmom = nmomenta;
     nsym = al[0].nsym;
@@ -472,50 +484,7 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     rec = &apply_tape;
     record_apply();
     rec = nullptr;
-    // device tables: programs and combinations
-    {
-        std::vector<char> host;
-        auto put = [&](const void* p, size_t bytes) {
-            size_t pos = (host.size() + 255) & ~size_t(255);
-            host.resize(pos + bytes);
-            if (bytes) std::memcpy(host.data() + pos, p, bytes);
-            return pos;
-        };
-        std::vector<std::vector<std::pair<size_t, size_t>>> ppos(prods.size());
-        for (size_t i = 0; i < prods.size(); ++i)
-            for (auto& st : prods[i].prog.stages) {
-                const size_t tp = put(st.tasks.data(), st.tasks.size() * sizeof(Task));
-                const size_t sp = put(st.segs.data(), st.segs.size() * sizeof(Seg));
-                ppos[i].push_back({tp, sp});
-            }
-        std::vector<std::pair<size_t, size_t>> cpos(combs.size());
-        for (size_t i = 0; i < combs.size(); ++i) {
-            const size_t tp = put(combs[i].tiles.data(), combs[i].tiles.size() * sizeof(CombTile));
-            const size_t sp = put(combs[i].terms.data(), combs[i].terms.size() * sizeof(CombTerm));
-            cpos[i] = {tp, sp};
-        }
-        progs_dev.ensure(std::max<size_t>(host.size(), 256));
-        h2d_sync(stream(), progs_dev.p, host.data(), host.size());
-        char* base = (char*)progs_dev.p;
-        for (size_t i = 0; i < prods.size(); ++i) {
-            prods[i].dst.clear();
-            for (size_t k = 0; k < prods[i].prog.stages.size(); ++k) {
-                DevStage ds;
-                ds.tasks = (const Task*)(base + ppos[i][k].first);
-                ds.segs = (const Seg*)(base + ppos[i][k].second);
-                ds.ntasks = (int)prods[i].prog.stages[k].tasks.size();
-                ds.out_slot = prods[i].prog.stages[k].out_slot;
-                prods[i].dst.push_back(ds);
-            }
-        }
-        for (size_t i = 0; i < combs.size(); ++i) {
-            combs[i].dtiles = (const CombTile*)(base + cpos[i].first);
-            combs[i].dterms = (const CombTerm*)(base + cpos[i].second);
-        }
-    }
-    arena.ensure(sizeof(double) * (size_t)std::max<long>(arena_elems, 1));
-    hip_check(launch_zero(arena.d(), arena_elems, stream()), "zero arena");
-    coef_dev.ensure(sizeof(double) * 2 * coefs.size() * nmom);
+    finalize();
     fac_dev.ensure(sizeof(double) * 2 * nmom * std::max<size_t>(geos.size(), 1));
     gam_dev.ensure(sizeof(double) * 2 * nmom * std::max<size_t>(geos.size(), 1));
     hip_check(hipStreamSynchronize(stream()), "sync");
@@ -644,11 +613,58 @@ void HeffEngine::record_apply() {
 
 // ================================================================================================
 // execution
-void HeffEngine::upload_coefs() {
+void BlockTape::finalize() {
+    // device tables: programs and combinations
+    {
+        std::vector<char> host;
+        auto put = [&](const void* p, size_t bytes) {
+            size_t pos = (host.size() + 255) & ~size_t(255);
+            host.resize(pos + bytes);
+            if (bytes) std::memcpy(host.data() + pos, p, bytes);
+            return pos;
+        };
+        std::vector<std::vector<std::pair<size_t, size_t>>> ppos(prods.size());
+        for (size_t i = 0; i < prods.size(); ++i)
+            for (auto& st : prods[i].prog.stages) {
+                const size_t tp = put(st.tasks.data(), st.tasks.size() * sizeof(Task));
+                const size_t sp = put(st.segs.data(), st.segs.size() * sizeof(Seg));
+                ppos[i].push_back({tp, sp});
+            }
+        std::vector<std::pair<size_t, size_t>> cpos(combs.size());
+        for (size_t i = 0; i < combs.size(); ++i) {
+            const size_t tp = put(combs[i].tiles.data(), combs[i].tiles.size() * sizeof(CombTile));
+            const size_t sp = put(combs[i].terms.data(), combs[i].terms.size() * sizeof(CombTerm));
+            cpos[i] = {tp, sp};
+        }
+        progs_dev.ensure(std::max<size_t>(host.size(), 256));
+        h2d_sync(stream(), progs_dev.p, host.data(), host.size());
+        char* base = (char*)progs_dev.p;
+        for (size_t i = 0; i < prods.size(); ++i) {
+            prods[i].dst.clear();
+            for (size_t k = 0; k < prods[i].prog.stages.size(); ++k) {
+                DevStage ds;
+                ds.tasks = (const Task*)(base + ppos[i][k].first);
+                ds.segs = (const Seg*)(base + ppos[i][k].second);
+                ds.ntasks = (int)prods[i].prog.stages[k].tasks.size();
+                ds.out_slot = prods[i].prog.stages[k].out_slot;
+                prods[i].dst.push_back(ds);
+            }
+        }
+        for (size_t i = 0; i < combs.size(); ++i) {
+            combs[i].dtiles = (const CombTile*)(base + cpos[i].first);
+            combs[i].dterms = (const CombTerm*)(base + cpos[i].second);
+        }
+    }
+    arena.ensure(sizeof(double) * (size_t)std::max<long>(arena_elems, 1));
+    hip_check(launch_zero(arena.d(), arena_elems, stream()), "zero arena");
+    coef_dev.ensure(sizeof(double) * 2 * coefs.size() * nmom);
+}
+
+void BlockTape::upload_coefs() {
     std::vector<double> h(2 * coefs.size() * nmom);
     for (size_t i = 0; i < coefs.size(); ++i)
         for (int t = 0; t < nmom; ++t) {
-            const double pa = (t < (int)p_cur.size() ? p_cur[t] : 0.0) * M_PI * N;
+            const double pa = (t < (int)p_cur.size() ? p_cur[t] : 0.0) * M_PI * ncell_phase;
             const cplx kf(std::cos(pa), std::sin(pa));
             cplx v = coefs[i].v;
             if (coefs[i].pow == 1) v *= kf;
@@ -659,7 +675,7 @@ void HeffEngine::upload_coefs() {
     h2d_sync(stream(), coef_dev.p, h.data(), h.size() * sizeof(double));
 }
 
-void HeffEngine::run_prod(const ProdOp& op) {
+void BlockTape::run_prod(const ProdOp& op) {
     Slots sl;
     sl.base[SLOT_X] = A() + op.a.off;
     sl.stride[SLOT_X] = op.a.cplx ? op.a.ps : 0;
@@ -671,24 +687,28 @@ void HeffEngine::run_prod(const ProdOp& op) {
     for (auto& ds : op.dst) hip_check(launch_stage(IMPS_FP64, ds, sl, A() + op.out.off, op.out.ps, true, nv, stream()), "heff product");
 }
 
-void HeffEngine::run_comb(const CombOp& op) {
+void BlockTape::run_comb(const CombOp& op) {
     hip_check(comb_launch(op.dtiles, (int)op.tiles.size(), op.dterms, coef_dev.d(), A(), op.out_ps, op.out_cplx ? 1 : 0, nmom, stream()),
               "heff combination");
 }
 
-void HeffEngine::run_tape(const std::vector<Op>& tape) {
+void BlockTape::run_tape(const std::vector<Op>& tape) {
     for (const Op& op : tape) {
-        switch (op.type) {
-            case 0: run_prod(prods[op.idx]); break;
-            case 1: run_comb(combs[op.idx]); break;
-            case 2:
-                run_geo(*geos[op.idx]);
-                stats_steps_total += stats.gmres_steps;
-                stats_relres_worst = std::max(stats_relres_worst, stats.gmres_relres);
-                break;
-            case 3: calls[op.idx](); break;
-            default: throw InvalidArg("heff: bad tape entry");
-        }
+        if (op.type == 0) run_prod(prods[op.idx]);
+        else if (op.type == 1) run_comb(combs[op.idx]);
+        else run_other(op);
+    }
+}
+
+void HeffEngine::run_other(const Op& op) {
+    if (op.type == 2) {
+        run_geo(*geos[op.idx]);
+        stats_steps_total += stats.gmres_steps;
+        stats_relres_worst = std::max(stats_relres_worst, stats.gmres_relres);
+    } else if (op.type == 3) {
+        calls[op.idx]();
+    } else {
+        throw InvalidArg("heff: bad tape entry");
     }
 }
 
@@ -1087,6 +1107,20 @@ int HeffEngine::eigs(const double* p, int nev, double tol, int maxit, int m, dou
                 res[(size_t)t * nev + i] = beta[t] * std::abs(Sv[t][(m - 1) + (size_t)i * m]);
                 if (res[(size_t)t * nev + i] > tol * std::max(1.0, std::abs(theta[t][i]))) all_ok = false;
             }
+        }
+        if (verbose()) {
+            double worst = 0;
+            int open = 0;
+            for (int t = 0; t < nmom; ++t) {
+                bool ok = true;
+                for (int i = 0; i < nev; ++i) {
+                    const double r = res[(size_t)t * nev + i] / std::max(1.0, std::abs(theta[t][i]));
+                    worst = std::max(worst, r);
+                    ok = ok && r <= tol;
+                }
+                open += ok ? 0 : 1;
+            }
+            std::fprintf(stderr, "heff eigs: %d applications, worst relative residual %.3e, %d of %d momenta open\n", napply, worst, open, nmom);
         }
         const bool stop = all_ok || napply >= maxit;
         const int nk = stop ? nev : keep;

@@ -52,36 +52,19 @@ struct CoefSpec {
     int pow;
 };
 
-struct HeffEngine {
-    // ---- structure ----
+// A recorded program of block-object operations over one device arena (the base of the excitation
+// engine and of the adjoint plans).  Objects are made while recording; `finalize` uploads the
+// recorded tables and allocates the arena.
+struct BlockTape {
     imps_context ctx = nullptr;
-    int N = 0, d = 0, nsym = 1, nmom = 1;
-    Key K;
-    std::vector<double> H0;  // d^2 x d^2 column-major, E0 subtracted
-    std::vector<Layout> ALl, ARl, NLl, Cl;
-    std::vector<Layout> xdims;
-    long mtot = 0;
-    double inv_tol = 1e-14;
-    int krylov = 40;
-    int gmres_maxit = 4000;
+    int nsym = 1, d = 1, nmom = 1;
+    std::vector<double> H0;  // d^2 x d^2 column-major two-site operator used by apply_operator/redop
 
-    // ---- arena ----
     long arena_elems = 0;
-    DevBuf arena, coef_dev, progs_dev, comb_dev;
+    DevBuf arena, coef_dev, progs_dev;
     std::vector<CoefSpec> coefs;
     double* A() const { return arena.d(); }
 
-    // ---- objects ----
-    std::vector<XObj> AL, AR, NL, C;  // constants, filled by set_state
-    std::vector<XObj> HLtot, HRtot, HLr, HRr;
-    XObj Xin, Xout;                   // contiguous complex objects in XDims order (views per site)
-    std::vector<XObj> Xin_v, Xout_v;
-    XObj LMo, RMo;                    // closed-form dominant eigenmatrices of the mixed TM (bond 0)
-    XObj one;                         // identity on bond 0
-    std::vector<XObj> AAL;            // AL[n-1] AL[n] two-site products
-    XObj checkHL, checkHR, checkL, checkR;  // operands of HeffConstants' trace check
-
-    // ---- tapes ----
     struct ProdOp {
         Program prog;
         std::vector<DevStage> dst;
@@ -95,6 +78,69 @@ struct HeffEngine {
         long out_ps = 0;
         bool out_cplx = false;
     };
+    struct Op {
+        int type;  // 0 prod, 1 comb, >= 2 engine-specific
+        int idx;
+    };
+    std::vector<ProdOp> prods;
+    std::vector<CombOp> combs;
+    std::vector<Op>* rec = nullptr;
+    std::vector<double> p_cur;  // momenta of the current batch (coefficient powers of e^{i p pi N})
+    int ncell_phase = 1;        // N of the phase e^{i p pi N}
+
+    virtual ~BlockTape() = default;
+    long alloc(long nelem, bool cplx);
+    XObj make(const Layout& lay, bool cplx);
+    XObj view_entry(const XObj& x, int e) const;
+    int coef(cplx v, int pow);
+    XObj prod(int kind, const XObj& a, const XObj& b);
+    struct Src {
+        int coef;
+        const XObj* obj;
+        int entry;
+    };
+    // out entry e = sum over rows[e] of coef * entry of obj (sector union in term order; or onto
+    // the fixed layout `fixed` when given, dropping source blocks outside it)
+    XObj comb(const std::vector<std::vector<Src>>& rows, const Layout* fixed = nullptr, const XObj* dest = nullptr, bool force_cplx = false);
+    XObj add(const XObj& a, const XObj& b, cplx ca = 1.0, cplx cb = 1.0, int powb = 0);
+    XObj scale(const XObj& a, cplx c, int pow);
+    XObj onto(const XObj& a, const Layout& lay, const XObj* dest = nullptr);
+    XObj diag_part(const XObj& a);
+    XObj apply_operator(const XObj& AA);
+    XObj tml(const XObj& A, const XObj& B, const XObj& X) { return prod(PROD_TM_LEFT, prod(PROD_MAT_MPS, X, A), B); }
+    XObj tmr(const XObj& A, const XObj& B, const XObj& X) { return prod(PROD_TM_RIGHT, prod(PROD_MPS_MAT, A, X), B); }
+    XObj redop(const XObj& A, int dir);
+    // uploads the recorded programs and combinations, allocates and zeroes the arena
+    void finalize();
+    void upload_coefs();
+    void run_prod(const ProdOp& op);
+    void run_comb(const CombOp& op);
+    void run_tape(const std::vector<Op>& tape);
+    virtual void run_other(const Op&) { throw InvalidArg("tape: unknown operation"); }
+    hipStream_t stream() const { return ctx->stream; }
+};
+
+struct HeffEngine : BlockTape {
+    // ---- structure ----
+    int N = 0;
+    Key K;
+    std::vector<Layout> ALl, ARl, NLl, Cl;
+    std::vector<Layout> xdims;
+    long mtot = 0;
+    double inv_tol = 1e-14;
+    int krylov = 40;
+    int gmres_maxit = 4000;
+
+    // ---- objects ----
+    std::vector<XObj> AL, AR, NL, C;  // constants, filled by set_state
+    std::vector<XObj> HLtot, HRtot, HLr, HRr;
+    XObj Xin, Xout;                   // contiguous complex objects in XDims order (views per site)
+    std::vector<XObj> Xin_v, Xout_v;
+    XObj LMo, RMo;                    // closed-form dominant eigenmatrices of the mixed TM (bond 0)
+    XObj one;                         // identity on bond 0
+    std::vector<XObj> AAL;            // AL[n-1] AL[n] two-site products
+    XObj checkHL, checkHR, checkL, checkR;  // operands of HeffConstants' trace check
+
     struct GeoOp {
         int dir = 0, pow = 1;
         XObj x, y;
@@ -105,21 +151,11 @@ struct HeffEngine {
         XObj w, rout;         // real, on x's sectors: trace weights (rho_in^T) and rho_out
         XObj rin, rout_src;   // the bond-0 objects they are built from (LM or RM)
     };
-    struct Op {
-        int type;  // 0 prod, 1 comb, 2 geo, 3 call
-        int idx;
-    };
-    std::vector<ProdOp> prods;
-    std::vector<CombOp> combs;
     std::vector<std::unique_ptr<GeoOp>> geos;
     std::vector<std::function<void()>> calls;
     std::vector<Op> setup_tape, apply_tape;
-    std::vector<Op>* rec = nullptr;
-    // bond-0 constants built on the host at set_state (identity, trace weights)
-    std::vector<std::function<void(const std::vector<std::vector<double>>&)>> host_fills;
 
     // ---- per-apply state ----
-    std::vector<double> p_cur;   // momenta of the current batch (units of pi per site)
     DevBuf fac_dev, gam_dev;     // per geo op: kfac^pow per momentum (complex), projection flags
     DevBuf kry;                  // batched Krylov scratch of the geometric sums
     DevBuf eig_kry;              // Lanczos basis and scratch of the eigensolver
@@ -145,42 +181,14 @@ struct HeffEngine {
     // y = H_eff x on the engine's planar vectors (Xin / Xout of the arena)
     void apply_planar(const double* xin, double* xout);
 
-    // ---- algebra used while recording ----
-    long alloc(long nelem, bool cplx);
-    XObj make(const Layout& lay, bool cplx);
-    XObj view_entry(const XObj& x, int e) const;
-    int coef(cplx v, int pow);
-    XObj prod(int kind, const XObj& a, const XObj& b);
-    struct Src {
-        int coef;
-        const XObj* obj;
-        int entry;
-    };
-    // out entry e = sum over rows[e] of coef * entry of obj (sector union in term order; or onto
-    // the fixed layout `fixed` when given, dropping source blocks outside it)
-    XObj comb(const std::vector<std::vector<Src>>& rows, const Layout* fixed = nullptr, const XObj* dest = nullptr, bool force_cplx = false);
-    XObj add(const XObj& a, const XObj& b, cplx ca = 1.0, cplx cb = 1.0, int powb = 0);
-    XObj scale(const XObj& a, cplx c, int pow);
-    XObj onto(const XObj& a, const Layout& lay, const XObj* dest = nullptr);
-    XObj diag_part(const XObj& a);
-    XObj apply_operator(const XObj& AA);
-    XObj tml(const XObj& A, const XObj& B, const XObj& X) { return prod(PROD_TM_LEFT, prod(PROD_MAT_MPS, X, A), B); }
-    XObj tmr(const XObj& A, const XObj& B, const XObj& X) { return prod(PROD_TM_RIGHT, prod(PROD_MPS_MAT, A, X), B); }
     XObj redop_left_mixed(const XObj& Abra, const XObj& Bket);
     XObj redop_right_mixed(const XObj& Aket, const XObj& Bbra);
-    XObj redop(const XObj& A, int dir);
     XObj invert_e(const std::vector<XObj>& As, const std::vector<XObj>& Bs, const XObj& x, int dir, int pow, const XObj* L,
                   const XObj* R);
     void record_setup();
     void record_apply();
-
-    // ---- execution ----
-    void run_tape(const std::vector<Op>& tape);
-    void run_prod(const ProdOp& op);
-    void run_comb(const CombOp& op);
     void run_geo(GeoOp& op);
-    void upload_coefs();
-    hipStream_t stream() const { return ctx->stream; }
+    void run_other(const Op& op) override;
 };
 
 }  // namespace imps

@@ -1,6 +1,7 @@
 // Plan construction helpers and execution of grouped-GEMM programs.
 #include "plan.hpp"
 
+#include "adjoint.hpp"
 #include "heff.hpp"
 
 #include <cstdlib>

@@ -43,6 +43,7 @@ enum Kind { K_TM = 0, K_HAC = 1, K_HC = 2, K_REDOP = 3, K_ENV = 4, K_PROD = 5, K
 
 struct EnvSolver;   // krylov.cpp
 struct HeffEngine;  // heff.cpp
+struct VjpTape;     // adjoint.cpp
 
 }  // namespace imps
 
@@ -73,6 +74,12 @@ struct imps_plan_s {
     size_t tev_used = 0;
     std::shared_ptr<imps::EnvSolver> env;  // K_ENV
     std::shared_ptr<imps::HeffEngine> heff;  // K_HEFF
+    // layouts of the operands (TM: A, B, x, y; H_AC: AC, EHL, EHR, HL, HR; H_C: C, EHL, EHR, HL, AR),
+    // the TM direction and whether it is mixed; the adjoint program, compiled on first use
+    std::vector<imps::Layout> lays;
+    int tm_dir = 0;
+    bool tm_mixed = false;
+    std::shared_ptr<imps::VjpTape> vjp;
     imps::QrPlanData qr;                   // K_QR
     int qr_dir = 0;
     size_t elem_bytes() const { return prec == IMPS_FP64 ? 8 : 4; }

@@ -136,6 +136,29 @@ imps_status imps_hc_set_operator(imps_plan plan, const double* EHL, const double
                                  const double* HL, const double* AR);
 imps_status imps_hc_apply(imps_plan plan, const double* x, double* y, int nvec);
 
+/* ---- adjoints (vector-Jacobian products) ------------------------------------------------------
+ * No counterpart in the reference (it has no derivatives); for gradient-based callers of the
+ * operator applications above.  Given the plan of a forward apply y = f(x; operator data) and the
+ * cotangent ybar (layout of y), each entry writes xbar = (df/dx)^T ybar and the gradient of
+ * <ybar, y> with respect to every operator part, restricted onto that part's sector structure.
+ * Arithmetic is FP64 whatever the plan's precision; the forward's intermediates are recomputed,
+ * not stored.  Operator data are passed as in the plan's set call; any output pointer may be NULL
+ * (not written); outputs are fully overwritten.  Formulas:
+ *   TM left  (y = sum_s B_s^T x A_s): xbar = sum_s B_s ybar A_s^T, Abar_s = x^T B_s ybar, Bbar_s = x A_s ybar^T
+ *   TM right (y = sum_s A_s x B_s^T): xbar = sum_s A_s^T ybar B_s, Abar_s = ybar B_s x^T, Bbar_s = ybar^T A_s x
+ *   (regular TM: B and Bbar NULL, Abar the sum of both roles)
+ *   H_AC: xbar[j] = EHL^T ybar[j] + ybar[j] EHR^T + sum_i HL[i][j]^T ybar[i] + ybar[i] HR[i][j]^T,
+ *         EHLbar = sum_i ybar[i] x[i]^T, EHRbar = sum_i x[i]^T ybar[i], HLbar[i][j] = ybar[i] x[j]^T, HRbar[i][j] = x[j]^T ybar[i]
+ *   H_C (reduced form): Cbar = EHL^T ybar + ybar EHR^T + sum_ij HL[i][j]^T ybar AR[i] AR[j]^T, EHLbar = ybar C^T,
+ *         EHRbar = C^T ybar, HLbar[i][j] = ybar AR[i] AR[j]^T C^T, ARbar[j] += C^T HL[i][j]^T ybar AR[i],
+ *         ARbar[i] += ybar^T HL[i][j] C AR[j] */
+imps_status imps_tm_vjp(imps_plan plan, const double* A, const double* B, const double* x, const double* ybar, double* xbar,
+                        double* Abar, double* Bbar);
+imps_status imps_hac_vjp(imps_plan plan, const double* EHL, const double* EHR, const double* HL, const double* HR, const double* x,
+                         const double* ybar, double* xbar, double* EHLbar, double* EHRbar, double* HLbar, double* HRbar);
+imps_status imps_hc_vjp(imps_plan plan, const double* EHL, const double* EHR, const double* HL, const double* AR, const double* x,
+                        const double* ybar, double* xbar, double* EHLbar, double* EHRbar, double* HLbar, double* ARbar);
+
 /* ---- reduced operators (A11) ---------------------------------------------------------------
  * ReducedOp(H, A, dir) (MPSBlockUtilities.hpp:39-74); H is a dense d^2 x d^2 column-major host
  * array whose nonzeros define the sparse two-site operator.  The output layout (d*d lists) is
