@@ -78,6 +78,7 @@ def lib():
             "imps_hac_plan": (I, [P, ctypes.POINTER(P), LP, LP, LP, LP, LP, I]),
             "imps_hac_set_operator": (I, [P, V, V, V, V]),
             "imps_hac_apply": (I, [P, V, V, I]),
+            "imps_hac_apply_static": (I, [P, V, V, I]),
             "imps_hc_plan": (I, [P, ctypes.POINTER(P), LP, LP, LP, LP, LP, I]),
             "imps_hc_set_operator": (I, [P, V, V, V, V]),
             "imps_hc_apply": (I, [P, V, V, I]),
@@ -460,6 +461,10 @@ class HACPlan(Plan):
 
     def apply(self, x, y, nvec: int = 1):
         _check(lib().imps_hac_apply(self.h, self._p(x, nvec * self.n, "x"), self._p(y, nvec * self.n, "y"), nvec))
+
+    def apply_static(self, x, y, nvec: int = 1):
+        """The apply without allocation (imps_hac_apply_static: split copies in the fixed scratch)."""
+        _check(lib().imps_hac_apply_static(self.h, self._p(x, nvec * self.n, "x"), self._p(y, nvec * self.n, "y"), nvec))
 
     def vjp(self, ops, x, ybar, xbar=None, opbars=(None, None, None, None)):
         """Adjoint of the apply (imps_hac_vjp, FP64): ops = (EHL, EHR, HL, HR), opbars their gradients."""

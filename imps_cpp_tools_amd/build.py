@@ -13,9 +13,10 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libimps_hip.so")
 
-SOURCES = ["gemm_kernels.hip", "panel_kernels.hip", "blas_kernels.hip", "qr_kernels.hip", "cplx_kernels.hip", "plan.cpp", "krylov.cpp",
+SOURCES = ["gemm_kernels.hip", "panel_kernels.hip", "blas_kernels.hip", "qr_kernels.hip", "cplx_kernels.hip", "static_scratch.hip", "plan.cpp",
+           "krylov.cpp",
            "heff.cpp", "adjoint.cpp", "capi.cpp", "models.cpp"]
-HEADERS = ["engine.hpp", "panel.hpp", "layout.hpp", "blas.hpp", "common.hpp", "plan.hpp", "krylov.hpp", "qr.hpp", "cplx.hpp", "heff.hpp", "adjoint.hpp"]
+HEADERS = ["engine.hpp", "panel.hpp", "layout.hpp", "blas.hpp", "common.hpp", "plan.hpp", "krylov.hpp", "qr.hpp", "cplx.hpp", "heff.hpp", "adjoint.hpp", "static_scratch.hpp"]
 
 # -packed-fp32-ops: keep the compiler from forming packed FP32 (v_pk_*_f32) instructions.
 HIPCC_FLAGS = [

@@ -12,6 +12,7 @@
 #include "heff.hpp"
 #include "krylov.hpp"
 #include "plan.hpp"
+#include "static_scratch.hpp"
 
 using namespace imps;
 
@@ -84,6 +85,12 @@ imps_status imps_create(imps_context* ctx, int device) {
             throw HipError("hipStreamCreate failed");
         }
         c->stream = c->own;
+        // set-up of the fixed scratch of imps_hac_apply_static on this device (its first call
+        // retries and reports an error if this fails)
+        try {
+            (void)static_scratch_address(device);
+        } catch (const std::exception&) {
+        }
         *ctx = c;
         return IMPS_OK;
     });
@@ -247,6 +254,14 @@ imps_status imps_hac_set_operator(imps_plan P, const double* EHL, const double* 
 }
 imps_status imps_hac_apply(imps_plan P, const double* x, double* y, int nvec) {
     return guarded([&] { check_kind(P, K_HAC); run(P, x, y, nvec); return IMPS_OK; });
+}
+imps_status imps_hac_apply_static(imps_plan P, const double* x, double* y, int nvec) {
+    return guarded([&] {
+        check_kind(P, K_HAC);
+        if (!P->ctx) throw InvalidArg("plan was created without a context");
+        run_static(P, x, y, nvec, static_scratch_address(P->ctx->device), kStaticScratchBytes);
+        return IMPS_OK;
+    });
 }
 
 // ---- H_C ----

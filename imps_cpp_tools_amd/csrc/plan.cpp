@@ -6,6 +6,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <string>
 
 imps_plan_s::~imps_plan_s() {
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
@@ -279,6 +280,40 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
             hip_check(launch_stage(P->prec, ds, sl, y, P->prog.y_elems, true, nvec, s), "grouped gemm");
     }
     time_mark(P, s);
+}
+
+size_t static_bytes(const imps_plan_s* P, int nvec) {
+    return (size_t)split_bytes(P->prec, P->prog.x_elems) * (size_t)std::max(nvec, 0);
+}
+
+void run_static(imps_plan_s* P, const double* x, double* y, int nvec, void* scratch, size_t bytes) {
+    if (!P->ctx) throw InvalidArg("plan was created without a context");
+    if (nvec < 1) throw InvalidArg("nvec must be >= 1");
+    if (!P->use_panels || P->prog.w_elems > 0) throw InvalidArg("static apply: needs a single-stage panel plan (H_AC)");
+    if (!P->op_set) throw InvalidArg("operator data not set");
+    if ((P->prog.x_elems > 0 && !x) || (P->prog.y_elems > 0 && !y)) throw InvalidArg("x or y is NULL");
+    const long xsb = split_bytes(P->prec, P->prog.x_elems);
+    if (static_bytes(P, nvec) > bytes)
+        throw InvalidArg("static apply: " + std::to_string(nvec) + " vectors need " + std::to_string(static_bytes(P, nvec)) +
+                         " bytes of scratch, the fixed scratch holds " + std::to_string(bytes));
+    hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
+    hipStream_t s = P->ctx->stream;
+    const PanelProgram& Q = P->pprog;
+    hip_check(launch_zero((double*)scratch, xsb * nvec / 8, s), "clear scratch");
+    hip_check(launch_split(P->prec, P->d_xblocks, (int)Q.xblocks.size(), x, P->prog.x_elems, scratch, xsb, Q.plx, nvec, s), "split copy");
+    PanelLaunch L;
+    L.pstore = P->d_pstore.p;
+    L.pstore_bytes = Q.nfrag * panel_frag_bytes(P->prec);
+    L.xs = scratch;
+    L.xs_stride = xsb;
+    L.ws = nullptr;
+    L.ws_stride = 0;
+    L.plx = Q.plx;
+    L.plw = Q.plw;
+    L.out = y;
+    L.out_stride = P->prog.y_elems;
+    L.nvec = nvec;
+    launch_panel_schedule(P, panel_schedule_of(P, nvec), L, s);
 }
 
 int panel_schedule_of(const imps_plan_s* P, int nvec) {

@@ -94,6 +94,13 @@ void set_operator(imps_plan_s* P, const std::vector<const double*>& parts);
 // y = op(x) for nvec vectors on the context stream; op_override (FP64 plans) replaces the operator
 // store by a caller buffer (two-operand products)
 void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_override = nullptr);
+// y = op(x) for a panel plan (H_AC) with the split copies of the input in the caller-given device
+// scratch of `bytes` bytes instead of the plan's own buffers: no allocation, no event, no
+// autotuning (the tuned schedule if one exists for nvec, else the default rule); the used part
+// of the scratch is cleared by a fill kernel first (pads must read as zeros)
+void run_static(imps_plan_s* P, const double* x, double* y, int nvec, void* scratch, size_t bytes);
+// bytes of scratch run_static needs for nvec vectors
+size_t static_bytes(const imps_plan_s* P, int nvec);
 // GPU milliseconds of the timed GEMM launches since the last call (waits for them), then resets
 double take_timing(imps_plan_s* P, int* launches);
 // panel schedule for nvec vectors: 0 = K-split 1, 1 = K-split 1 with shared panels, 2 = K-split 2,

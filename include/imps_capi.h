@@ -124,6 +124,19 @@ imps_status imps_hac_plan(imps_context ctx, imps_plan* plan, const imps_layout* 
 imps_status imps_hac_set_operator(imps_plan plan, const double* EHL, const double* EHR,
                                   const double* HL, const double* HR);
 imps_status imps_hac_apply(imps_plan plan, const double* x, double* y, int nvec);
+/* The same apply without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  imps_hac_apply keeps the split copies of its input (the compute-format copy
+ * of x, stored as is and transposed) in buffers the plan allocates on first use; this entry point
+ * keeps them in a fixed 64 MiB scratch array compiled into the library, one per GPU (held until
+ * the process ends), and calls no hipMalloc, hipFree, event or autotuning routine (the autotuned
+ * schedule is used if imps_hac_apply already chose one for nvec, else the default rule).  It
+ * fails with IMPS_INVALID_ARG when nvec split copies exceed the scratch: one vector's split copy takes
+ * P = 64 * ceil((len + 64) / 64) elements per plane, two planes in FP64 (16 P bytes) and FP32 (8 P
+ * bytes), four bf16 planes in BF16X3 (8 P bytes), len the H_AC vector length.  Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (several streams or host threads) race.  The scratch is cleared by a fill
+ * kernel at every call (its pads must read as zeros). */
+imps_status imps_hac_apply_static(imps_plan plan, const double* x, double* y, int nvec);
 
 /* H_C: ApplyHCvec (VUMPS_helpers.hpp:113-133) / ApplyHC (:91-111).  The reference evaluates
  * sum_{nnz H} h*AL[i0]^T*AL[j0]*C*AR[j1]*AR[i1]^T; this plan executes the algebraically identical
