@@ -18,6 +18,9 @@
 #include <cassert>
 #include <cmath>
 #include <complex>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -935,6 +938,120 @@ void* orc_hc(void* EHL, void* EHR, const double* H, int d, void* AL, void* AR, v
         return wrap(((OList*)x)->nsym, {out});
     )
 }
+// ---- H_AC on flat vectors, compiled once and threaded (the CPU baseline) ----
+// The same products as apply_hac (EHL*in + in*EHR and HL[i][j]*in[j] + in[j]*HR[i][j], matched as
+// bmul matches blocks, kept where the output has the block), listed once per output block as
+// segments over flat Vectorize-order arrays; apply runs the output blocks in parallel (OpenMP,
+// dynamic schedule, one output block per task, each summing its segments into a zeroed buffer with
+// an i-k-j loop that the compiler vectorises along the contiguous rows).
+struct HacSeg { long a, b; int K, lda, ldb; int a_op; };  // a_op 0: operator block in the op array, 1: x
+struct HacOut { long off; int M, N; int s0, ns; };
+struct HacPlan { std::vector<HacOut> outs; std::vector<HacSeg> segs; long nx = 0, nop = 0; std::vector<double> op; };
+
+void* orc_hac_plan(void* EHL, void* EHR, void* HL, void* HR, void* x) {
+    ORC_TRY(
+        auto* P = new HacPlan;
+        const MPS in = as_mps(x);
+        const BM& ehl = ((OList*)EHL)->v[0];
+        const BM& ehr = ((OList*)EHR)->v[0];
+        const std::vector<BM>& hl = ((OList*)HL)->v;
+        const std::vector<BM>& hr = ((OList*)HR)->v;
+        const size_t d = in.size();
+        // flat offsets: x in Vectorize order; operators [EHL | EHR | HL | HR]
+        std::vector<std::map<Key, long>> xo(d);
+        long pos = 0;
+        for (size_t s = 0; s < d; ++s) for (auto& b : in[s]) { xo[s][b.first] = pos; pos += (long)b.second.m.v.size(); }
+        P->nx = pos;
+        std::map<const Mat*, long> oo;
+        auto addop = [&](const BM& bm) { for (auto& b : bm) { oo[&b.second.m] = (long)P->op.size(); P->op.insert(P->op.end(), b.second.m.v.begin(), b.second.m.v.end()); } };
+        addop(ehl); addop(ehr);
+        for (auto& b : hl) addop(b);
+        for (auto& b : hr) addop(b);
+        P->nop = (long)P->op.size();
+        for (size_t i = 0; i < d; ++i)
+            for (auto& ob : in[i]) {
+                HacOut o{xo[i][ob.first], ob.second.m.r, ob.second.m.c, (int)P->segs.size(), 0};
+                auto push = [&](long a, int aop, long b, int K, int lda, int ldb) { P->segs.push_back(HacSeg{a, b, K, lda, ldb, aop}); };
+                // EHL*in[i]: EHL block keyed by Qin of the out block
+                if (auto e = ehl.find(ob.first); e != ehl.end() && e->second.qout == ob.first)
+                    push(oo[&e->second.m], 0, xo[i][ob.first], e->second.m.c, e->second.m.r, ob.second.m.r);
+                // in[i]*EHR: EHR block keyed by Qout of the out block
+                if (auto e = ehr.find(ob.second.qout); e != ehr.end())
+                    push(xo[i][ob.first], 1, oo[&e->second.m], ob.second.m.c, ob.second.m.r, e->second.m.r);
+                for (size_t j = 0; j < d; ++j) {
+                    // HL[i][j]*in[j]: HL block q -> p, in[j] block p -> Qout(out)
+                    if (auto h = hl[i * d + j].find(ob.first); h != hl[i * d + j].end())
+                        if (auto xb = in[j].find(h->second.qout); xb != in[j].end() && xb->second.qout == ob.second.qout)
+                            push(oo[&h->second.m], 0, xo[j][xb->first], h->second.m.c, h->second.m.r, xb->second.m.r);
+                    // in[j]*HR[i][j]: in[j] block q -> p, HR block p -> Qout(out)
+                    if (auto xb = in[j].find(ob.first); xb != in[j].end())
+                        if (auto h = hr[i * d + j].find(xb->second.qout); h != hr[i * d + j].end() && h->second.qout == ob.second.qout)
+                            push(xo[j][ob.first], 1, oo[&h->second.m], xb->second.m.c, xb->second.m.r, h->second.m.r);
+                }
+                o.ns = (int)P->segs.size() - o.s0;
+                P->outs.push_back(o);
+            }
+        return P;
+    )
+}
+void orc_hac_plan_free(void* p) { delete (HacPlan*)p; }
+long orc_hac_plan_nx(void* p) { return ((HacPlan*)p)->nx; }
+// y = H_AC x on flat arrays with nthreads OpenMP threads (0: the runtime's default)
+void orc_hac_plan_apply(void* p, const double* x, double* y, int nthreads) {
+    const HacPlan* P = (const HacPlan*)p;
+    const double* op = P->op.data();
+#ifdef _OPENMP
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+    for (long t = 0; t < (long)P->outs.size(); ++t) {
+        const HacOut& o = P->outs[t];
+        double* c = y + o.off;
+        std::fill(c, c + (long)o.M * o.N, 0.0);
+        for (int si = o.s0; si < o.s0 + o.ns; ++si) {
+            const HacSeg& g = P->segs[si];
+            // segment C += A B; operator block either left (a_op 0: A in op, B in x) or right (A in x, B in op)
+            const double* A = g.a_op == 0 ? op + g.a : x + g.a;
+            const double* B = g.a_op == 0 ? x + g.b : op + g.b;
+            int j = 0;
+            for (; j + 4 <= o.N; j += 4) {  // four output columns share each column of A
+                double* c0 = c + (long)j * o.M;
+                double* c1 = c0 + o.M;
+                double* c2 = c1 + o.M;
+                double* c3 = c2 + o.M;
+                const double* b0 = B + (long)j * g.ldb;
+                for (int k = 0; k < g.K; ++k) {
+                    const double* ak = A + (long)k * g.lda;
+                    const double x0 = b0[k], x1 = b0[k + g.ldb], x2 = b0[k + 2L * g.ldb], x3 = b0[k + 3L * g.ldb];
+                    for (int i = 0; i < o.M; ++i) {
+                        const double a = ak[i];
+                        c0[i] += a * x0;
+                        c1[i] += a * x1;
+                        c2[i] += a * x2;
+                        c3[i] += a * x3;
+                    }
+                }
+            }
+            for (; j < o.N; ++j) {
+                double* cj = c + (long)j * o.M;
+                for (int k = 0; k < g.K; ++k) {
+                    const double bkj = B[k + (long)j * g.ldb];
+                    const double* ak = A + (long)k * g.lda;
+                    for (int i = 0; i < o.M; ++i) cj[i] += ak[i] * bkj;
+                }
+            }
+        }
+    }
+    (void)nthreads;
+}
+int orc_max_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 // ---- environments ----
 void* orc_invert_e_proj(int dir, void** Avec, int n, void* x, void* L, void* R, double tol, int maxit, void* y0, int* flag) {
     ORC_TRY(

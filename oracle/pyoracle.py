@@ -51,6 +51,11 @@ def lib():
             "orc_op_tm_diag": (P, [I, dp, I, P]),
             "orc_get_h": (P, [I, pp, I, dp, I]),
             "orc_hac": (P, [P, P, P, P, P]),
+            "orc_hac_plan": (P, [P, P, P, P, P]),
+            "orc_hac_plan_free": (None, [P]),
+            "orc_hac_plan_nx": (ctypes.c_long, [P]),
+            "orc_hac_plan_apply": (None, [P, dp, dp, I]),
+            "orc_max_threads": (I, []),
             "orc_hc": (P, [P, P, dp, I, P, P, P]),
             "orc_invert_e_proj": (P, [I, pp, I, P, P, P, D, I, P, ip]),
             "orc_mps_qr": (P, [P, I, pp]),
@@ -358,3 +363,28 @@ def model_ham(model, params):
     H = np.zeros(d ** 4)
     lib().orc_model_ham(model, _dp(p), _dp(H))
     return H.reshape(d * d, d * d, order="F")
+
+
+class HacPlanCPU:
+    """H_AC on flat Vectorize-order arrays, compiled once, OpenMP-threaded (orc_hac_plan): the CPU
+    baseline of the H_AC workload (the products of apply_hac, VUMPS_helpers.hpp:38-88)."""
+
+    def __init__(self, EHL, EHR, HL, HR, AC):
+        hs = [_h(o) for o in (EHL, EHR, HL, HR, AC)]
+        self.h = lib().orc_hac_plan(*[x.h for x in hs])
+        if not self.h:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self.n = int(lib().orc_hac_plan_nx(self.h))
+
+    def apply(self, x: np.ndarray, y: np.ndarray, threads: int = 0):
+        lib().orc_hac_plan_apply(self.h, _dp(x), _dp(y), int(threads))
+
+    def __del__(self):
+        try:
+            lib().orc_hac_plan_free(self.h)
+        except Exception:
+            pass
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())

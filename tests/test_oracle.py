@@ -401,3 +401,25 @@ def test_oracle_random_state_is_mixed_canonical():
         lhs = orc.mps_mat(v.AL[n], v.C[n]).data
         rhs = orc.mat_mps(v.C[(n - 1) % N], v.AR[n]).data
         assert np.abs(lhs - rhs).max() < 1e-12
+
+
+def test_threaded_flat_hac_equals_oracle_hac():
+    """The compiled, threaded CPU H_AC (the CPU baseline) equals the oracle's H_AC on the Hubbard
+    cell, single- and multi-threaded."""
+    from states import FHUB, Cell, fhub_bond0, random_like
+    k, s = fhub_bond0(5, 10)
+    c = Cell(FHUB, 2, [2, 0], k, s, seed=7)
+    H = orc.model_ham(FHUB, [1.0, 4.0, 0.0, 0.0, 0.0])
+    rng = np.random.default_rng(2)
+    HL = orc.reduced_op(H, 4, c.AL[1], 0)
+    HR = orc.reduced_op(H, 4, c.AR[1], 1)
+    EHL = random_like(c.diag(0), rng, sym=True)
+    EHR = random_like(c.diag(1), rng, sym=True)
+    AC = random_like(c.struct[0], rng)
+    ref = orc.hac(EHL, EHR, HL, HR, AC).data
+    P = orc.HacPlanCPU(EHL, EHR, HL, HR, AC)
+    assert P.n == AC.nelem
+    for threads in (1, 4):
+        y = np.full(AC.nelem, np.nan)
+        P.apply(np.ascontiguousarray(AC.data), y, threads)
+        assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
