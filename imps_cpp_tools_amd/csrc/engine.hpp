@@ -37,15 +37,18 @@ struct DevPStage {
     int out_slot = SLOT_OUT;
 };
 hipError_t launch_panelize(int prec, const PanelDesc* d_panels, int npanels, const double* op, void* store, hipStream_t stream);
-// Split copies (panel.hpp) of the input vectors' raw blocks: grid (nblocks, nvec).
-hipError_t launch_split(int prec, const SplitBlock* d_blocks, int nblocks, const double* x, long x_stride, void* split, long split_stride,
-                        long pl, int nvec, hipStream_t stream);
+// Split copies (panel.hpp) of the input vectors' raw blocks: grid (nunits, nvec).  write_n = false
+// (FP64 only) writes plane T alone: the kernels then read plane N from the input itself.
+hipError_t launch_split(int prec, const SplitUnit* d_units, int nunits, const SplitSpan* d_spans, const double* x, long x_stride,
+                        void* split, long split_stride, long pl, int nvec, hipStream_t stream, bool write_n = true);
 // Buffers of one panel-program apply.  Strides and sizes of split copies are bytes per vector.
 struct PanelLaunch {
     const void* pstore = nullptr;
     long pstore_bytes = 0;
     const void* xs = nullptr;
     long xs_stride = 0;
+    const double* xd = nullptr;  // FP64 input read directly for plane-N chunks (nullptr: from the split copy)
+    long xd_stride = 0;          // elements per vector
     void* ws = nullptr;
     long ws_stride = 0;
     long plx = 0, plw = 0;

@@ -45,7 +45,10 @@ inline long split_plane(long len) { return (len + 64 + 63) / 64 * 64; }
 inline long split_elem_bytes(int prec) { return prec == IMPS_FP64 ? 8 : prec == IMPS_FP32 ? 4 : 2; }
 inline long split_bytes(int prec, long len) { return (prec == IMPS_BF16X3 ? 4 : 2) * split_plane(len) * split_elem_bytes(prec); }
 
-enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_RAW_W = 4 };
+// PCF_RAW_XN: the raw factor is an input-vector block read k-contiguously (plane N).  In FP64 plane N
+// equals the input itself, so such chunks read the caller's vector directly when it is passed to the
+// kernel (no copy of plane N), else the split copy's plane N.
+enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_RAW_W = 4, PCF_RAW_XN = 8 };
 
 struct PChunk {       // 32 bytes
     long raw_off;     // element index, within the (hi) N|T planes, of raw(idx0, k0)
@@ -72,12 +75,20 @@ struct PanelDesc {    // one packed operator block
     double alpha;
 };
 
-struct SplitBlock {   // one tile (at most SPLIT_TILE x SPLIT_TILE) of an input-vector block that is
-    long off;         // read as a raw factor: block offset and shape, tile origin (r0, c0)
+struct SplitSpan {    // one input-vector block read as a raw factor: offset and shape
+    long off;
     int rows, cols;
-    int r0, c0;
 };
-constexpr int SPLIT_TILE = 64;
+// A split unit is one workgroup's share of the split copy: the flat element range [o0, o0 + n) of
+// the vector, covered by the spans b0 .. b0 + nb - 1 (several whole small blocks, or a slice of one
+// large block).  Each thread takes elements o of the range and writes plane T at o (the element of
+// block (r, c) = divmod(o - off, cols) read from N position off + r + c rows) and plane N at o.
+struct SplitUnit {
+    long o0;
+    int n, b0, nb, pad;
+};
+constexpr int SPLIT_UNIT = 1024;      // elements per unit at most
+constexpr int SPLIT_UNIT_SPANS = 64;  // spans per unit at most (staged in LDS)
 
 struct PStage {
     int out_slot = SLOT_OUT;
@@ -89,7 +100,8 @@ struct PanelProgram {
     int prec = IMPS_FP64;
     std::vector<PStage> stages;
     std::vector<PanelDesc> panels;
-    std::vector<SplitBlock> xblocks;  // tiles of the input-vector blocks to split per apply
+    std::vector<SplitSpan> xspans;    // input-vector blocks to split per apply, by offset
+    std::vector<SplitUnit> xunits;    // their packing into workgroup units
     long nfrag = 0;                   // panel fragments in total
     long plx = 0, plw = 0;            // split plane lengths of the input vector and the workspace
 };
@@ -102,6 +114,26 @@ inline bool panel_compatible(const Program& P) {
     return P.op_elems > 0;
 }
 
+// Packs the split spans (by ascending offset; disjoint) into units: consecutive small blocks share
+// a unit, a block longer than SPLIT_UNIT is cut into slices, and the last slice of a block stays
+// open for the small blocks after it.
+inline void pack_split_units(const std::map<long, SplitSpan>& xb, std::vector<SplitSpan>& spans, std::vector<SplitUnit>& units) {
+    for (auto& kv : xb) {
+        const SplitSpan& b = kv.second;
+        const long len = (long)b.rows * b.cols;
+        if (len == 0) continue;
+        const int bi = (int)spans.size();
+        spans.push_back(b);
+        SplitUnit* u = units.empty() ? nullptr : &units.back();
+        if (u && u->nb < SPLIT_UNIT_SPANS && b.off + len - u->o0 <= SPLIT_UNIT) {
+            u->n = (int)(b.off + len - u->o0);
+            ++u->nb;
+            continue;
+        }
+        for (long o = 0; o < len; o += SPLIT_UNIT) units.push_back(SplitUnit{b.off + o, (int)std::min<long>(SPLIT_UNIT, len - o), bi, 1, 0});
+    }
+}
+
 inline PanelProgram build_panel_program(const Program& P, int prec) {
     PanelProgram Q;
     Q.prec = prec;
@@ -112,7 +144,7 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
         throw InvalidArg("panel: vector too long for 32-bit offsets");
     const int KC = panel_kc(prec);
     std::map<std::tuple<long, int, double>, int> index;  // (src, tr, alpha) -> panel
-    std::map<long, SplitBlock> xb;
+    std::map<long, SplitSpan> xb;
     for (auto& st : P.stages) {
         PStage ps;
         ps.out_slot = st.out_slot;
@@ -158,7 +190,7 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                 const bool kmajor = pa ? (rtr == 0) : (rtr != 0);
                 const int idx0 = pa ? t.n0 : t.m0;
                 const long pl = rslot == SLOT_X ? Q.plx : Q.plw;
-                if (rslot == SLOT_X) xb[boff] = SplitBlock{boff, Rb, Cb, 0, 0};
+                if (rslot == SLOT_X) xb[boff] = SplitSpan{boff, Rb, Cb};
                 const int s_idx = kmajor ? Rb : Cb;
                 const long base = kmajor ? boff : pl + boff;
                 const int rg0 = (pa ? t.m0 : t.n0) / 16;
@@ -168,7 +200,8 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                     c.pan = (int)(pd.frag0 + (long)rg0 * pd.nkc + kc);
                     c.pan_sub = rg0 + 1 < pd.nrg ? pd.nkc : -1;  // -1: no second row group
                     c.s_idx = s_idx;
-                    c.flags = (pa ? PCF_PANEL_IS_A : 0) | (rslot == SLOT_W ? PCF_RAW_W : 0);
+                    c.flags = (pa ? PCF_PANEL_IS_A : 0) | (rslot == SLOT_W ? PCF_RAW_W : 0) |
+                              (rslot == SLOT_X && kmajor && prec == IMPS_FP64 ? PCF_RAW_XN : 0);
                     c.krem = s.K - kc * KC;
                     c.pad = 0;
                     chs.push_back(c);
@@ -189,14 +222,7 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
         }
         Q.stages.push_back(std::move(ps));
     }
-    for (auto& kv : xb)
-        for (int r0 = 0; r0 < kv.second.rows; r0 += SPLIT_TILE)
-            for (int c0 = 0; c0 < kv.second.cols; c0 += SPLIT_TILE) {
-                SplitBlock t = kv.second;
-                t.r0 = r0;
-                t.c0 = c0;
-                Q.xblocks.push_back(t);
-            }
+    pack_split_units(xb, Q.xspans, Q.xunits);
     if (Q.nfrag * panel_frag_bytes(prec) >= (1L << 31)) throw InvalidArg("panel store too large for 32-bit offsets");
     return Q;
 }

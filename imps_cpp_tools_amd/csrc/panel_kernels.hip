@@ -199,29 +199,37 @@ __device__ __forceinline__ void put_split_t(void* __restrict__ sp, long pl, long
 // the same order; plane T is written row by row (lanes along c) from an LDS copy, so that both
 // planes are stored in contiguous runs.
 template <int PREC>
-__global__ __launch_bounds__(256) void split_kernel(const SplitBlock* __restrict__ tiles, const double* __restrict__ x, long x_stride,
-                                                    char* __restrict__ split, long split_stride, long pl) {
+__global__ __launch_bounds__(256) void split_kernel(const SplitUnit* __restrict__ units, const SplitSpan* __restrict__ spans,
+                                                    const double* __restrict__ x, long x_stride, char* __restrict__ split,
+                                                    long split_stride, long pl, int write_n) {
     using T = typename std::conditional<PREC == IMPS_FP64, double, float>::type;
-    __shared__ T tile[SPLIT_TILE][SPLIT_TILE + 1];
-    const SplitBlock b = tiles[blockIdx.x];
+    __shared__ long soff[SPLIT_UNIT_SPANS];
+    __shared__ int srows[SPLIT_UNIT_SPANS], scols[SPLIT_UNIT_SPANS];
+    const SplitUnit u = units[blockIdx.x];
     const long v = blockIdx.y;
     const double* xv = x + x_stride * v;
     void* sp = split + split_stride * v;
-    const int tr = min(SPLIT_TILE, b.rows - b.r0), tc = min(SPLIT_TILE, b.cols - b.c0);
-    const int lane = threadIdx.x % SPLIT_TILE, grp = threadIdx.x / SPLIT_TILE;
-    constexpr int NG = 256 / SPLIT_TILE;
-    if (lane < tr) {
-        for (int j = grp; j < tc; j += NG) {
-            const long at = b.off + (b.r0 + lane) + (long)(b.c0 + j) * b.rows;
-            const T val = (T)xv[at];
-            tile[lane][j] = val;
-            put_split_n<PREC>(sp, pl, at, (double)val);
-        }
+    if ((int)threadIdx.x < u.nb) {
+        const SplitSpan b = spans[u.b0 + threadIdx.x];
+        soff[threadIdx.x] = b.off;
+        srows[threadIdx.x] = b.rows;
+        scols[threadIdx.x] = b.cols;
     }
     __syncthreads();
-    if (lane < tc) {
-        for (int i = grp; i < tr; i += NG)
-            put_split_t<PREC>(sp, pl, b.off + (b.c0 + lane) + (long)(b.r0 + i) * b.cols, (double)tile[i][lane]);
+    for (int e = threadIdx.x; e < u.n; e += 256) {
+        const long o = u.o0 + e;
+        int lo = 0, hi = u.nb - 1;  // last span starting at or before o
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (soff[mid] <= o) lo = mid;
+            else hi = mid - 1;
+        }
+        const unsigned rows = (unsigned)srows[lo], cols = (unsigned)scols[lo];
+        const unsigned t = (unsigned)(o - soff[lo]);
+        if (t >= rows * cols) continue;  // between two spans
+        const unsigned r = t / cols, c = t - r * cols;
+        put_split_t<PREC>(sp, pl, o, (double)(T)xv[soff[lo] + r + (long)c * rows]);
+        if (write_n) put_split_n<PREC>(sp, pl, o, (double)(T)xv[o]);
     }
 }
 
@@ -235,7 +243,8 @@ struct Frag {
 // of output-side indices idx0 + [0, 16) and idx0 + [16, 32).
 template <int PREC>
 __device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
-                                           __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx, int li, int lk, int lane) {
+                                           __amdgpu_buffer_rsrc_t rxn, __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx,
+                                           int li, int lk, int lane) {
     using TR = PTraits<PREC>;
     constexpr int RUN = TR::RUN;
     constexpr unsigned FB = PREC == IMPS_FP32 ? 64 * 16 : 64 * 32;  // bytes of one panel fragment
@@ -246,7 +255,7 @@ __device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amd
     F.p[0] = load_frg<PREC>(rp, p0, 1024u);
     F.p[1] = load_frg<PREC>(rp, p1, 1024u);
     const bool w = (c.flags & PCF_RAW_W) != 0;
-    const __amdgpu_buffer_rsrc_t rs = w ? rw : rx;
+    const __amdgpu_buffer_rsrc_t rs = w ? rw : ((c.flags & PCF_RAW_XN) ? rxn : rx);
     const unsigned lo = w ? wlo : xlo;
     const bool kok = RUN * lk < c.krem;
     const unsigned e0 = (unsigned)(c.raw_off + (long)li * c.s_idx + RUN * lk) * EB;
@@ -407,6 +416,8 @@ struct PArgs {
     long pstore_bytes;
     const char* xs;  // split copy of the input vectors
     long xs_stride;  // bytes per vector
+    const char* xd;  // FP64 input vectors read directly by plane-N chunks, or nullptr
+    long xd_stride;  // bytes per vector
     const char* ws;  // split copy of the workspace
     long ws_stride;
     long plx, plw;   // plane lengths (elements)
@@ -439,6 +450,7 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
 
     if (active && part < t.nch) {
         const __amdgpu_buffer_rsrc_t rx = rsrc(A.xs + A.xs_stride * v, A.xs_stride);
+        const __amdgpu_buffer_rsrc_t rxn = A.xd ? rsrc(A.xd + A.xd_stride * v, A.xd_stride) : rx;
         const __amdgpu_buffer_rsrc_t rw = rsrc(A.ws + A.ws_stride * v, A.ws_stride);
         const __amdgpu_buffer_rsrc_t rp = rsrc(A.pstore, A.pstore_bytes);
         const unsigned xlo = (unsigned)(2 * A.plx) * EB, wlo = (unsigned)(2 * A.plw) * EB;  // lo planes (BF16X3)
@@ -448,18 +460,18 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
             int c = t.ch0 + part;
             Frag<PREC> F0, F1;
             PChunk d0 = A.chunks[c];
-            load_chunk<PREC>(F0, d0, rp, rx, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+            load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
             while (true) {
                 // prefetch the next chunk (the current one again at the end: same count of loads)
                 const int c1 = c + S < cend ? c + S : c;
                 const PChunk d1 = A.chunks[c1];
-                load_chunk<PREC>(F1, d1, rp, rx, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                load_chunk<PREC>(F1, d1, rp, rx, rxn, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
                 compute_chunk_n<PREC>(acc, F0, (d0.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c1;
                 const int c2 = c + S < cend ? c + S : c;
                 d0 = A.chunks[c2];
-                load_chunk<PREC>(F0, d0, rp, rx, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
                 compute_chunk_n<PREC>(acc, F1, (d1.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c2;
@@ -529,6 +541,7 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_shared_kernel(const P
 
     const int vs = active ? v : 0;
     const __amdgpu_buffer_rsrc_t rx = rsrc(A.xs + A.xs_stride * vs, A.xs_stride);
+    const __amdgpu_buffer_rsrc_t rxn = A.xd ? rsrc(A.xd + A.xd_stride * vs, A.xd_stride) : rx;
     const __amdgpu_buffer_rsrc_t rw = rsrc(A.ws + A.ws_stride * vs, A.ws_stride);
     const __amdgpu_buffer_rsrc_t rp = rsrc(A.pstore, A.pstore_bytes);
     const unsigned xlo = (unsigned)(2 * A.plx) * EB, wlo = (unsigned)(2 * A.plw) * EB;  // lo planes (BF16X3)
@@ -544,7 +557,7 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_shared_kernel(const P
     // raw fragments of output-side indices idx0 + [0, 16) and idx0 + [16, 32) (zeros when inactive)
     auto load_raw = [&](Frag<PREC>& F, const PChunk& c) {
         const bool w = (c.flags & PCF_RAW_W) != 0;
-        const __amdgpu_buffer_rsrc_t rs = w ? rw : rx;
+        const __amdgpu_buffer_rsrc_t rs = w ? rw : ((c.flags & PCF_RAW_XN) ? rxn : rx);
         const unsigned lo = w ? wlo : xlo;
         const int nidx = active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0;
         const bool kok = RUN * lk < c.krem;
@@ -675,20 +688,25 @@ hipError_t launch_panelize(int prec, const PanelDesc* d_panels, int npanels, con
     return hipGetLastError();
 }
 
-hipError_t launch_split(int prec, const SplitBlock* d_blocks, int nblocks, const double* x, long x_stride, void* split, long split_stride,
-                        long pl, int nvec, hipStream_t stream) {
-    if (nblocks == 0 || nvec == 0) return hipSuccess;
+hipError_t launch_split(int prec, const SplitUnit* d_units, int nunits, const SplitSpan* d_spans, const double* x, long x_stride,
+                        void* split, long split_stride, long pl, int nvec, hipStream_t stream, bool write_n) {
+    if (nunits == 0 || nvec == 0) return hipSuccess;
     if (nvec > 65535) return hipErrorInvalidValue;
-    const dim3 grid(nblocks, nvec);
+    if (!write_n && prec != IMPS_FP64) return hipErrorInvalidValue;
+    const dim3 grid(nunits, nvec);
+    const int wn = write_n ? 1 : 0;
     switch (prec) {
         case IMPS_FP64:
-            hipLaunchKernelGGL(split_kernel<IMPS_FP64>, grid, dim3(256), 0, stream, d_blocks, x, x_stride, (char*)split, split_stride, pl);
+            hipLaunchKernelGGL(split_kernel<IMPS_FP64>, grid, dim3(256), 0, stream, d_units, d_spans, x, x_stride, (char*)split, split_stride,
+                               pl, wn);
             break;
         case IMPS_FP32:
-            hipLaunchKernelGGL(split_kernel<IMPS_FP32>, grid, dim3(256), 0, stream, d_blocks, x, x_stride, (char*)split, split_stride, pl);
+            hipLaunchKernelGGL(split_kernel<IMPS_FP32>, grid, dim3(256), 0, stream, d_units, d_spans, x, x_stride, (char*)split, split_stride,
+                               pl, wn);
             break;
         case IMPS_BF16X3:
-            hipLaunchKernelGGL(split_kernel<IMPS_BF16X3>, grid, dim3(256), 0, stream, d_blocks, x, x_stride, (char*)split, split_stride, pl);
+            hipLaunchKernelGGL(split_kernel<IMPS_BF16X3>, grid, dim3(256), 0, stream, d_units, d_spans, x, x_stride, (char*)split, split_stride,
+                               pl, wn);
             break;
         default: return hipErrorInvalidValue;
     }
@@ -704,6 +722,8 @@ hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, bool sh
     a.pstore_bytes = L.pstore_bytes;
     a.xs = (const char*)L.xs;
     a.xs_stride = L.xs_stride;
+    a.xd = (const char*)L.xd;
+    a.xd_stride = L.xd_stride * (long)sizeof(double);
     a.ws = (const char*)L.ws;
     a.ws_stride = L.ws_stride;
     a.plx = L.plx;

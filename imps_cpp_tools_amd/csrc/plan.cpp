@@ -138,8 +138,10 @@ void upload_panel_program(imps_plan_s* P) {
     }
     const size_t ppos = put(host, pos, P->pprog.panels);
     pos = ppos + P->pprog.panels.size() * sizeof(PanelDesc);
-    const size_t xpos = put(host, pos, P->pprog.xblocks);
-    pos = xpos + P->pprog.xblocks.size() * sizeof(SplitBlock);
+    const size_t xpos = put(host, pos, P->pprog.xspans);
+    pos = xpos + P->pprog.xspans.size() * sizeof(SplitSpan);
+    const size_t upos = put(host, pos, P->pprog.xunits);
+    pos = upos + P->pprog.xunits.size() * sizeof(SplitUnit);
     P->d_pprog.ensure(pos + 256);
     if (pos) hip_check(hipMemcpyAsync(P->d_pprog.p, host.data(), pos, hipMemcpyHostToDevice, P->ctx->stream), "upload panel program");
     hip_check(hipStreamSynchronize(P->ctx->stream), "upload panel program");
@@ -154,13 +156,21 @@ void upload_panel_program(imps_plan_s* P) {
         P->dpstages.push_back(ds);
     }
     P->d_panels = (const PanelDesc*)(base + ppos);
-    P->d_xblocks = (const SplitBlock*)(base + xpos);
+    P->d_xspans = (const SplitSpan*)(base + xpos);
+    P->d_xunits = (const SplitUnit*)(base + upos);
     P->d_pstore.ensure((size_t)std::max<long>(P->pprog.nfrag, 1) * panel_frag_bytes(P->prec));
     P->d_op.ensure(std::max<long>(P->prog.op_elems, 1) * sizeof(double));  // FP64 staging of the operator pieces
     P->xs_ready = P->ws_ready = 0;
 }
 
 }  // namespace
+
+// true when the element ranges [a, a + na) and [b, b + nb) share memory
+static bool overlaps(const double* a, long na, const double* b, long nb) {
+    const uintptr_t a0 = (uintptr_t)a, a1 = a0 + sizeof(double) * (size_t)na;
+    const uintptr_t b0 = (uintptr_t)b, b1 = b0 + sizeof(double) * (size_t)nb;
+    return a0 < b1 && b0 < a1;
+}
 
 void upload_program(imps_plan_s* P) {
     if (!P->ctx) return;
@@ -247,12 +257,18 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
             hip_check(launch_zero((double*)P->d_ws.p, wsb * nvec / 8, s), "zero workspace");
             P->ws_ready = nvec;
         }
-        hip_check(launch_split(P->prec, P->d_xblocks, (int)Q.xblocks.size(), x, P->prog.x_elems, P->d_xs.p, xsb, Q.plx, nvec, s), "split copy");
+        // FP64: plane-N chunks read x itself unless y overlaps it (in-place applies keep the copy)
+        const bool direct = P->prec == IMPS_FP64 && !overlaps(x, P->prog.x_elems * nvec, y, P->prog.y_elems * nvec);
+        hip_check(launch_split(P->prec, P->d_xunits, (int)Q.xunits.size(), P->d_xspans, x, P->prog.x_elems, P->d_xs.p, xsb, Q.plx, nvec, s,
+                               !direct),
+                  "split copy");
         PanelLaunch L;
         L.pstore = P->d_pstore.p;
         L.pstore_bytes = Q.nfrag * panel_frag_bytes(P->prec);
         L.xs = P->d_xs.p;
         L.xs_stride = xsb;
+        L.xd = direct ? x : nullptr;
+        L.xd_stride = P->prog.x_elems;
         L.ws = P->d_ws.p;
         L.ws_stride = wsb;
         L.plx = Q.plx;
@@ -300,12 +316,17 @@ void run_static(imps_plan_s* P, const double* x, double* y, int nvec, void* scra
     hipStream_t s = P->ctx->stream;
     const PanelProgram& Q = P->pprog;
     hip_check(launch_zero((double*)scratch, xsb * nvec / 8, s), "clear scratch");
-    hip_check(launch_split(P->prec, P->d_xblocks, (int)Q.xblocks.size(), x, P->prog.x_elems, scratch, xsb, Q.plx, nvec, s), "split copy");
+    const bool direct = P->prec == IMPS_FP64 && !overlaps(x, P->prog.x_elems * nvec, y, P->prog.y_elems * nvec);
+    hip_check(launch_split(P->prec, P->d_xunits, (int)Q.xunits.size(), P->d_xspans, x, P->prog.x_elems, scratch, xsb, Q.plx, nvec, s,
+                           !direct),
+              "split copy");
     PanelLaunch L;
     L.pstore = P->d_pstore.p;
     L.pstore_bytes = Q.nfrag * panel_frag_bytes(P->prec);
     L.xs = scratch;
     L.xs_stride = xsb;
+    L.xd = direct ? x : nullptr;
+    L.xd_stride = P->prog.x_elems;
     L.ws = nullptr;
     L.ws_stride = 0;
     L.plx = Q.plx;

@@ -63,7 +63,8 @@ struct imps_plan_s {
     imps::PanelProgram pprog;
     imps::DevBuf d_pprog, d_pstore, d_xs, d_ws;  // program tables, panels, split copies of input and workspace
     const imps::PanelDesc* d_panels = nullptr;
-    const imps::SplitBlock* d_xblocks = nullptr;
+    const imps::SplitSpan* d_xspans = nullptr;
+    const imps::SplitUnit* d_xunits = nullptr;
     std::vector<imps::DevPStage> dpstages;
     long xs_ready = 0, ws_ready = 0;  // vectors of the split buffers already zero-filled
     std::map<int, int> sched;         // autotuned panel schedule per vector count (see panel_schedule)
