@@ -424,6 +424,7 @@ imps_status imps_qr_plan(imps_context ctx, imps_plan* plan, const imps_layout* A
             P->d_prog.ensure(b1 + b2 + 64);
             if (b1) hip_check(hipMemcpy(P->d_prog.p, P->qr.secs.data(), b1, hipMemcpyHostToDevice), "upload");
             if (b2) hip_check(hipMemcpy((char*)P->d_prog.p + ((b1 + 15) & ~size_t(15)), P->qr.parts.data(), b2, hipMemcpyHostToDevice), "upload");
+            if (P->qr.gwork > 0) P->d_w.ensure((size_t)P->qr.gwork * sizeof(double));  // work areas of the sectors beyond LDS
         }
         *plan = P.release();
         return IMPS_OK;
@@ -438,7 +439,8 @@ imps_status imps_qr_apply(imps_plan P, const double* A, double* Q, double* R) {
         const size_t b1 = P->qr.secs.size() * sizeof(QrSector);
         const QrSector* secs = (const QrSector*)P->d_prog.p;
         const QrPart* parts = (const QrPart*)((char*)P->d_prog.p + ((b1 + 15) & ~size_t(15)));
-        hip_check(launch_sector_qr(secs, (int)P->qr.secs.size(), parts, A, Q, R, P->qr_dir == IMPS_RIGHT, P->qr.lds, P->ctx->stream),
+        hip_check(launch_sector_qr(secs, P->qr.nlds, (int)P->qr.secs.size() - P->qr.nlds, parts, A, Q, R, P->qr_dir == IMPS_RIGHT, P->qr.lds,
+                                   P->qr.gwork > 0 ? (double*)P->d_w.p : nullptr, P->ctx->stream),
                   "sector qr");
         return IMPS_OK;
     });

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <vector>
 
@@ -24,18 +25,28 @@ struct QrSector {
     int m, n;          // factorised matrix: m x n (left: stacked rows x sector size; right: transposed)
     int part0, nparts;
     long rdst;         // element offset of the sector's R block
+    long woff;         // element offset of the sector's work area in device memory (-1: factorised in LDS)
 };
+
+// Sectors whose work area (matrix, tau, per-wave column scratch) fits kQrLdsMax are factorised in
+// LDS by 4-wave workgroups; larger ones in a device work buffer by 16-wave workgroups, so that no
+// sector size is out of reach.  secs lists the LDS sectors first (nlds of them).
+constexpr size_t kQrLdsMax = 160 * 1024;
+constexpr int kQrWavesLds = 4, kQrWavesGlobal = 16;
 
 struct QrPlanData {
     std::vector<QrSector> secs;
     std::vector<QrPart> parts;
     Layout R;          // BlockDiagMat layout of R
-    size_t lds = 0;    // dynamic LDS bytes (largest sector)
+    size_t lds = 0;    // dynamic LDS bytes (largest LDS sector)
+    int nlds = 0;      // sectors factorised in LDS (the first nlds of secs)
+    long gwork = 0;    // doubles of the device work buffer of the other sectors
 };
 
-size_t sector_qr_lds_bytes(int m, int n);
-hipError_t launch_sector_qr(const QrSector* secs, int nsec, const QrPart* parts, const double* A, double* Q, double* R, bool right,
-                            size_t lds_bytes, hipStream_t s);
+// doubles of one sector's work area: m x n matrix, min(m, n) taus, `waves` columns of m
+inline long sector_qr_work(int m, int n, int waves) { return (long)m * n + std::min(m, n) + (long)waves * m; }
+hipError_t launch_sector_qr(const QrSector* secs, int nlds, int nglobal, const QrPart* parts, const double* A, double* Q, double* R,
+                            bool right, size_t lds_bytes, double* gwork, hipStream_t s);
 
 // Sector regrouping of GetSingleSecData (MPSBlockMat.hpp:636-719): sectors keyed by the right
 // charge (left form) or left charge (right form), parts in ascending physical index.
@@ -68,11 +79,19 @@ inline QrPlanData build_qr(const Layout& A, int dir) {
         Blk rb{kv.first, kv.first, sc.n, sc.n, rpos};
         rpos += rb.size();
         P.R.l[0].b.push_back(rb);
+        const size_t lds = sizeof(double) * (size_t)sector_qr_work(sc.m, sc.n, kQrWavesLds);
+        if (lds <= kQrLdsMax) {
+            sc.woff = -1;
+            P.lds = std::max(P.lds, lds);
+        } else {
+            sc.woff = P.gwork;
+            P.gwork += (sector_qr_work(sc.m, sc.n, kQrWavesGlobal) + 31) / 32 * 32;  // 256-byte aligned areas
+        }
         P.secs.push_back(sc);
-        P.lds = std::max(P.lds, sector_qr_lds_bytes(sc.m, sc.n));
     }
     P.R.nelem = rpos;
-    if (P.lds > 160 * 1024) throw InvalidArg("qr: sector matrix too large for LDS");
+    std::stable_partition(P.secs.begin(), P.secs.end(), [](const QrSector& s) { return s.woff < 0; });
+    P.nlds = (int)std::count_if(P.secs.begin(), P.secs.end(), [](const QrSector& s) { return s.woff < 0; });
     return P;
 }
 

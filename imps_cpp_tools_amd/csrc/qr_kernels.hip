@@ -1,9 +1,11 @@
 // Batched per-sector QR with non-negative R diagonal (qr_pos, helpers.hpp:181-195) of the dense
 // regrouped sector matrices of an MPS tensor (GetDenseMat, MPSBlockMat.hpp:468-526), FP64.
 //
-// One workgroup (4 waves) per charge sector.  The sector matrix (rows = sum of the parts' row
-// counts, cols = the sector size) is gathered from the MPS buffer into LDS and factorised by
-// Householder reflections (LAPACK dlarfg convention, v(0) = 1).  Per reflector, the waves split the
+// One workgroup per charge sector.  The sector matrix (rows = sum of the parts' row counts, cols =
+// the sector size) is gathered from the MPS buffer into LDS (4 waves), or, when it does not fit,
+// into a device work area of its own (16 waves; the workgroup barriers order its global accesses
+// as they do the LDS ones), and factorised by Householder reflections (LAPACK dlarfg convention,
+// v(0) = 1).  Per reflector, the waves split the
 // trailing columns between them; each column's dot product is a fixed-order shuffle reduction
 // within one wave, so the result is deterministic and needs one workgroup barrier per reflector.
 // Q (economy, rows x min(rows, cols)) is then formed column by column (columns split across waves)
@@ -15,21 +17,20 @@
 
 namespace imps {
 
-constexpr int kQrWaves = 4;
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
 
+template <int kQrWaves, bool GLOBAL>
 __global__ __launch_bounds__(64 * kQrWaves) void sector_qr_kernel(const QrSector* __restrict__ secs, const QrPart* __restrict__ parts,
                                                                  const double* __restrict__ A, double* __restrict__ Q,
-                                                                 double* __restrict__ R, int right) {
+                                                                 double* __restrict__ R, int right, double* __restrict__ gwork) {
     extern __shared__ double lds[];
     const QrSector sc = secs[blockIdx.x];
     const int m = sc.m, n = sc.n, k = min(m, n);
-    double* D = lds;                            // m x n, column-major
+    double* D = GLOBAL ? gwork + sc.woff : lds;  // m x n, column-major
     double* taus = D + (long)m * n;             // k
     double* vs = taus + k;                      // kQrWaves x m column scratch
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -107,18 +108,24 @@ __global__ __launch_bounds__(64 * kQrWaves) void sector_qr_kernel(const QrSector
     }
 }
 
-size_t sector_qr_lds_bytes(int m, int n) {
-    return sizeof(double) * ((size_t)m * n + (size_t)std::min(m, n) + (size_t)kQrWaves * m);
-}
-
-hipError_t launch_sector_qr(const QrSector* secs, int nsec, const QrPart* parts, const double* A, double* Q, double* R, bool right,
-                            size_t lds_bytes, hipStream_t s) {
-    if (nsec == 0) return hipSuccess;
-    if (lds_bytes > 64 * 1024) {  // beyond the default dynamic-LDS limit of a launch
-        const hipError_t e = hipFuncSetAttribute((const void*)sector_qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+hipError_t launch_sector_qr(const QrSector* secs, int nlds, int nglobal, const QrPart* parts, const double* A, double* Q, double* R,
+                            bool right, size_t lds_bytes, double* gwork, hipStream_t s) {
+    if (nlds > 0) {
+        if (lds_bytes > 64 * 1024) {  // beyond the default dynamic-LDS limit of a launch
+            const hipError_t e = hipFuncSetAttribute((const void*)sector_qr_kernel<kQrWavesLds, false>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL((sector_qr_kernel<kQrWavesLds, false>), dim3(nlds), dim3(64 * kQrWavesLds), lds_bytes, s, secs, parts, A, Q, R,
+                           right ? 1 : 0, gwork);
+        const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(sector_qr_kernel, dim3(nsec), dim3(64 * kQrWaves), lds_bytes, s, secs, parts, A, Q, R, right ? 1 : 0);
+    if (nglobal > 0) {
+        if (!gwork) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((sector_qr_kernel<kQrWavesGlobal, true>), dim3(nglobal), dim3(64 * kQrWavesGlobal), 0, s, secs + nlds, parts, A,
+                           Q, R, right ? 1 : 0, gwork);
+    }
     return hipGetLastError();
 }
 

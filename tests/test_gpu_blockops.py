@@ -42,6 +42,10 @@ def cell(name):
         elif name == "fhub":
             k, s = fhub_bond0(4, 8)
             CELLS[name] = Cell(FHUB, 2, [2, 0], k, s, seed=302)
+        elif name == "hubbig":  # sectors up to 100 stacking ~250 rows: beyond LDS, factorised in device memory
+            from imps_cpp_tools_amd.workloads import fhub_profile
+            k, s = fhub_profile(3, 100, 1, 512)
+            CELLS[name] = Cell(FHUB, 2, [2, 0], k, s, seed=304)
         else:  # full Hubbard D=1024 profile: largest sectors stack 4 x 60 rows
             from imps_cpp_tools_amd.workloads import HUBBARD_1024, fhub_profile
             k, s = fhub_profile(HUBBARD_1024["nside"], HUBBARD_1024["peak"], 1, HUBBARD_1024["total"])
@@ -49,7 +53,7 @@ def cell(name):
     return CELLS[name]
 
 
-@pytest.mark.parametrize("name", ["xxz", "fhub", "hub1024"])
+@pytest.mark.parametrize("name", ["xxz", "fhub", "hub1024", "hubbig"])
 @pytest.mark.parametrize("direction", [0, 1])
 def test_sector_qr(ctx, name, direction):
     c = cell(name)
