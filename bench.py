@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--no-tm", action="store_true")
     ap.add_argument("--exc-maxit", type=int, default=240, help="bound on the batched H_eff applications of the ground-state eigensolve")
     ap.add_argument("--gs-iters", type=int, default=1200, help="bound on the VUMPS iterations of the D = 512 ground state")
+    ap.add_argument("--d1024-iters", type=int, default=1000, help="bound on the D = 1024 VUMPS iterations to F <= 1e-10")
     ap.add_argument("--precs", default="fp64,fp32,bf16x3")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
@@ -263,35 +264,64 @@ def main():
             settle()
             log(f"vumps xxz D=64 {name}: {sweeps} sweeps, {dt / sweeps * 1e3:.1f} ms/sweep, F {max(st.F):.2e}, E {st.energy:.15f}")
         out["vumps_xxz_d64"] = vres
-        # VUMPS outer-iteration time on the D = 1024 Hubbard chain, as the reference reaches it: from a
-        # D = 64 bond (25 sectors, seeded randMPS_LR state) the bonds grow by ExpandFromH (at most 256
-        # states per bond and iteration) to D = 1024; after one more iteration on the final structure
-        # (plans built), the median wall time of the next five iterations is the iteration time.
+        # VUMPS on the D = 1024 Hubbard chain, as the reference reaches it: from a D = 64 bond (25
+        # sectors, seeded randMPS_LR state) the bonds grow by ExpandFromH (at most 256 states per bond
+        # and iteration) to D = 1024, in FP64; that state goes through a state file.  From it, FP64
+        # and FP32 (operators in FP32 under FP64 refinement) iterate at fixed structure to max F <=
+        # 1e-10 (bounded by --d1024-iters); the iteration time is the median over the run (plans
+        # are built in the first).  BF16X3 runs five iterations from the same state for its timing.
+        import tempfile
         k64, s64 = wl.fhub_profile(5, 8, 1, 64)
+
+        def hub(prec, **kw):
+            return VUMPS(vctx, cfg["model"], cfg["N"], cfg["QN"], k64, s64, cfg["params"], prec=prec, seed=1003, krylov=30,
+                         env_krylov=300, **kw)
+
         hres = {}
-        for name in [k for k in ("fp64", "fp32", "bf16x3") if k in results]:
-            v = VUMPS(vctx, cfg["model"], cfg["N"], cfg["QN"], k64, s64, cfg["params"], prec=precs[name], seed=1003, krylov=30,
-                      env_krylov=300, mmax=1024, dm_max=256, expand_pause=1, expand_thresh=1.0)
+        names = [k for k in ("fp64", "fp32", "bf16x3") if k in results]
+        if names:
+            t0 = time.perf_counter()
+            v = hub(im.FP64, mmax=1024, dm_max=256, expand_pause=1, expand_thresh=1.0)
             grow = 0
             while min(v.bond_dims()) < 1024 and grow < 20:
                 v.sweep()
                 grow += 1
-            v.sweep()
+            t_grow = time.perf_counter() - t0
+            state = os.path.join(tempfile.mkdtemp(), "hub_d1024.imps")
+            v.save(state)
+            dims, secs = v.bond_dims(), [len(b[1]) for b in v.bonds]
+            v.close()
+            del v
             settle()
+            log(f"vumps hubbard D=1024: grown in {grow} FP64 iterations ({t_grow:.1f} s) to {dims} ({secs} sectors)")
+        for name in names:
+            v = hub(precs[name])
+            v.load(state)
+            conv = name in ("fp64", "fp32")
+            cap = args.d1024_iters if conv else 5
             sts, its = [], []
-            for _ in range(5):
-                t0 = time.perf_counter()
+            t0 = time.perf_counter()
+            while len(sts) < cap:
+                t1 = time.perf_counter()
                 sts.append(v.sweep())
-                torch.cuda.synchronize()
-                its.append(time.perf_counter() - t0)
-            dt = float(np.median(its))
+                its.append(time.perf_counter() - t1)
+                if len(sts) % 100 == 0:
+                    log(f"  {name}: iteration {len(sts)}, F {max(sts[-1].F):.2e}")
+                if conv and max(sts[-1].F) <= 1e-10:
+                    break
+            torch.cuda.synchronize()
+            total = time.perf_counter() - t0
             st = sts[-1]
-            hres[name] = dict(s_per_iter=dt, iter_s=its, growth_iters=grow, bond_dims=v.bond_dims(), sectors=[len(b[1]) for b in v.bonds],
-                              energy=st.energy, F=max(st.F), eig_iters_per_iter=sum(x.eig_iters for x in sts) / len(sts),
-                              env_iters_per_iter=sum(x.env_iters for x in sts) / len(sts))
-            log(f"vumps hubbard D=1024 {name}: grown in {grow} iterations to {v.bond_dims()} ({hres[name]['sectors']} sectors), "
-                f"{dt:.3f} s/iteration (median of {' '.join(f'{t:.3f}' for t in its)}; {hres[name]['eig_iters_per_iter']:.0f} Lanczos, {hres[name]['env_iters_per_iter']:.0f} GMRES "
-                f"steps), E {st.energy:.12f}, F {max(st.F):.2e}")
+            dt = float(np.median(its[1:] if len(its) > 1 else its))
+            r = dict(s_per_iter=dt, iterations=len(sts), growth_iters=grow, bond_dims=v.bond_dims(), sectors=[len(b[1]) for b in v.bonds],
+                     energy=st.energy, F=max(st.F), eig_iters_per_iter=sum(x.eig_iters for x in sts) / len(sts),
+                     env_iters_per_iter=sum(x.env_iters for x in sts) / len(sts))
+            if conv:
+                r.update(seconds_to_F=total, converged=bool(max(st.F) <= 1e-10))
+            hres[name] = r
+            log(f"vumps hubbard D=1024 {name}: {len(sts)} iterations" + (f" to F {max(st.F):.2e} in {total:.1f} s" if conv else "") +
+                f", {dt:.3f} s/iteration (median; {r['eig_iters_per_iter']:.0f} Lanczos, {r['env_iters_per_iter']:.0f} GMRES "
+                f"steps per iteration), E {st.energy:.15f}")
             v.close()
             del v
             settle()
@@ -303,11 +333,12 @@ def main():
         else:
             os.environ["IMPS_AUTOTUNE"] = prev_tune
         if "fp64" in hres:
-            mx = [k for k in ("fp32", "bf16x3") if k in hres]
+            mx = [k for k in ("fp32", "bf16x3") if k in hres and "seconds_to_F" in hres[k]]  # converged runs only
             best = min(mx, key=lambda k: hres[k]["s_per_iter"]) if mx else None
             out["vumps_iter_s"] = {**{k: v["s_per_iter"] for k, v in hres.items()},
                                    "mixed_vs_fp64": hres["fp64"]["s_per_iter"] / hres[best]["s_per_iter"] if best else None,
                                    "mixed_path": best}
+            out["vumps_d1024_seconds_to_F1e-10"] = {k: v["seconds_to_F"] for k, v in hres.items() if "seconds_to_F" in v}
             lw = lieb_wu(cfg["params"][1]) - cfg["params"][1] / 4.0
             out["vumps_hubbard_d1024_energy_vs_lieb_wu"] = {k: v["energy"] - lw for k, v in hres.items()}
             out["vumps_hubbard_d1024_energy_vs_fp64"] = {k: v["energy"] - hres["fp64"]["energy"] for k, v in hres.items()}

@@ -368,8 +368,16 @@ imps_status imps_eigsh(imps_plan P, imps_plan P64, double* x, double tol, int ma
             // (FP32 ~1e-7, BF16X3 ~1e-6); nearer that floor its Ritz vector carries rounding-
             // correlated error (handing over at 1e-6, FP32 VUMPS at D = 64 stalled at a stationary
             // point 2e-8 above the FP64 energy), and the FP64 phase finishes the solve.
+            // A start vector that already meets the hand-over tolerance (the later iterations of a
+            // converging VUMPS: the previous eigenvector) skips the low-precision phase, whose first
+            // cycle would take `krylov` steps before its convergence is read.
             const double tlo = std::max(tol, P->prec == IMPS_FP32 ? 1e-5 : 1e-4);
-            KrylovStats a = lanczos_lowest(s, op_of(P), x, tlo, maxit, krylov);
+            KrylovStats a;
+            a.iters = 1;
+            if (rayleigh_relres(s, op_of(P64), x) > tlo) {
+                a = lanczos_lowest(s, op_of(P), x, tlo, maxit, krylov);
+                a.iters += 1;
+            }
             st = lanczos_lowest(s, op_of(P64), x, tol, std::max(1, maxit - a.iters), krylov);
             st.iters += a.iters;
         }

@@ -411,6 +411,23 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
     return st;
 }
 
+double rayleigh_relres(hipStream_t s, const LinOp& A, const double* x) {
+    const long n = A.n;
+    Work W;
+    W.init(n, 1);
+    d2d(s, W.Vi(0), x, n);
+    A.apply(x, W.Vi(1));
+    double h[2] = {0, 0}, xx = 0;
+    hip_check(blas_multidot(W.V.d(), n, 2, W.Vi(1), n, W.dh.d(), W.scratch.d(), s), "rayleigh");  // x.Ax, Ax.Ax
+    d2h(s, h, W.dh.d(), 2);
+    hip_check(blas_multidot(x, 0, 1, x, n, W.dh.d(), W.scratch.d(), s), "rayleigh");
+    d2h(s, &xx, W.dh.d(), 1);
+    if (!(xx > 0)) throw InvalidArg("lanczos: zero start vector");
+    const double theta = h[0] / xx;
+    // ||Ax - theta x||^2 = Ax.Ax - (x.Ax)^2 / x.x: cancellation limits this to ~1e-8 relative
+    return std::sqrt(std::max(h[1] - h[0] * theta, 0.0)) / std::max(std::abs(theta) * std::sqrt(xx), 1e-300);
+}
+
 KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol, int maxit, int m) {
     // Thick-restart Lanczos (Wu-Simon): with full CGS2 re-orthogonalisation the projected matrix
     // T = V^T A V is assembled column by column from the orthogonalisation coefficients; at a

@@ -39,6 +39,9 @@ KrylovStats gmres_refined(hipStream_t s, const LinOp& A64, const LinOp& Alo, con
 // x: start vector in, normalised eigenvector out.  Converged when ||A x - theta x|| <= tol |theta|.
 KrylovStats lanczos_lowest(hipStream_t s, const LinOp& A, double* x, double tol, int maxit, int m);
 
+// ||A x - theta x|| / (|theta| ||x||), theta = x.Ax / x.x (one application; resolves down to ~1e-8).
+double rayleigh_relres(hipStream_t s, const LinOp& A, const double* x);
+
 // Environment solver for one unit cell (InvertE_proj, InvEBlockFunctions.hpp:205-266):
 //   dir l: (y|[1 - T + |R)(L|] = (x|[1 - |R)(L|],  T = T_L[A_{N-1}] ... T_L[A_0]
 //   dir r: [1 - T + |R)(L|]|y) = [1 - |R)(L|]|x),  T = T_R[A_0] ... T_R[A_{N-1}]
