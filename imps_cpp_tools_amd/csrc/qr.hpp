@@ -28,11 +28,11 @@ struct QrSector {
     long woff;         // element offset of the sector's work area in device memory (-1: factorised in LDS)
 };
 
-// Sectors whose work area (matrix, tau, per-wave column scratch) fits kQrLdsMax are factorised in
-// LDS by 4-wave workgroups; larger ones in a device work buffer by 16-wave workgroups, so that no
-// sector size is out of reach.  secs lists the LDS sectors first (nlds of them).
-constexpr size_t kQrLdsMax = 160 * 1024;
-constexpr int kQrWavesLds = 4, kQrWavesGlobal = 16;
+// Sectors whose work area (matrix, taus, scales, R diagonal) fits kQrLdsMax are factorised in LDS, larger ones
+// in a device work buffer, so that no sector size is out of reach; 16-wave workgroups either way.
+// secs lists the LDS sectors first (nlds of them).
+constexpr size_t kQrLdsMax = 160 * 1024 - 256;  // leaves room for the kernel's static LDS
+constexpr int kQrWaves = 16;
 
 struct QrPlanData {
     std::vector<QrSector> secs;
@@ -43,8 +43,8 @@ struct QrPlanData {
     long gwork = 0;    // doubles of the device work buffer of the other sectors
 };
 
-// doubles of one sector's work area: m x n matrix, min(m, n) taus, `waves` columns of m
-inline long sector_qr_work(int m, int n, int waves) { return (long)m * n + std::min(m, n) + (long)waves * m; }
+// doubles of one sector's work area: m x n matrix, and min(m, n) each of taus, scales and R's diagonal
+inline long sector_qr_work(int m, int n) { return (long)m * n + 3L * std::min(m, n); }
 hipError_t launch_sector_qr(const QrSector* secs, int nlds, int nglobal, const QrPart* parts, const double* A, double* Q, double* R,
                             bool right, size_t lds_bytes, double* gwork, hipStream_t s);
 
@@ -79,13 +79,13 @@ inline QrPlanData build_qr(const Layout& A, int dir) {
         Blk rb{kv.first, kv.first, sc.n, sc.n, rpos};
         rpos += rb.size();
         P.R.l[0].b.push_back(rb);
-        const size_t lds = sizeof(double) * (size_t)sector_qr_work(sc.m, sc.n, kQrWavesLds);
+        const size_t lds = sizeof(double) * (size_t)sector_qr_work(sc.m, sc.n);
         if (lds <= kQrLdsMax) {
             sc.woff = -1;
             P.lds = std::max(P.lds, lds);
         } else {
             sc.woff = P.gwork;
-            P.gwork += (sector_qr_work(sc.m, sc.n, kQrWavesGlobal) + 31) / 32 * 32;  // 256-byte aligned areas
+            P.gwork += (sector_qr_work(sc.m, sc.n) + 31) / 32 * 32;  // 256-byte aligned areas
         }
         P.secs.push_back(sc);
     }
