@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--exc-maxit", type=int, default=240, help="bound on the batched H_eff applications of the ground-state eigensolve")
     ap.add_argument("--gs-iters", type=int, default=1200, help="bound on the VUMPS iterations of the D = 512 ground state")
     ap.add_argument("--d1024-iters", type=int, default=1000, help="bound on the D = 1024 VUMPS iterations to F <= 1e-10")
+    ap.add_argument("--exc-synthetic-only", action="store_true", help="excitations on the synthetic 81-sector state only (no ground state)")
     ap.add_argument("--precs", default="fp64,fp32,bf16x3")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
@@ -406,19 +407,25 @@ def main():
         from imps_cpp_tools_amd.vumps import VUMPS
         settle()
         torch.cuda.set_stream(torch.cuda.default_stream())
-        ectx = im.Context(0)
+        # the ground state's VUMPS on torch's default stream; the excitation engine on a stream of its
+        # own (default priority), so that its kernels are the only ones there; every hand-over between
+        # the two is a device synchronisation
+        gctx = im.Context(0)
+        xstream = torch.cuda.Stream()
+        ectx = im.Context(0, xstream)
         prev_tune = os.environ.get("IMPS_AUTOTUNE")
         os.environ["IMPS_AUTOTUNE"] = "0"
         ec = wl.HUBBARD_512
         ps = [j / 31 for j in range(32)]
         exc = {}
 
-        def heff_run(gs, label, maxit, check_conv):
+        def heff_run(gs, label, maxit, check_conv, prec=im.FP64):
             E0 = gs.energy()
-            nls = [left_nullspace(ectx, gs.AL[n], gs.slay[n], seed=n) for n in range(ec["N"])]
+            nls = [left_nullspace(gctx, gs.AL[n], gs.slay[n], seed=n) for n in range(ec["N"])]
+            torch.cuda.synchronize()
             t0 = time.perf_counter()
             P = im.HeffPlan(ectx, gs.slay, gs.slay, [l for _, l in nls], [gs.blay[(n + 1) % ec["N"]] for n in range(ec["N"])], gs.H, E0,
-                            (0, 0), len(ps))
+                            (0, 0), len(ps), prec=prec)
             P.set_state(gs.AL, gs.AR, [a for a, _ in nls], gs.C, inv_tol=1e-11)
             torch.cuda.synchronize()
             setup = time.perf_counter() - t0
@@ -426,6 +433,7 @@ def main():
             zr = np.random.default_rng(7)
             x = torch.from_numpy(zr.standard_normal((len(ps), P.mtot)) + 1j * zr.standard_normal((len(ps), P.mtot))).cuda()
             y = torch.empty_like(x)
+            torch.cuda.synchronize()
             P.apply(ps, x, y)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -436,9 +444,11 @@ def main():
             # Hermiticity of H_eff(k) at every momentum: <x_t, H y_t> = conj(<y_t, H x_t>)
             y2 = torch.empty_like(x)
             w = torch.from_numpy(zr.standard_normal((len(ps), P.mtot)) + 1j * zr.standard_normal((len(ps), P.mtot))).cuda()
+            torch.cuda.synchronize()
             P.apply(ps, w, y2)
-            herm = max(abs(torch.vdot(w[t], y[t]).item() - torch.vdot(x[t], y2[t]).item().conjugate()) / abs(torch.vdot(w[t], y[t]).item())
-                       for t in range(len(ps)))
+            torch.cuda.synchronize()
+            hx, hy, hw, hy2 = (a.cpu().numpy() for a in (x, y, w, y2))  # the check on the host
+            herm = max(abs(np.vdot(hw[t], hy[t]) - np.conj(np.vdot(hx[t], hy2[t]))) / abs(np.vdot(hw[t], hy[t])) for t in range(len(ps)))
             t0 = time.perf_counter()
             ev, _, res, na = P.eigs(ps, nev=4, tol=1e-10, krylov=20, maxit=maxit, check=False)
             torch.cuda.synchronize()
@@ -450,40 +460,56 @@ def main():
                  "gmres_worst_relres": rr, "hermiticity_worst_rel": herm,
                  "eigs_s_32_momenta": t_eig, "eigs_batched_applies": na, "eigs_worst_rel_residual": float(rel.max()),
                  "eigs_converged": bool(rel.max() <= 1e-10), "lowest_energy_per_momentum": [float(v) for v in ev[:, 0]],
-                 "four_lowest_p0": [float(v) for v in ev[0]], "min_energy": float(ev.min())}
+                 "four_lowest_p0": [float(v) for v in ev[0]], "min_energy": float(ev.min()), "_ev": ev}
+            label = label + ("" if prec == im.FP64 else f" [{ {im.FP32: 'fp32', im.BF16X3: 'bf16x3'}[prec]} TM chains, FP64 refinement]")
             log(f"excitations {label}: D {gs.bond_dims()}, mtot {P.mtot}, batched apply {t_apply * 1e3:.1f} ms for {len(ps)} momenta "
                 f"({steps} GMRES steps), eigs {t_eig:.2f} s / {na} applies, worst residual {rel.max():.1e}, min E {ev.min():.6f}")
             del P
             return r
 
-        # ground state
-        k64, s64 = wl.fhub_profile(5, 8, 1, 64)
-        t0 = time.perf_counter()
-        gs = VUMPS(ectx, ec["model"], ec["N"], ec["QN"], k64, s64, ec["params"], prec=im.FP64, seed=1005, krylov=30, env_krylov=300,
-                   mmax=ec["total"], dm_max=128, expand_pause=1, expand_thresh=1.0)
-        it = 0
-        while min(gs.bond_dims()) < ec["total"] and it < 40:
-            st = gs.sweep()
-            it += 1
-        gs.mmax = None
-        while it < args.gs_iters:
-            st = gs.sweep()
-            it += 1
-            if max(st.F) <= 1e-10:
-                break
-        torch.cuda.synchronize()
-        t_gs = time.perf_counter() - t0
-        exc["ground_state"] = {"vumps_iterations": it, "seconds": t_gs, "F": max(st.F), "energy": st.energy,
-                               "energy_vs_lieb_wu": st.energy - (lieb_wu(ec["params"][1]) - ec["params"][1] / 4.0)}
-        log(f"ground state D=512: {it} VUMPS iterations in {t_gs:.1f} s, F {max(st.F):.2e}, E {st.energy:.14f}")
-        exc["hubbard_d512_ground_state"] = heff_run(gs, "VUMPS ground state (grown from D=64, F<=1e-10)", args.exc_maxit, False)
-        gs.close()
-        del gs
-        settle()
+        if not args.exc_synthetic_only:
+            # ground state
+            k64, s64 = wl.fhub_profile(5, 8, 1, 64)
+            t0 = time.perf_counter()
+            gs = VUMPS(gctx, ec["model"], ec["N"], ec["QN"], k64, s64, ec["params"], prec=im.FP64, seed=1005, krylov=30, env_krylov=300,
+                       mmax=ec["total"], dm_max=128, expand_pause=1, expand_thresh=1.0)
+            it = 0
+            while min(gs.bond_dims()) < ec["total"] and it < 40:
+                st = gs.sweep()
+                it += 1
+            gs.mmax = None
+            while it < args.gs_iters:
+                st = gs.sweep()
+                it += 1
+                if max(st.F) <= 1e-10:
+                    break
+            torch.cuda.synchronize()
+            t_gs = time.perf_counter() - t0
+            exc["ground_state"] = {"vumps_iterations": it, "seconds": t_gs, "F": max(st.F), "energy": st.energy,
+                                   "energy_vs_lieb_wu": st.energy - (lieb_wu(ec["params"][1]) - ec["params"][1] / 4.0)}
+            log(f"ground state D=512: {it} VUMPS iterations in {t_gs:.1f} s, F {max(st.F):.2e}, E {st.energy:.14f}")
+            exc["hubbard_d512_ground_state"] = heff_run(gs, "VUMPS ground state (grown from D=64, F<=1e-10)", args.exc_maxit, False)
+            exc["hubbard_d512_ground_state_mixed"] = heff_run(gs, "VUMPS ground state (grown from D=64, F<=1e-10)", args.exc_maxit, False,
+                                                              prec=im.FP32)
+            gs.close()
+            del gs
+            settle()
         # the synthetic structure (81 sectors) with its seeded random mixed-canonical state
         k5, s5 = wl.fhub_profile(ec["nside"], ec["peak"], 1, ec["total"])
-        gr = VUMPS(ectx, ec["model"], ec["N"], ec["QN"], k5, s5, ec["params"], prec=im.FP64, seed=1005, env_krylov=300)
+        gr = VUMPS(gctx, ec["model"], ec["N"], ec["QN"], k5, s5, ec["params"], prec=im.FP64, seed=1005, env_krylov=300)
         exc["hubbard_d512_random_81_sectors"] = heff_run(gr, "seeded random mixed-canonical state, 81 sectors", 2000, True)
+        exc["hubbard_d512_random_81_sectors_mixed"] = heff_run(gr, "seeded random mixed-canonical state, 81 sectors", 2000, True, prec=im.FP32)
+        # the mixed runs against FP64: energies (all 32 momenta x 4 levels) and speed
+        for key in ("hubbard_d512_ground_state", "hubbard_d512_random_81_sectors"):
+            if key in exc and key + "_mixed" in exc:
+                a, b = exc[key], exc[key + "_mixed"]
+                b["energy_err_vs_fp64"] = float(np.abs(b["_ev"] - a["_ev"]).max())
+                b["apply_speedup_vs_fp64"] = a["batched_apply_s"] / b["batched_apply_s"]
+                b["eigs_speedup_vs_fp64"] = a["eigs_s_32_momenta"] / b["eigs_s_32_momenta"]
+                log(f"excitations {key}: FP32 TM chains vs FP64: energies {b['energy_err_vs_fp64']:.1e}, apply x{b['apply_speedup_vs_fp64']:.2f}, "
+                    f"eigs x{b['eigs_speedup_vs_fp64']:.2f}")
+        for r in exc.values():
+            r.pop("_ev", None)
         gr.close()
         del gr
         settle()

@@ -98,7 +98,7 @@ def lib():
             "imps_tm_vjp": (I, [P, V, V, V, V, V, V, V]),
             "imps_hac_vjp": (I, [P, V, V, V, V, V, V, V, V, V, V, V]),
             "imps_hc_vjp": (I, [P, V, V, V, V, V, V, V, V, V, V, V]),
-            "imps_heff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I]),
+            "imps_heff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I, I]),
             "imps_heff_set_state": (I, [P, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), D]),
             "imps_heff_check": (I, [P, dp, dp, ip]),
             "imps_heff_apply": (I, [P, dp, V, V, ip, dp]),
@@ -617,19 +617,20 @@ class HeffPlan(Plan):
     constants, Excitations_helpers.hpp:20-347): imps_heff_plan / set_state / apply / eigs.
 
     AL, AR, NL: per-site MPS layouts; C: per-site bond layouts (C[n] right of site n); H: dense
-    d^2 x d^2 two-site Hamiltonian; E0: ground-state energy per site (subtracted from H); K: charge.
-    Excitation vectors are complex128 CUDA tensors of shape (nmom, mtot) in XDims order."""
+    d^2 x d^2 two-site Hamiltonian; E0: ground-state energy per site (subtracted from H); K: charge;
+    prec: FP64, or FP32 / BF16X3 transfer-matrix chains in the geometric sums' Arnoldi steps under
+    FP64 restarts.  Excitation vectors are complex128 CUDA tensors of shape (nmom, mtot) in XDims order."""
     kind = "heff"
 
     def __init__(self, ctx: Context, AL: Sequence[BlockLayout], AR: Sequence[BlockLayout], NL: Sequence[BlockLayout],
-                 C: Sequence[BlockLayout], H: np.ndarray, E0: float, K, nmom: int = 1):
+                 C: Sequence[BlockLayout], H: np.ndarray, E0: float, K, nmom: int = 1, prec: int = FP64):
         h = ctypes.c_void_p()
         N = len(AL)
         arr = [(_CLayout * N)(*[x.c() for x in L]) for L in (AL, AR, NL, C)]
         self._keep_layouts = (list(AL), list(AR), list(NL), list(C))
         Hf = np.asfortranarray(H, dtype=np.float64).reshape(-1, order="F").copy()
         Kq = np.ascontiguousarray(np.atleast_1d(K), np.int32)
-        _check(lib().imps_heff_plan(ctx.h, ctypes.byref(h), N, AL[0].nlist, *arr, _dptr(Hf), float(E0), _iptr(Kq), int(nmom)))
+        _check(lib().imps_heff_plan(ctx.h, ctypes.byref(h), N, AL[0].nlist, *arr, _dptr(Hf), float(E0), _iptr(Kq), int(nmom), int(prec)))
         super().__init__(h, ctx)
         self.N, self.nmom = N, int(nmom)
         self.xdims = self._out_layout(AL[0].nsym)

@@ -478,9 +478,11 @@ imps_status imps_env_fixed_point(imps_plan P, double* x, double tol, int maxit, 
 
 // ---- excitations ---------------------------------------------------------------------------------
 imps_status imps_heff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, const imps_layout* AL, const imps_layout* AR,
-                           const imps_layout* NL, const imps_layout* C, const double* H, double E0, const int* K, int nmom) {
+                           const imps_layout* NL, const imps_layout* C, const double* H, double E0, const int* K, int nmom,
+                           int prec) {
     return guarded([&] {
         if (!plan || !AL || !AR || !NL || !C || !K) throw InvalidArg("plan, layouts or K is NULL");
+        if (prec != IMPS_FP64 && prec != IMPS_FP32 && prec != IMPS_BF16X3) throw InvalidArg("prec must be IMPS_FP64, IMPS_FP32 or IMPS_BF16X3");
         if (ncell < 1 || ncell > 64) throw InvalidArg("ncell out of range");
         std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_HEFF, IMPS_FP64));
         std::vector<Layout> la, lr, ln, lc;
@@ -492,7 +494,7 @@ imps_status imps_heff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, 
         }
         if (ctx) hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         P->heff = std::make_shared<HeffEngine>();
-        P->heff->build(ctx, ncell, d, la, lr, ln, lc, H, E0, K, nmom);
+        P->heff->build(ctx, ncell, d, la, lr, ln, lc, H, E0, K, nmom, prec);
         P->out_layout = P->heff->Xin.lay;
         *plan = P.release();
         return IMPS_OK;

@@ -379,11 +379,13 @@ XObj HeffEngine::invert_e(const std::vector<XObj>& As, const std::vector<XObj>& 
             out = l2;
             g->bond_elems.push_back(out.nelem);
         }
-        std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_TM, IMPS_FP64));
-        P->prog = build_tm(dir, A.lay, &B.lay, in, out);
-        P->op_parts = {A.lay.nelem, B.lay.nelem};
-        upload_program(P.get());
-        g->chain.push_back(std::move(P));
+        for (int lo = 0; lo < (prec == IMPS_FP64 ? 1 : 2); ++lo) {
+            std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_TM, lo ? prec : IMPS_FP64));
+            P->prog = build_tm(dir, A.lay, &B.lay, in, out);
+            P->op_parts = {A.lay.nelem, B.lay.nelem};
+            upload_program(P.get());
+            (lo ? g->chain_lo : g->chain).push_back(std::move(P));
+        }
         g->chain_ops.push_back({A, B});
         in = out;
     }
@@ -401,8 +403,10 @@ XObj HeffEngine::invert_e(const std::vector<XObj>& As, const std::vector<XObj>& 
 }
 
 void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<Layout>& al, const std::vector<Layout>& ar,
-                       const std::vector<Layout>& nl, const std::vector<Layout>& cl, const double* H, double E0, const int* Kq, int nmomenta) {
+                       const std::vector<Layout>& nl, const std::vector<Layout>& cl, const double* H, double E0, const int* Kq, int nmomenta,
+                       int precision) {
     if (!c) throw InvalidArg("heff: a device context is required");
+    prec = precision;
     if (ncell < 1 || dphys < 1) throw InvalidArg("heff: bad cell size or physical dimension");
     if ((int)al.size() != ncell || (int)ar.size() != ncell || (int)nl.size() != ncell || (int)cl.size() != ncell)
         throw InvalidArg("heff: one layout per site is needed for AL, AR, NL and C");
@@ -743,8 +747,10 @@ void HeffEngine::set_state(const std::vector<const double*>& al, const std::vect
     h2d_sync(s, A() + LMo.off, lm.data(), sizeof(double) * b0.nelem);
     for (auto& gp : geos) {
         GeoOp& g = *gp;
-        for (size_t k = 0; k < g.chain.size(); ++k)
+        for (size_t k = 0; k < g.chain.size(); ++k) {
             set_operator(g.chain[k].get(), {A() + g.chain_ops[k].first.off, A() + g.chain_ops[k].second.off});
+            if (!g.chain_lo.empty()) set_operator(g.chain_lo[k].get(), {A() + g.chain_ops[k].first.off, A() + g.chain_ops[k].second.off});
+        }
         if (!g.proj) continue;
         const std::vector<double>& rin = (g.rin.off == LMo.off) ? lm : ch;
         const std::vector<double>& ro = (g.rout_src.off == LMo.off) ? lm : ch;
@@ -848,19 +854,26 @@ void HeffEngine::run_geo(GeoOp& g) {
     const double* dones = dgam + 2 * nmom;
     bool anyproj = false;
     for (int t = 0; t < nmom; ++t) anyproj = anyproj || gam[t] != 0;
-    // operator: out = v - fac T(v) + gam tr(v rho_in) rho_out
-    auto op = [&](const double* v, double* out) {
+    // operator: out = v - fac T(v) + gam tr(v rho_in) rho_out, T from the FP64 chain or (Arnoldi
+    // steps of a mixed engine) the low-precision one
+    const bool mixed = !g.chain_lo.empty();
+    auto op_chain = [&](const std::vector<std::unique_ptr<imps_plan_s>>& chain, const double* v, double* out) {
         const double* in = v;
         long off = 0;
-        for (size_t k = 0; k < g.chain.size(); ++k) {
-            double* o = (k + 1 == g.chain.size()) ? Tb : bonds + off * nv;
-            run(g.chain[k].get(), in, o, nv);
+        for (size_t k = 0; k < chain.size(); ++k) {
+            double* o = (k + 1 == chain.size()) ? Tb : bonds + off * nv;
+            run(chain[k].get(), in, o, nv);
             in = o;
-            if (k + 1 < g.chain.size()) off += g.bond_elems[k];
+            if (k + 1 < chain.size()) off += g.bond_elems[k];
         }
         if (anyproj) hip_check(rdot_partials(A() + g.w.off, v, n, nmom, part, s), "trace");
         hip_check(geo_combine(out, 1.0, v, Tb, dfac, dgpos, anyproj ? part : nullptr, nch, A() + g.rout.off, n, nmom, s), "1 - fac T");
     };
+    auto op = [&](const double* v, double* out) { op_chain(g.chain, v, out); };
+    auto op_step = [&](const double* v, double* out) { op_chain(mixed ? g.chain_lo : g.chain, v, out); };
+    // a mixed cycle reduces its starting residual at most this far: below, the low-precision
+    // products' rounding dominates and the next restart's FP64 residual takes over
+    const double inner_tol = prec == IMPS_FP32 ? 1e-5 : 1e-4;
     // right-hand side b = x - gam tr(x rho_in) rho_out
     const double* x = A() + g.x.off;
     double* y = A() + g.y.off;
@@ -929,7 +942,7 @@ void HeffEngine::run_geo(GeoOp& g) {
             const int q = std::min(batch, m - j0);
             for (int j = j0; j < j0 + q; ++j) {
                 double* w = V + (size_t)vl * (j + 1);
-                op(V + (size_t)vl * j, w);
+                op_step(V + (size_t)vl * j, w);
                 for (int pass = 0; pass < 2; ++pass) {
                     hip_check(cdot_partials(V, vl, j + 1, w, n, nmom, part, s), "cgs2 dots");
                     hip_check(cgemv_sub_partials(V, vl, j + 1, part, nch, cbuf + (size_t)(j - j0) * sst + pass * 2 * (m + 1), cst, w, n, nmom,
@@ -971,7 +984,8 @@ void HeffEngine::run_geo(GeoOp& g) {
                     H[j + 1 + (size_t)j * (m + 1)] = 0.0;
                     gg[t][j + 1] = -std::conj(sn[t][j]) * gg[t][j];
                     gg[t][j] *= cs[t][j];
-                    if (std::abs(gg[t][j + 1]) < tol * bnorm[t] || hn == 0) cyc[t] = 0;
+                    const double stop = mixed ? std::max(tol * bnorm[t], inner_tol * beta[t]) : tol * bnorm[t];
+                    if (std::abs(gg[t][j + 1]) < stop || hn == 0) cyc[t] = 0;
                 }
                 if (cyc[t]) stepping = true;
             }

@@ -130,6 +130,9 @@ struct HeffEngine : BlockTape {
     double inv_tol = 1e-14;
     int krylov = 40;
     int gmres_maxit = 4000;
+    // IMPS_FP64, or IMPS_FP32 / IMPS_BF16X3: the Arnoldi steps of the geometric sums apply their TM
+    // chains at that precision; every restart takes the true FP64 residual (iterative refinement)
+    int prec = IMPS_FP64;
 
     // ---- objects ----
     std::vector<XObj> AL, AR, NL, C;  // constants, filled by set_state
@@ -145,6 +148,7 @@ struct HeffEngine : BlockTape {
         int dir = 0, pow = 1;
         XObj x, y;
         std::vector<std::unique_ptr<imps_plan_s>> chain;  // TM plans in application order
+        std::vector<std::unique_ptr<imps_plan_s>> chain_lo;  // the same at the engine's low precision (mixed engines)
         std::vector<long> bond_elems;
         std::vector<std::pair<XObj, XObj>> chain_ops;     // (A, B) of each chain plan
         bool proj = false;
@@ -170,7 +174,8 @@ struct HeffEngine : BlockTape {
 
     // build: layouts (AL, AR, NL per site; C[n] on bond n + 1), H (dense d^2 x d^2 column-major), E0, K
     void build(imps_context c, int ncell, int dphys, const std::vector<Layout>& al, const std::vector<Layout>& ar,
-               const std::vector<Layout>& nl, const std::vector<Layout>& cl, const double* H, double E0, const int* Kq, int nmomenta);
+               const std::vector<Layout>& nl, const std::vector<Layout>& cl, const double* H, double E0, const int* Kq, int nmomenta,
+               int precision = IMPS_FP64);
     void set_state(const std::vector<const double*>& al, const std::vector<const double*>& ar, const std::vector<const double*>& nl,
                    const std::vector<const double*>& cl, double tol);
     // y = H_eff(p_t) x for every momentum t (interleaved complex [nmom][mtot] device vectors)

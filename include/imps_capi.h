@@ -258,9 +258,14 @@ imps_status imps_qr_apply(imps_plan plan, const double* A, double* Q, double* R)
  * at tolerance inv_tol), the reduced operators and the dominant eigenmatrices of the mixed transfer
  * matrix in closed form, LM = C^T and RM = C on bond 0 (LRoverlaps, :915-946, for AL C = C AR).
  * imps_heff_check: the reference's check of HeffConstants (:287-303), tr(HL R) and tr(L HR), which
- * vanish when E0 is the state's energy; and the environment solver's operator applications. */
+ * vanish when E0 is the state's energy; and the environment solver's operator applications.
+ * prec: IMPS_FP64 (the reference's arithmetic), or IMPS_FP32 / IMPS_BF16X3: the Arnoldi steps of the
+ * geometric sums apply their transfer-matrix chains in that precision and every GMRES restart
+ * takes the FP64 residual (iterative refinement), so the sums still reach inv_tol in FP64; the
+ * constants, the eigensolver and everything else stay FP64. */
 imps_status imps_heff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, const imps_layout* AL, const imps_layout* AR,
-                           const imps_layout* NL, const imps_layout* C, const double* H, double E0, const int* K, int nmom);
+                           const imps_layout* NL, const imps_layout* C, const double* H, double E0, const int* K, int nmom,
+                           int prec);
 imps_status imps_heff_set_state(imps_plan plan, const double* const* AL, const double* const* AR, const double* const* NL,
                                 const double* const* C, double inv_tol);
 imps_status imps_heff_check(imps_plan plan, double* trHL_R, double* trL_HR, int* env_iters);

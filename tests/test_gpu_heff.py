@@ -36,14 +36,15 @@ def ctx():
     return _CTX[0]
 
 
-def plan(which, K, nmom):
-    key = (which, K, nmom)
+def plan(which, K, nmom, prec=im.FP64, inv_tol=1e-14):
+    key = (which, K, nmom, prec, inv_tol)
     if key not in _PLANS:
         s = es.state() if which == "xxz" else es.hubbard_state()
-        print(f"heff plan {which} K={K} nmom={nmom}", flush=True)
+        print(f"heff plan {which} K={K} nmom={nmom} prec={prec}", flush=True)
         P = im.HeffPlan(ctx(), [_lay(a) for a in s["ALob"]], [_lay(a) for a in s["ARob"]], [_lay(a) for a in s["NLob"]],
-                        [_lay(c) for c in s["Cob"]], s["H"], s["E0"], K, nmom)
-        P.set_state([_dev(a) for a in s["ALob"]], [_dev(a) for a in s["ARob"]], [_dev(a) for a in s["NLob"]], [_dev(c) for c in s["Cob"]])
+                        [_lay(c) for c in s["Cob"]], s["H"], s["E0"], K, nmom, prec=prec)
+        P.set_state([_dev(a) for a in s["ALob"]], [_dev(a) for a in s["ARob"]], [_dev(a) for a in s["NLob"]], [_dev(c) for c in s["Cob"]],
+                    inv_tol=inv_tol)
         _PLANS[key] = (P, s)
     return _PLANS[key]
 
@@ -128,3 +129,34 @@ def test_lowest_energies_batched(which, K):
         assert np.abs(V.conj() @ V.T - np.eye(4)).max() < 1e-10
         R = M @ V.T - V.T * ev[t]
         assert np.abs(R).max() < 1e-8
+
+
+@pytest.mark.parametrize("prec", [im.FP32, im.BF16X3], ids=["fp32", "bf16x3"])
+@pytest.mark.parametrize("which,K", [("xxz", (2,)), ("hub", (0, 0)), ("hub", (1, 1))])
+def test_mixed_apply_refined_to_fp64(which, K, prec):
+    """Low-precision TM chains in the geometric sums' Arnoldi steps, FP64 residual at every restart:
+    the sums reach the FP64 tolerance (1e-12 here), so H_eff X matches the oracle as the FP64 path does."""
+    P, s = plan(which, K, len(MOMENTA), prec=prec, inv_tol=1e-12)
+    rng = np.random.default_rng(24)
+    z = rng.standard_normal((len(MOMENTA), P.mtot)) + 1j * rng.standard_normal((len(MOMENTA), P.mtot))
+    x = torch.from_numpy(z).cuda()
+    y = torch.empty_like(x)
+    steps, rr = P.apply(MOMENTA, x, y)
+    got = y.cpu().numpy()
+    assert rr < 1e-12
+    for t, p in enumerate(MOMENTA):
+        ref = es.oracle_heff(K, p, z[t], s, proj_at_unit_phase=True)
+        err = np.linalg.norm(got[t] - ref) / np.linalg.norm(ref)
+        print(f"{which} K={K} p={p} prec={prec}: relerr {err:.2e} (gmres {steps} steps, relres {rr:.1e})", flush=True)
+        assert err < 1e-10
+
+
+@pytest.mark.parametrize("which,K", [("xxz", (0,)), ("hub", (0, 0))])
+def test_mixed_lowest_energies_equal_fp64(which, K):
+    moms = [0.0, 0.3]
+    P64, s = plan(which, K, len(moms))
+    P32, _ = plan(which, K, len(moms), prec=im.FP32, inv_tol=1e-14)
+    e64, _, _, _ = P64.eigs(moms, nev=4, tol=1e-10)
+    e32, _, _, na = P32.eigs(moms, nev=4, tol=1e-10)
+    print(f"{which} K={K}: FP32-refined {e32.ravel()} vs FP64 {e64.ravel()} ({na} applies)", flush=True)
+    assert np.abs(e32 - e64).max() < 1e-9
