@@ -1043,6 +1043,8 @@ int HeffEngine::eigs(const double* p, int nev, double tol, int maxit, int m, dou
     if (maxit <= 0) maxit = 100 * m;
     p_cur.assign(p, p + nmom);
     upload_coefs();
+    stats_steps_total = 0;
+    stats_relres_worst = 0;
     const int nv = 2 * nmom;
     const long vl = (long)nv * n;
     const int nch = cblas_chunks(n), gch = cblas_gemv_grid(n, nmom);
@@ -1134,7 +1136,8 @@ int HeffEngine::eigs(const double* p, int nev, double tol, int maxit, int m, dou
                 }
                 open += ok ? 0 : 1;
             }
-            std::fprintf(stderr, "heff eigs: %d applications, worst relative residual %.3e, %d of %d momenta open\n", napply, worst, open, nmom);
+            std::fprintf(stderr, "heff eigs: %d applications, worst relative residual %.3e, %d of %d momenta open (geometric sums: worst GMRES residual %.1e)\n",
+                         napply, worst, open, nmom, stats_relres_worst);
         }
         const bool stop = all_ok || napply >= maxit;
         const int nk = stop ? nev : keep;

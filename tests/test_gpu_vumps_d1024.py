@@ -16,7 +16,7 @@ import imps_cpp_tools_amd as im
 
 pytestmark = pytest.mark.gpu
 
-MAX_ITERS = 900
+MAX_ITERS = 1200
 _E = {}
 
 
