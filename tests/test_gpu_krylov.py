@@ -2,9 +2,10 @@
 low-precision inner solves inside FP64 refinement, and the Lanczos lowest eigenpair of H_AC / H_C
 against a dense eigensolve of the explicitly assembled operator.
 
-Tolerances: environments ||y - y_oracle|| / ||y_oracle|| < 1e-9, both solved to a relative residual
+Tolerances: environments ||y - y_oracle|| / ||y_oracle|| < 1e-11, both solved to a relative residual
 of 1e-13 (the random XXZ cell has a subdominant TM eigenvalue 0.997, so the projected system has a
-condition number near 1e3 and 1e-14 is below what the reference's unrestarted MGS GMRES attains);
+condition number near 1e3 and 1e-14 is below what the reference's unrestarted MGS GMRES attains;
+observed 2e-14 .. 5e-13); GradNorm 1e-12 relative (observed 1e-14);
 eigenvalues |e - e_dense| / |e_dense| < 1e-11 in every precision (low-precision Lanczos is refined
 with FP64 applies); eigenvector residual ||H x - e x|| < 1e-9 |e|.
 """
@@ -98,7 +99,7 @@ def test_env_solve_parity(ctx, prec, name, direction):
     assert rr < 1e-13
     err = relerr(y.cpu().numpy(), yref.data)
     print(name, direction, prec, "env relerr", err, "steps", it, flush=True)
-    assert err < 1e-9, err
+    assert err < 1e-11, err
 
 
 @pytest.mark.parametrize("name", ["xxz", "fhub"])
@@ -120,7 +121,7 @@ def test_env_solve_long_krylov_space(ctx, name, direction):
     err = relerr(y.cpu().numpy(), yref.data)
     print(name, direction, "n", x.nelem, "steps", it, "env relerr", err, flush=True)
     assert rr < 1e-13
-    assert err < 1e-9, err
+    assert err < 1e-11, err
 
 
 def dense_operator(apply, n):
@@ -230,7 +231,7 @@ def test_gradnorm_matches_oracle():
         HR = orc.reduced_op(o.H, o.d, o.AR[(n + 1) % g["N"]], 1)
         F_ref = orc.gradnorm(1, o.AR[n], o.C[left], o.LE[n], o.RE[(n + 1) % g["N"]], HL, HR)
         print("gradnorm", n, F_gpu, F_ref, abs(F_gpu - F_ref) / F_ref, flush=True)
-        assert abs(F_gpu - F_ref) <= 1e-9 * F_ref, (n, F_gpu, F_ref)
+        assert abs(F_gpu - F_ref) <= 1e-12 * F_ref, (n, F_gpu, F_ref)
 
 
 @pytest.mark.parametrize("prec", [im.FP64, im.BF16X3], ids=["fp64", "bf16x3"])
