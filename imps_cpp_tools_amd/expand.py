@@ -263,3 +263,57 @@ def append_rows(AR, new_rows):
             else:
                 out[s][Q] = (None, rows)
     return out
+
+
+# ---- Truncate (MPSBlockUtilities.hpp:846-936) --------------------------------------------------
+def truncate_sizes(lam: Dict[Key, np.ndarray], mmax: int, lamthresh: float = 1e-10) -> Tuple[Dict[Key, int], bool]:
+    """Truncate's selection (MPSBlockUtilities.hpp:846-906): of the Schmidt values on a bond
+    ({Q: values in descending order}), those above lamthresh survive, and when more than mmax do,
+    the mmax largest of them.  Returns ({Q: kept count}, changed).  A sector keeps its leading
+    values, so the count is all that is needed (0: the sector goes).  Ties at the mmax cut are kept
+    in ascending charge order, then by position (the reference's std::sort leaves that order open)."""
+    vals = []
+    small = False
+    for q in sorted(lam):
+        for i, v in enumerate(np.asarray(lam[q], float)):
+            if v > lamthresh:
+                vals.append((-float(v), q, i))
+            else:
+                small = True
+    big = len(vals) > mmax
+    keep = {q: 0 for q in lam}
+    if big:
+        vals.sort()
+        vals = vals[:mmax]
+    for _, q, _ in vals:
+        keep[q] += 1
+    return keep, bool(big or small)
+
+
+def schmidt_basis(C: Dict[Key, np.ndarray]):
+    """Per sector C_Q = U_Q diag(S_Q) V_Q^T (singular values descending): ({Q: U}, {Q: S}, {Q: V})."""
+    U, S, V = {}, {}, {}
+    for q, m in C.items():
+        u, s, vt = np.linalg.svd(m)
+        U[q], S[q], V[q] = u, s, vt.T
+    return U, S, V
+
+
+def rotate_right_index(A, R: Dict[Key, np.ndarray]):
+    """A's blocks (q_in -> Q) times R_Q on the right (A from mps_blocks; Q the block's out charge)."""
+    return [{qi: (Q, blk @ R[Q]) for qi, (Q, blk) in ent.items()} for ent in A]
+
+
+def rotate_left_index(A, R: Dict[Key, np.ndarray]):
+    """R_Q^T times A's blocks (Q -> q_out) on the left."""
+    return [{Q: (qo, R[Q].T @ blk) for Q, (qo, blk) in ent.items()} for ent in A]
+
+
+def cut_right_index(A, keep: Dict[Key, int]):
+    """A's blocks (q_in -> Q) with their first keep[Q] columns; blocks into a dropped sector go."""
+    return [{qi: (Q, blk[:, :keep[Q]]) for qi, (Q, blk) in ent.items() if keep.get(Q, 0) > 0} for ent in A]
+
+
+def cut_left_index(A, keep: Dict[Key, int]):
+    """A's blocks (Q -> q_out) with their first keep[Q] rows; blocks out of a dropped sector go."""
+    return [{Q: (qo, blk[:keep[Q], :]) for Q, (qo, blk) in ent.items() if keep.get(Q, 0) > 0} for ent in A]

@@ -578,6 +578,50 @@ class VUMPS:
         self.RE[b] = self._tm(RIGHT, self.AR[r], r, self.RE[self._b(r + 1)], self._b(r + 1), r) + self._optm_right(r)
         return int(sum(add.values()))
 
+    def truncate(self, b: int, mmax: int, lamthresh: float = 1e-10) -> int:
+        """Truncate (MPSBlockUtilities.hpp:846-936) on bond b: the bond is brought to its Schmidt
+        basis (C[b-1] = U S V^T per sector; AL[b-1] U, AR[b-1] V, U^T AL[b], V^T AR[b], C = S: the
+        state and the isometries are unchanged), then only the Schmidt values above lamthresh and, of
+        those, the mmax largest are kept (expand.truncate_sizes): each sector keeps its leading
+        states and a sector with none left is dropped.  C is renormalised to unit norm.  AL[b-1] and
+        AR[b] stay isometric; AR[b-1] and AL[b] are only approximately so until the next iteration.
+        Plans are rebuilt and the environments on bond b recomputed, as after a growth.  Returns the
+        number of states removed."""
+        from .expand import (cut_left_index, cut_right_index, diag_blocks, flatten_diag, flatten_mps, mps_blocks, rotate_left_index,
+                             rotate_right_index, schmidt_basis, truncate_sizes)
+        N = self.N
+        b = self._b(b)
+        l, r = self._b(b - 1), b
+        host = lambda t: t.cpu().numpy()  # noqa: E731
+        Cb = diag_blocks(self.blay[b], host(self.C[l]))
+        U, S, V = schmidt_basis(Cb)
+        keep, changed = truncate_sizes(S, mmax, lamthresh)
+        if not changed:
+            return 0
+        removed = int(sum(len(S[q]) - keep[q] for q in S))
+        AL_l = cut_right_index(rotate_right_index(mps_blocks(self.slay[l], host(self.AL[l])), U), keep)
+        AR_l = cut_right_index(rotate_right_index(mps_blocks(self.slay[l], host(self.AR[l])), V), keep)
+        AL_r = cut_left_index(rotate_left_index(mps_blocks(self.slay[r], host(self.AL[r])), U), keep)
+        AR_r = cut_left_index(rotate_left_index(mps_blocks(self.slay[r], host(self.AR[r])), V), keep)
+        nrm = np.sqrt(sum(float(np.sum(S[q][:keep[q]] ** 2)) for q in S))
+        Cn = {q: np.diag(S[q][:keep[q]] / nrm) for q in S if keep[q] > 0}
+        qs = sorted(Cn)
+        self.bonds[b] = (np.asarray(qs, np.int32).reshape(len(qs), -1), np.asarray([keep[q] for q in qs], np.int32))
+        self.blay[b] = BlockLayout.diag(self.nsym, self.bonds[b][0], self.bonds[b][1])
+        for n in (l, r):
+            self.slay[n] = mps_structure(self.model, N, self.QN, self.bonds[n], self.bonds[self._b(n + 1)])
+        self.AL[l] = self._dev(flatten_mps(self.slay[l], AL_l))
+        self.AR[l] = self._dev(flatten_mps(self.slay[l], AR_l))
+        self.AL[r] = self._dev(flatten_mps(self.slay[r], AL_r))
+        self.AR[r] = self._dev(flatten_mps(self.slay[r], AR_r))
+        self.C[l] = self._dev(flatten_diag(self.blay[b], Cn))
+        self._plans = {}
+        self._units = {}
+        self.LE[b] = self._tm(LEFT, self.AL[l], l, self.LE[l], l, b) + self._optm_left(l - 1)
+        self.RE[b] = self._tm(RIGHT, self.AR[r], r, self.RE[self._b(r + 1)], self._b(r + 1), r) + self._optm_right(r)
+        self.prec_n[l] = self.prec_n[r] = 1.0  # the next solves start from the loose end of the tolerance range
+        return removed
+
     def _maybe_expand(self, exp_prec: float):
         """The growth policy of VUMPS_main.cpp:273-306, applied to every bond after an iteration."""
         from .expand import schmidt_values

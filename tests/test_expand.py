@@ -143,3 +143,59 @@ def test_schmidt_values_are_block_singular_values():
     lam = ex.schmidt_values(_lay(C), C.data)
     dense = _dense_diag(*c.bonds[1], ex.diag_blocks(_lay(C), C.data))
     np.testing.assert_allclose(np.sort(np.concatenate(list(lam.values()))), np.sort(np.linalg.svd(dense, compute_uv=False)), atol=1e-12)
+
+
+# ---- Truncate's selection (MPSBlockUtilities.hpp:846-906) ----------------------------------------
+def _brute_keep(lam, mmax, thresh):
+    allv = sorted(((v, q) for q, vs in lam.items() for v in vs if v > thresh), key=lambda t: -t[0])[:mmax]
+    keep = {q: 0 for q in lam}
+    for _, q in allv:
+        keep[q] += 1
+    return keep
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_truncate_sizes_match_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    lam = {(2 * i,): np.sort(rng.random(int(rng.integers(1, 9))) ** 3)[::-1] for i in range(-3, 4)}
+    for mmax in (1, 5, 17, 1000):
+        for thresh in (1e-10, 0.05, 0.3):
+            keep, changed = ex.truncate_sizes(lam, mmax, thresh)
+            assert keep == _brute_keep(lam, mmax, thresh)
+            n_all = sum(len(v) for v in lam.values())
+            assert changed == (sum(keep.values()) < n_all)
+
+
+def test_truncate_sizes_rules():
+    lam = {(0,): np.array([0.8, 0.3, 1e-13]), (2,): np.array([0.5, 1e-11]), (4,): np.array([1e-12])}
+    keep, changed = ex.truncate_sizes(lam, 10)  # only the values below lamthresh 1e-10 go
+    assert changed and keep == {(0,): 2, (2,): 1, (4,): 0}
+    keep, changed = ex.truncate_sizes(lam, 2)   # then the mmax largest: 0.8 and 0.5
+    assert keep == {(0,): 1, (2,): 1, (4,): 0}
+    keep, changed = ex.truncate_sizes({(0,): np.array([0.9, 0.1])}, 5)
+    assert not changed and keep == {(0,): 2}
+    # a tie at the cut keeps the lower charge first (the reference's sort leaves the order open)
+    keep, _ = ex.truncate_sizes({(0,): np.array([0.5]), (2,): np.array([0.5])}, 1)
+    assert keep == {(0,): 1, (2,): 0}
+
+
+def test_schmidt_basis_rotation_keeps_the_product():
+    rng = np.random.default_rng(3)
+    C = {(0,): rng.standard_normal((4, 4)), (2,): rng.standard_normal((2, 2))}
+    U, S, V = ex.schmidt_basis(C)
+    for q in C:
+        np.testing.assert_allclose(U[q] @ np.diag(S[q]) @ V[q].T, C[q], atol=1e-13)
+        assert np.all(np.diff(S[q]) <= 0)
+    A = [{(-1,): ((0,), rng.standard_normal((3, 4)))}, {(1,): ((2,), rng.standard_normal((5, 2)))}]
+    B = [{(0,): ((1,), rng.standard_normal((4, 6)))}, {(2,): ((3,), rng.standard_normal((2, 3)))}]
+    Ar, Br = ex.rotate_right_index(A, U), ex.rotate_left_index(B, V)
+    # A C B = (A U) S (V^T B) blockwise
+    for s in range(2):
+        for qi, (Q, a) in A[s].items():
+            for t in range(2):
+                if Q in B[t]:
+                    ref = a @ C[Q] @ B[t][Q][1]
+                    got = Ar[s][qi][1] @ np.diag(S[Q]) @ Br[t][Q][1]
+                    np.testing.assert_allclose(got, ref, atol=1e-12)
+    cut = ex.cut_right_index(Ar, {(0,): 2, (2,): 0})
+    assert cut[0][(-1,)][1].shape == (3, 2) and not cut[1]

@@ -240,3 +240,46 @@ def test_vumps_from_config_file(tmp_path):
     w = _small_xxz()
     w.load(str(files[0]))
     assert abs(w.energy() - v.energy()) <= 1e-12
+
+
+def test_truncate_keeps_largest_schmidt_values():
+    """Truncate (MPSBlockUtilities.hpp:846-936) on the bonds of the D = 64 Heisenberg state: bond 0
+    keeps the mmax largest Schmidt values (renormalised), the isometries next to it hold; with both
+    bonds truncated (a sector of one bond cannot exceed its stacked neighbours on the other, or the
+    QR gauge update has no isometry), VUMPS continues to an energy above the untruncated one."""
+    from imps_cpp_tools_amd.expand import diag_blocks, mps_blocks, schmidt_values
+    from imps_cpp_tools_amd.vumps import VUMPS
+    v = VUMPS(im.Context(0), im.XXZ, GOLD["N"], GOLD["QN"], np.array(GOLD["keys0"], np.int32), np.array(GOLD["sizes0"], np.int32),
+              GOLD["params"], seed=GOLD["seed"])
+    for _ in range(60):
+        st = v.sweep()
+    e_full = st.energy
+    N = GOLD["N"]
+    before = schmidt_values(v.blay[0], v.C[N - 1].cpu().numpy())
+    allv = np.sort(np.concatenate(list(before.values())))[::-1]
+    removed = v.truncate(0, 40)
+    assert removed == int(sum(len(x) for x in before.values())) - 40
+    assert v.bond_dims()[0] == 40
+    after = np.sort(np.concatenate(list(schmidt_values(v.blay[0], v.C[N - 1].cpu().numpy()).values())))[::-1]
+    np.testing.assert_allclose(after, allv[:40] / np.linalg.norm(allv[:40]), rtol=0, atol=1e-12)
+    # C is diagonal: the bond is in its Schmidt basis
+    for q, m in diag_blocks(v.blay[0], v.C[N - 1].cpu().numpy()).items():
+        assert np.abs(m - np.diag(np.diag(m))).max() == 0.0
+    # AL[N-1] stays left-isometric, AR[0] right-isometric on the truncated bond
+    for lay, data, left in ((v.slay[N - 1], v.AL[N - 1], True), (v.slay[0], v.AR[0], False)):
+        acc = {}
+        for ent in mps_blocks(lay, data.cpu().numpy()):
+            for qi, (qo, blk) in ent.items():
+                k, g = (qo, blk.T @ blk) if left else (qi, blk @ blk.T)
+                acc[k] = acc.get(k, 0) + g
+        for k, g in acc.items():
+            np.testing.assert_allclose(g, np.eye(g.shape[0]), atol=1e-12)
+    v.truncate(1, 40)
+    assert v.bond_dims() == [40, 40]
+    hist = []
+    for _ in range(150):
+        st = v.sweep()
+        hist.append((st.energy, max(st.F)))
+    print("full", e_full, "truncated then iterated", hist[-1], flush=True)
+    assert hist[-1][1] < 1e-6
+    assert hist[-1][0] > e_full - 1e-10
