@@ -99,6 +99,7 @@ def lib():
             "imps_hac_vjp": (I, [P, V, V, V, V, V, V, V, V, V, V, V]),
             "imps_hc_vjp": (I, [P, V, V, V, V, V, V, V, V, V, V, V]),
             "imps_heff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I, I]),
+            "imps_opeff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I, I]),
             "imps_heff_set_state": (I, [P, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), D]),
             "imps_heff_check": (I, [P, dp, dp, ip]),
             "imps_heff_apply": (I, [P, dp, V, V, ip, dp]),
@@ -619,18 +620,25 @@ class HeffPlan(Plan):
     AL, AR, NL: per-site MPS layouts; C: per-site bond layouts (C[n] right of site n); H: dense
     d^2 x d^2 two-site Hamiltonian; E0: ground-state energy per site (subtracted from H); K: charge;
     prec: FP64, or FP32 / BF16X3 transfer-matrix chains in the geometric sums' Arnoldi steps under
-    FP64 restarts.  Excitation vectors are complex128 CUDA tensors of shape (nmom, mtot) in XDims order."""
+    FP64 restarts.  Excitation vectors are complex128 CUDA tensors of shape (nmom, mtot) in XDims order.
+    With `observable` (a d x d one-site operator; H is then ignored and E0 is the observable's
+    ground-state expectation) the plan is the effective operator of that observable
+    (imps_opeff_plan: OPeffConstants / ApplyOPeff); `.apply` applies it and `.measure` gives
+    MeasureExcitations' per-site values."""
     kind = "heff"
 
     def __init__(self, ctx: Context, AL: Sequence[BlockLayout], AR: Sequence[BlockLayout], NL: Sequence[BlockLayout],
-                 C: Sequence[BlockLayout], H: np.ndarray, E0: float, K, nmom: int = 1, prec: int = FP64):
+                 C: Sequence[BlockLayout], H: Optional[np.ndarray], E0: float, K, nmom: int = 1, prec: int = FP64,
+                 observable: Optional[np.ndarray] = None):
         h = ctypes.c_void_p()
         N = len(AL)
         arr = [(_CLayout * N)(*[x.c() for x in L]) for L in (AL, AR, NL, C)]
         self._keep_layouts = (list(AL), list(AR), list(NL), list(C))
-        Hf = np.asfortranarray(H, dtype=np.float64).reshape(-1, order="F").copy()
+        op = H if observable is None else observable
+        Hf = np.asfortranarray(op, dtype=np.float64).reshape(-1, order="F").copy()
         Kq = np.ascontiguousarray(np.atleast_1d(K), np.int32)
-        _check(lib().imps_heff_plan(ctx.h, ctypes.byref(h), N, AL[0].nlist, *arr, _dptr(Hf), float(E0), _iptr(Kq), int(nmom), int(prec)))
+        entry = lib().imps_heff_plan if observable is None else lib().imps_opeff_plan
+        _check(entry(ctx.h, ctypes.byref(h), N, AL[0].nlist, *arr, _dptr(Hf), float(E0), _iptr(Kq), int(nmom), int(prec)))
         super().__init__(h, ctx)
         self.N, self.nmom = N, int(nmom)
         self.xdims = self._out_layout(AL[0].nsym)
@@ -671,6 +679,18 @@ class HeffPlan(Plan):
         st, rr = ctypes.c_int(), ctypes.c_double()
         _check(lib().imps_heff_apply(self.h, _dptr(pv), self._cp(x, n, "x"), self._cp(y, n, "y"), ctypes.byref(st), ctypes.byref(rr)))
         return st.value, rr.value
+
+    def measure(self, p, x):
+        """MeasureExcitations' values (Excitations_helpers.hpp:819-820) of an observable plan:
+        <x[t] restricted to site n | (O_eff x[t]) restricted to site n> as an (nmom, N) array."""
+        import torch
+        y = torch.empty_like(x)
+        self.apply(p, x, y)
+        out = np.zeros((self.nmom, self.N), complex)
+        for n in range(self.N):
+            a, b = self.xdims.entry_range(n)
+            out[:, n] = (x[:, a:b].conj() * y[:, a:b]).sum(1).cpu().numpy()
+        return out
 
     def eigs(self, p, nev: int = 4, tol: float = 1e-10, maxit: int = 0, krylov: int = 0, vectors: bool = False, check: bool = True):
         """Lowest nev eigenpairs per momentum: (energies (nmom, nev), vectors (nmom, nev, mtot) or

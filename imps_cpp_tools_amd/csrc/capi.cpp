@@ -501,6 +501,29 @@ imps_status imps_heff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, 
     });
 }
 
+imps_status imps_opeff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, const imps_layout* AL, const imps_layout* AR,
+                            const imps_layout* NL, const imps_layout* C, const double* O, double O0, const int* K, int nmom, int prec) {
+    return guarded([&] {
+        if (!plan || !AL || !AR || !NL || !C || !K || !O) throw InvalidArg("plan, layouts, O or K is NULL");
+        if (ncell < 1 || ncell > 64) throw InvalidArg("ncell out of range");
+        if (prec != IMPS_FP64 && prec != IMPS_FP32 && prec != IMPS_BF16X3) throw InvalidArg("prec must be IMPS_FP64, IMPS_FP32 or IMPS_BF16X3");
+        std::unique_ptr<imps_plan_s> P(new_plan(ctx, K_HEFF, IMPS_FP64));
+        std::vector<Layout> la, lr, ln, lc;
+        for (int k = 0; k < ncell; ++k) {
+            la.push_back(layout_from_c(&AL[k], "AL"));
+            lr.push_back(layout_from_c(&AR[k], "AR"));
+            ln.push_back(layout_from_c(&NL[k], "NL"));
+            lc.push_back(layout_from_c(&C[k], "C"));
+        }
+        if (ctx) hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        P->heff = std::make_shared<HeffEngine>();
+        P->heff->build(ctx, ncell, d, la, lr, ln, lc, O, O0, K, nmom, prec, 1);
+        P->out_layout = P->heff->Xin.lay;
+        *plan = P.release();
+        return IMPS_OK;
+    });
+}
+
 imps_status imps_heff_set_state(imps_plan P, const double* const* AL, const double* const* AR, const double* const* NL,
                                 const double* const* C, double inv_tol) {
     return guarded([&] {

@@ -404,9 +404,11 @@ XObj HeffEngine::invert_e(const std::vector<XObj>& As, const std::vector<XObj>& 
 
 void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<Layout>& al, const std::vector<Layout>& ar,
                        const std::vector<Layout>& nl, const std::vector<Layout>& cl, const double* H, double E0, const int* Kq, int nmomenta,
-                       int precision) {
+                       int precision, int sites) {
     if (!c) throw InvalidArg("heff: a device context is required");
+    if (sites != 1 && sites != 2) throw InvalidArg("heff: the operator acts on one or two sites");
     prec = precision;
+    op_sites = sites;
     if (ncell < 1 || dphys < 1) throw InvalidArg("heff: bad cell size or physical dimension");
     if ((int)al.size() != ncell || (int)ar.size() != ncell || (int)nl.size() != ncell || (int)cl.size() != ncell)
         throw InvalidArg("heff: one layout per site is needed for AL, AR, NL and C");
@@ -430,9 +432,24 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     NLl = nl;
     Cl = cl;
     K = Key(Kq, nsym);
-    const int D = d * d;
+    const int D = op_sites == 2 ? d * d : d;
     H0.assign(H, H + (size_t)D * D);
     for (int i = 0; i < D; ++i) H0[i + (size_t)i * D] -= E0;
+    if (op_sites == 1) {
+        // a one-site observable must keep the block structure: O(r, c) != 0 only between physical
+        // indices whose blocks connect the same charges
+        auto keys = [](const Layout& L, int e) {
+            std::vector<std::pair<Key, Key>> k;
+            for (const Blk& b : L.l[e].b) k.push_back({b.qin, b.qout});
+            return k;
+        };
+        for (int r = 0; r < d; ++r)
+            for (int c2 = 0; c2 < d; ++c2)
+                if (r != c2 && H0[r + (size_t)c2 * d] != 0.0)
+                    for (int n = 0; n < N; ++n)
+                        if (keys(al[n], r) != keys(al[n], c2) || keys(ar[n], r) != keys(ar[n], c2))
+                            throw InvalidArg("heff: the observable couples physical indices of different charge (it must conserve the charges)");
+    }
     // XDims (Excitations_helpers.hpp:639-678): X[n] blocks Qout(NL[n]) -> + K with sizes from NL's
     // columns and AR[n+1]'s rows
     xdims.clear();
@@ -486,7 +503,8 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     rec = &setup_tape;
     record_setup();
     rec = &apply_tape;
-    record_apply();
+    if (op_sites == 2) record_apply();
+    else record_apply_observable();
     rec = nullptr;
     finalize();
     fac_dev.ensure(sizeof(double) * 2 * nmom * std::max<size_t>(geos.size(), 1));
@@ -495,15 +513,26 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
 }
 
 void HeffEngine::record_setup() {
-    // HeffConstants (Excitations_helpers.hpp:260-347) with E0 already subtracted from H
+    // HeffConstants (Excitations_helpers.hpp:260-347) with E0 already subtracted from H, or, for a
+    // one-site observable, OPeffConstants (:531-600) with its expectation subtracted: the same
+    // environment solves on the one-site sums OPL / OPR
     auto P = [this](int x) { return ((x % N) + N) % N; };
-    auto opl = [this](const XObj& AA) { return diag_part(prod(PROD_TM_LEFT, apply_operator(AA), AA)); };
-    auto opr = [this](const XObj& AA) { return diag_part(prod(PROD_TM_RIGHT, apply_operator(AA), AA)); };
     std::vector<XObj> HL(N), HR(N);
-    HL[0] = opl(prod(PROD_MPS_MPS, AL[N - 1], AL[0]));
-    for (int n = 1; n < N; ++n) HL[n] = add(opl(prod(PROD_MPS_MPS, AL[n - 1], AL[n])), diag_part(tml(AL[n], AL[n], HL[n - 1])));
-    HR[N - 1] = opr(prod(PROD_MPS_MPS, AR[N - 1], AR[0]));
-    for (int n = N - 2; n >= 0; --n) HR[n] = add(opr(prod(PROD_MPS_MPS, AR[n], AR[n + 1])), diag_part(tmr(AR[n], AR[n], HR[n + 1])));
+    if (op_sites == 2) {
+        auto opl = [this](const XObj& AA) { return diag_part(prod(PROD_TM_LEFT, apply_operator(AA), AA)); };
+        auto opr = [this](const XObj& AA) { return diag_part(prod(PROD_TM_RIGHT, apply_operator(AA), AA)); };
+        HL[0] = opl(prod(PROD_MPS_MPS, AL[N - 1], AL[0]));
+        for (int n = 1; n < N; ++n) HL[n] = add(opl(prod(PROD_MPS_MPS, AL[n - 1], AL[n])), diag_part(tml(AL[n], AL[n], HL[n - 1])));
+        HR[N - 1] = opr(prod(PROD_MPS_MPS, AR[N - 1], AR[0]));
+        for (int n = N - 2; n >= 0; --n) HR[n] = add(opr(prod(PROD_MPS_MPS, AR[n], AR[n + 1])), diag_part(tmr(AR[n], AR[n], HR[n + 1])));
+    } else {
+        auto opl = [this](const XObj& A1) { return diag_part(prod(PROD_TM_LEFT, apply_site_operator(A1), A1)); };
+        auto opr = [this](const XObj& A1) { return diag_part(prod(PROD_TM_RIGHT, apply_site_operator(A1), A1)); };
+        HL[0] = opl(AL[0]);
+        for (int n = 1; n < N; ++n) HL[n] = add(opl(AL[n]), diag_part(tml(AL[n], AL[n], HL[n - 1])));
+        HR[N - 1] = opr(AR[N - 1]);
+        for (int n = N - 2; n >= 0; --n) HR[n] = add(opr(AR[n]), diag_part(tmr(AR[n], AR[n], HR[n + 1])));
+    }
     const Layout& b0 = Cl[N - 1];
     const XObj Rr = prod(PROD_MPS_MATT, C[N - 1], C[N - 1]);  // C C^T
     const XObj Ll = prod(PROD_MATT_MPS, C[N - 1], C[N - 1]);  // C^T C
@@ -548,12 +577,70 @@ void HeffEngine::record_setup() {
     }
     HLr.clear();
     HRr.clear();
+    AAL.clear();
+    if (op_sites == 1) return;
     for (int n = 0; n < N; ++n) {
         HLr.push_back(redop(AL[n], IMPS_LEFT));
         HRr.push_back(redop(AR[n], IMPS_RIGHT));
     }
-    AAL.clear();
     for (int n = 0; n < N; ++n) AAL.push_back(prod(PROD_MPS_MPS, AL[P(n - 1)], AL[n]));
+}
+
+XObj HeffEngine::apply_site_operator(const XObj& A1) {
+    if ((int)A1.lay.l.size() != d) throw InvalidArg("heff: a one-site operator needs a one-site object");
+    std::vector<std::vector<Src>> rows(d);
+    for (int r = 0; r < d; ++r)
+        for (int c2 = 0; c2 < d; ++c2) {
+            const double o = H0[r + (size_t)c2 * d];
+            if (o != 0.0) rows[r].push_back(Src{coef(o, 0), &A1, c2});
+        }
+    return comb(rows);
+}
+
+void HeffEngine::record_apply_observable() {
+    // ApplyOPeff (Excitations_helpers.hpp:350-528) for B[n] = NL[n] X[n] and the one-site operator
+    // H0 (expectation subtracted); OPLtot / OPRtot are in HLtot / HRtot.  The reference skips the
+    // left geometric sum when the norm of its source is below 10 InvETol; here it always runs (a
+    // zero source gives a zero solution at once), which changes the result by less than that.
+    auto P = [this](int x) { return ((x % N) + N) % N; };
+    std::vector<XObj> Bin(N);
+    for (int n = 0; n < N; ++n) Bin[n] = prod(PROD_MPS_MAT, NL[n], Xin_v[n]);
+    // terms where Bin is right of Bout (as in ApplyHeff)
+    std::vector<XObj> ABR(N);
+    ABR[N - 1] = prod(PROD_TM_RIGHT, Bin[N - 1], AR[N - 1]);
+    for (int n = N - 1; n > 0; --n) ABR[n - 1] = prod(PROD_TM_RIGHT, add(Bin[n - 1], prod(PROD_MPS_MAT, AL[n - 1], ABR[n])), AR[n - 1]);
+    XObj EBR = invert_e(AL, AR, ABR[0], IMPS_RIGHT, +1, &LMo, &RMo);
+    std::vector<XObj> ABRtot(N);
+    ABRtot[0] = scale(EBR, 1.0, +1);
+    for (int n = N - 1; n > 0; --n) {
+        EBR = tmr(AL[n], AR[n], EBR);
+        ABRtot[n] = add(ABR[n], EBR, 1.0, 1.0, +1);
+    }
+    // terms where Bin is left of Bout: OPBL and its geometric sum
+    std::vector<XObj> OPBL(N);
+    OPBL[0] = add(prod(PROD_TM_LEFT, prod(PROD_MAT_MPS, HLtot[N - 1], Bin[0]), AL[0]), prod(PROD_TM_LEFT, apply_site_operator(Bin[0]), AL[0]));
+    for (int n = 1; n < N; ++n)
+        OPBL[n] = add(prod(PROD_TM_LEFT, add(prod(PROD_MAT_MPS, OPBL[n - 1], AR[n]), prod(PROD_MAT_MPS, HLtot[n - 1], Bin[n])), AL[n]),
+                      prod(PROD_TM_LEFT, apply_site_operator(Bin[n]), AL[n]));
+    XObj EOPBL = invert_e(AR, AL, OPBL[N - 1], IMPS_LEFT, -1, &RMo, &LMo);
+    // Bin and Bout on the same site
+    std::vector<XObj> Bout(N);
+    for (int n = 0; n < N; ++n) {
+        XObj b = add(prod(PROD_MAT_MPS, HLtot[P(n - 1)], Bin[n]), prod(PROD_MPS_MAT, Bin[n], HRtot[P(n + 1)]));
+        Bout[n] = add(b, apply_site_operator(Bin[n]));
+    }
+    // Bin left of Bout
+    Bout[0] = add(Bout[0], prod(PROD_MAT_MPS, scale(EOPBL, 1.0, -1), AR[0]));
+    for (int n = 1; n < N; ++n) {
+        EOPBL = tml(AR[n - 1], AL[n - 1], EOPBL);
+        Bout[n] = add(Bout[n], prod(PROD_MAT_MPS, add(OPBL[n - 1], EOPBL, 1.0, 1.0, -1), AR[n]));
+    }
+    // Bin right of Bout
+    for (int n = 0; n < N; ++n) {
+        Bout[n] = add(Bout[n], prod(PROD_MAT_MPS, HLtot[P(n - 1)], prod(PROD_MPS_MAT, AL[n], ABRtot[P(n + 1)])));
+        Bout[n] = add(Bout[n], prod(PROD_MPS_MAT, apply_site_operator(AL[n]), ABRtot[P(n + 1)]));
+    }
+    for (int n = 0; n < N; ++n) onto(prod(PROD_TM_LEFT, Bout[n], NL[n]), Xin_v[n].lay, &Xout_v[n]);
 }
 
 void HeffEngine::record_apply() {

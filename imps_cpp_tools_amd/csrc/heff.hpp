@@ -133,6 +133,10 @@ struct HeffEngine : BlockTape {
     // IMPS_FP64, or IMPS_FP32 / IMPS_BF16X3: the Arnoldi steps of the geometric sums apply their TM
     // chains at that precision; every restart takes the true FP64 residual (iterative refinement)
     int prec = IMPS_FP64;
+    // 2: H_eff of the two-site Hamiltonian (H0 is d^2 x d^2); 1: the effective operator of a one-site
+    // observable (ApplyOPeff with OPeffConstants; H0 is the d x d operator, HLtot / HRtot hold
+    // OPLtot / OPRtot)
+    int op_sites = 2;
 
     // ---- objects ----
     std::vector<XObj> AL, AR, NL, C;  // constants, filled by set_state
@@ -175,7 +179,7 @@ struct HeffEngine : BlockTape {
     // build: layouts (AL, AR, NL per site; C[n] on bond n + 1), H (dense d^2 x d^2 column-major), E0, K
     void build(imps_context c, int ncell, int dphys, const std::vector<Layout>& al, const std::vector<Layout>& ar,
                const std::vector<Layout>& nl, const std::vector<Layout>& cl, const double* H, double E0, const int* Kq, int nmomenta,
-               int precision = IMPS_FP64);
+               int precision = IMPS_FP64, int sites = 2);
     void set_state(const std::vector<const double*>& al, const std::vector<const double*>& ar, const std::vector<const double*>& nl,
                    const std::vector<const double*>& cl, double tol);
     // y = H_eff(p_t) x for every momentum t (interleaved complex [nmom][mtot] device vectors)
@@ -192,6 +196,10 @@ struct HeffEngine : BlockTape {
                   const XObj* R);
     void record_setup();
     void record_apply();
+    void record_apply_observable();
+    // sum_c O(r, c) A[c] per physical index r for the one-site operator H0 (ApplyOperator with a
+    // one-site operator, MPSBlockUtilities.hpp:18-34)
+    XObj apply_site_operator(const XObj& A);
     void run_geo(GeoOp& op);
     void run_other(const Op& op) override;
 };

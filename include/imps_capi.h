@@ -266,6 +266,17 @@ imps_status imps_qr_apply(imps_plan plan, const double* A, double* Q, double* R)
 imps_status imps_heff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, const imps_layout* AL, const imps_layout* AR,
                            const imps_layout* NL, const imps_layout* C, const double* H, double E0, const int* K, int nmom,
                            int prec);
+/* Replaces OPeffConstants + ApplyOPeff (Excitations_helpers.hpp:350-600), the excitation observables
+ * of MeasureExcitations (:774-826): a plan of the same kind as imps_heff_plan's for the effective
+ * operator of a one-site observable O (d x d column-major, charge-conserving: O(r, c) != 0 only
+ * between physical indices whose blocks connect the same charges) with O0 (its ground-state
+ * expectation, MeasureExcitations' eobs) subtracted.  imps_heff_set_state computes OPLtot and OPRtot
+ * through two environment solves, imps_heff_check returns the reference's checks tr(OPL R) and
+ * tr(L OPR), imps_heff_apply applies O_eff(p_t) to the nmom momenta at once; <X_n | (O_eff X)_n>
+ * per site n is MeasureExcitations' value.  The reference skips the left geometric sum when its
+ * source is below 10 InvETol in norm; this plan always solves it. */
+imps_status imps_opeff_plan(imps_context ctx, imps_plan* plan, int ncell, int d, const imps_layout* AL, const imps_layout* AR,
+                            const imps_layout* NL, const imps_layout* C, const double* O, double O0, const int* K, int nmom, int prec);
 imps_status imps_heff_set_state(imps_plan plan, const double* const* AL, const double* const* AR, const double* const* NL,
                                 const double* const* C, double inv_tol);
 imps_status imps_heff_check(imps_plan plan, double* trHL_R, double* trL_HR, int* env_iters);
