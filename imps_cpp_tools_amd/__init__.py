@@ -23,7 +23,18 @@ LIB_PATH = os.path.join(HERE, "libimps_hip.so")
 FP64, FP32, BF16X3 = 0, 1, 2
 LEFT, RIGHT = 0, 1
 PREC_NAMES = {FP64: "fp64", FP32: "fp32", BF16X3: "bf16x3"}
-XXZ, FHUB = 0, 1
+XXZ, FHUB, QIM = 0, 1, 2
+
+
+def QPOTTS(q: int) -> int:
+    """Model id of the q-state quantum Potts chain (Z(q) charges)."""
+    return 3 | (int(q) << 8)
+
+
+def BHUB(d: int) -> int:
+    """Model id of the Bose-Hubbard chain with local dimension d (U(1) charge)."""
+    return 4 | (int(d) << 8)
+
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "NOT_CONVERGED", 3: "NUMERICAL", 4: "HIP_ERROR", 5: "NO_DEVICE"}
 
@@ -37,7 +48,8 @@ class ImpsError(RuntimeError):
 class _CLayout(ctypes.Structure):
     _fields_ = [("nsym", ctypes.c_int), ("nlist", ctypes.c_int), ("nblk", ctypes.POINTER(ctypes.c_int)),
                 ("qin", ctypes.POINTER(ctypes.c_int)), ("qout", ctypes.POINTER(ctypes.c_int)),
-                ("rows", ctypes.POINTER(ctypes.c_int)), ("cols", ctypes.POINTER(ctypes.c_int))]
+                ("rows", ctypes.POINTER(ctypes.c_int)), ("cols", ctypes.POINTER(ctypes.c_int)),
+                ("periods", ctypes.POINTER(ctypes.c_int))]
 
 
 _lib = None
@@ -106,6 +118,7 @@ def lib():
             "imps_heff_eigs": (I, [P, dp, I, D, I, I, dp, V, dp, ip]),
             "imps_model_nsym": (I, [I]),
             "imps_model_d": (I, [I]),
+            "imps_model_periods": (I, [I, ip]),
             "imps_model_i2k": (I, [I, I, ip, ip]),
             "imps_model_validpath": (I, [I, I, ip, ip]),
             "imps_model_dimmap": (I, [I, I, ip, ip, I, ip, ip]),
@@ -138,15 +151,21 @@ def _dptr(a: np.ndarray):
 @dataclass
 class BlockLayout:
     """Structure of a block-sparse object: nlist entries (1 = BlockMat/BlockDiagMat, d = MPS,
-    d*d = RedOp), each a list of blocks in ascending ingoing charge."""
+    d*d = RedOp), each a list of blocks in ascending ingoing charge.  `periods`: the symmetry group
+    (per charge 0 for U(1), N for Z(N)); None means U(1) for every charge."""
     nsym: int
     nblk: np.ndarray
     qin: np.ndarray
     qout: np.ndarray
     rows: np.ndarray
     cols: np.ndarray
+    periods: Optional[np.ndarray] = None
 
     def __post_init__(self):
+        per = np.zeros(self.nsym, np.int32) if self.periods is None else np.ascontiguousarray(self.periods, dtype=np.int32).reshape(-1)
+        if per.shape != (self.nsym,):
+            raise ValueError(f"periods: {per.size} entries for {self.nsym} charges")
+        self.periods = np.where(per == 1, 0, per).astype(np.int32)  # period 1 is U(1), as in symobj::init
         self.nblk = np.ascontiguousarray(self.nblk, dtype=np.int32).reshape(-1)
         nb = int(self.nblk.sum())
         self.qin = np.ascontiguousarray(self.qin, dtype=np.int32).reshape(nb, self.nsym)
@@ -187,7 +206,7 @@ class BlockLayout:
     def entry(self, e: int) -> "BlockLayout":
         s = int(self.nblk[:e].sum())
         n = int(self.nblk[e])
-        return BlockLayout(self.nsym, [n], self.qin[s:s + n], self.qout[s:s + n], self.rows[s:s + n], self.cols[s:s + n])
+        return BlockLayout(self.nsym, [n], self.qin[s:s + n], self.qout[s:s + n], self.rows[s:s + n], self.cols[s:s + n], self.periods)
 
     def entry_range(self, e: int):
         """Element range [a, b) of list entry e in the flat buffer."""
@@ -197,27 +216,35 @@ class BlockLayout:
         return a, a + int(self.sizes[s:s + n].sum())
 
     def c(self) -> _CLayout:
-        self._keep = (self.nblk, self.qin, self.qout, self.rows, self.cols)
-        return _CLayout(self.nsym, self.nlist, _iptr(self.nblk), _iptr(self.qin), _iptr(self.qout), _iptr(self.rows), _iptr(self.cols))
+        self._keep = (self.nblk, self.qin, self.qout, self.rows, self.cols, self.periods)
+        return _CLayout(self.nsym, self.nlist, _iptr(self.nblk), _iptr(self.qin), _iptr(self.qout), _iptr(self.rows), _iptr(self.cols),
+                        _iptr(self.periods))
 
     def __eq__(self, o):
-        return (self.nsym == o.nsym and np.array_equal(self.nblk, o.nblk) and np.array_equal(self.qin, o.qin)
-                and np.array_equal(self.qout, o.qout) and np.array_equal(self.rows, o.rows) and np.array_equal(self.cols, o.cols))
+        return (self.nsym == o.nsym and np.array_equal(self.periods, o.periods) and np.array_equal(self.nblk, o.nblk)
+                and np.array_equal(self.qin, o.qin) and np.array_equal(self.qout, o.qout) and np.array_equal(self.rows, o.rows)
+                and np.array_equal(self.cols, o.cols))
 
     @staticmethod
     def concat(parts: Sequence["BlockLayout"]) -> "BlockLayout":
         return BlockLayout(parts[0].nsym, np.concatenate([p.nblk for p in parts]), np.concatenate([p.qin for p in parts]),
                            np.concatenate([p.qout for p in parts]), np.concatenate([p.rows for p in parts]),
-                           np.concatenate([p.cols for p in parts]))
+                           np.concatenate([p.cols for p in parts]), parts[0].periods)
 
     @staticmethod
-    def diag(nsym: int, keys: np.ndarray, sizes: np.ndarray) -> "BlockLayout":
+    def diag(nsym: int, keys: np.ndarray, sizes: np.ndarray, periods=None) -> "BlockLayout":
         keys = np.asarray(keys, dtype=np.int32).reshape(-1, nsym)
-        return BlockLayout(nsym, [len(keys)], keys, keys, sizes, sizes)
+        return BlockLayout(nsym, [len(keys)], keys, keys, sizes, sizes, periods)
 
 
 # ----------------------------------------------------------------------------------------------
 # bookkeeping (host only)
+def _qn(QN) -> np.ndarray:
+    """The quantum numbers as int32 (the Z(q) models take none: one placeholder entry)."""
+    q = np.asarray(list(QN), np.int32).reshape(-1)
+    return q if q.size else np.zeros(1, np.int32)
+
+
 def model_nsym(model: int) -> int:
     return lib().imps_model_nsym(model)
 
@@ -226,10 +253,19 @@ def model_d(model: int) -> int:
     return lib().imps_model_d(model)
 
 
+def model_periods(model: int) -> np.ndarray:
+    """The model's symmetry group: per charge 0 (U(1)) or N (Z(N))."""
+    out = np.zeros(4, np.int32)
+    n = lib().imps_model_periods(model, _iptr(out))
+    if n < 0:
+        raise ImpsError(1, "imps_model_periods: unknown model")
+    return out[:n].copy()
+
+
 def i2k(model: int, N: int, QN: Sequence[int]) -> np.ndarray:
     ns, d = model_nsym(model), model_d(model)
     out = np.zeros(d * ns, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     r = lib().imps_model_i2k(model, N, _iptr(q), _iptr(out))
     if r < 0:
         raise ImpsError(1, "imps_model_i2k failed")
@@ -238,7 +274,7 @@ def i2k(model: int, N: int, QN: Sequence[int]) -> np.ndarray:
 
 def validpath(model: int, N: int, QN: Sequence[int]) -> np.ndarray:
     out = np.zeros(N, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     if lib().imps_model_validpath(model, N, _iptr(q), _iptr(out)) < 0:
         raise ImpsError(1, "imps_model_validpath failed")
     return out
@@ -246,10 +282,10 @@ def validpath(model: int, N: int, QN: Sequence[int]) -> np.ndarray:
 
 def dimmap(model: int, N: int, QN: Sequence[int], dims: Sequence[int]):
     ns = model_nsym(model)
-    nmax = len(dims) ** (1 if model == XXZ else 2)
+    nmax = len(dims) ** (2 if model == FHUB else 1)
     keys = np.zeros(nmax * ns, np.int32)
     sizes = np.zeros(nmax, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     dd = np.asarray(dims, np.int32)
     n = lib().imps_model_dimmap(model, N, _iptr(q), _iptr(dd), len(dd), _iptr(keys), _iptr(sizes))
     if n < 0:
@@ -265,7 +301,7 @@ def make_dims(model: int, N: int, QN: Sequence[int], keys0: np.ndarray, sizes0: 
     nsec = len(s0)
     keys = np.zeros(N * nsec * ns, np.int32)
     sizes = np.zeros(N * nsec, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     if lib().imps_model_make_dims(model, N, _iptr(q), nsec, _iptr(k0), _iptr(s0), _iptr(keys), _iptr(sizes)) < 0:
         raise ImpsError(1, "imps_model_make_dims failed")
     keys = keys.reshape(N, nsec, ns)
@@ -292,12 +328,12 @@ def mps_structure(model: int, N: int, QN: Sequence[int], left, right) -> BlockLa
     qout = np.zeros(cap * ns, np.int32)
     rows = np.zeros(cap, np.int32)
     cols = np.zeros(cap, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     n = lib().imps_mps_structure(model, N, _iptr(q), len(sl), _iptr(kl.reshape(-1)), _iptr(sl), len(sr), _iptr(kr.reshape(-1)),
                                  _iptr(sr), _iptr(nblk), _iptr(qin), _iptr(qout), _iptr(rows), _iptr(cols))
     if n < 0:
         raise ImpsError(1, "imps_mps_structure failed")
-    return BlockLayout(ns, nblk, qin[:n * ns], qout[:n * ns], rows[:n], cols[:n])
+    return BlockLayout(ns, nblk, qin[:n * ns], qout[:n * ns], rows[:n], cols[:n], model_periods(model))
 
 
 # ----------------------------------------------------------------------------------------------
@@ -361,7 +397,7 @@ class Plan:
     def _p(self, t, n: int, what: str):
         return _ptr(t, n, self.device, f"{self.kind} {what}")
 
-    def _out_layout(self, nsym: int) -> BlockLayout:
+    def _out_layout(self, nsym: int, periods=None) -> BlockLayout:
         nl, nb, ne = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
         z = ctypes.POINTER(ctypes.c_int)()
         _check(lib().imps_plan_out_layout(self.h, ctypes.byref(nl), ctypes.byref(nb), ctypes.byref(ne), z, z, z, z, z))
@@ -372,7 +408,7 @@ class Plan:
         rows = np.zeros(max(n, 1), np.int32)
         cols = np.zeros(max(n, 1), np.int32)
         _check(lib().imps_plan_out_layout(self.h, None, None, None, _iptr(nblk), _iptr(qin), _iptr(qout), _iptr(rows), _iptr(cols)))
-        return BlockLayout(nsym, nblk, qin[:n * nsym], qout[:n * nsym], rows[:n], cols[:n])
+        return BlockLayout(nsym, nblk, qin[:n * nsym], qout[:n * nsym], rows[:n], cols[:n], periods)
 
     def cost(self):
         f, b = ctypes.c_double(), ctypes.c_double()
@@ -514,7 +550,7 @@ class RedOpPlan(Plan):
         cA = A.c()
         _check(lib().imps_redop_plan(_ctx_handle(ctx), ctypes.byref(h), _dptr(Hf), d, ctypes.byref(cA), direction, prec))
         super().__init__(h, ctx)
-        self.out_layout = self._out_layout(A.nsym)
+        self.out_layout = self._out_layout(A.nsym, A.periods)
         self.alen = A.nelem
 
     def apply(self, A, y):
@@ -582,7 +618,7 @@ class ProdPlan(Plan):
         c1, c2 = X1.c(), X2.c()
         _check(lib().imps_prod_plan(_ctx_handle(ctx), ctypes.byref(h), kind, ctypes.byref(c1), ctypes.byref(c2)))
         super().__init__(h, ctx)
-        self.out_layout = self._out_layout(X1.nsym)
+        self.out_layout = self._out_layout(X1.nsym, X1.periods)
         self.len1, self.len2 = X1.nelem, X2.nelem
 
     def apply(self, x1, x2, y):
@@ -605,7 +641,7 @@ class QRPlan(Plan):
         cA = A.c()
         _check(lib().imps_qr_plan(_ctx_handle(ctx), ctypes.byref(h), ctypes.byref(cA), direction))
         super().__init__(h, ctx)
-        self.out_layout = self._out_layout(A.nsym)
+        self.out_layout = self._out_layout(A.nsym, A.periods)
         self.alen = A.nelem
 
     def apply(self, A, Q, R):
@@ -641,7 +677,7 @@ class HeffPlan(Plan):
         _check(entry(ctx.h, ctypes.byref(h), N, AL[0].nlist, *arr, _dptr(Hf), float(E0), _iptr(Kq), int(nmom), int(prec)))
         super().__init__(h, ctx)
         self.N, self.nmom = N, int(nmom)
-        self.xdims = self._out_layout(AL[0].nsym)
+        self.xdims = self._out_layout(AL[0].nsym, AL[0].periods)
         self.mtot = self.xdims.nelem
 
     def set_state(self, AL, AR, NL, C, inv_tol: float = 1e-14):

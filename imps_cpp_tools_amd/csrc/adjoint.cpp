@@ -134,6 +134,7 @@ std::shared_ptr<VjpTape> compile(imps_plan_s* P) {
     T->nmom = 1;
     const std::vector<Layout>& L = P->lays;
     T->nsym = L.empty() ? 1 : L[0].nsym;
+    if (!L.empty()) for (int i = 0; i < kMaxSym; ++i) T->per[i] = L[0].per[i];
     T->coef(1.0, 0);
     T->rec = &T->tape;
     if (P->kind == K_TM) {

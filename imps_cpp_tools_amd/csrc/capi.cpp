@@ -312,6 +312,8 @@ imps_status imps_env_plan(imps_context ctx, imps_plan* plan, int dir, int ncell,
         std::vector<Layout> LA;
         for (int k = 0; k < ncell; ++k) LA.push_back(layout_from_c(&A[k], "A"));
         Layout lx = layout_from_c(x, "x");
+        for (auto& l : LA) require_same_nsym(LA[0], l, "env");
+        require_same_nsym(LA[0], lx, "env");
         P->env = std::make_shared<EnvSolver>();
         P->env->build(ctx, dir, LA, lx, prec);
         P->op_set = true;

@@ -41,9 +41,10 @@ bool same_layout(const Layout& a, const Layout& b) {
 }
 
 // Layout from per-entry maps qin -> block (offsets assigned in entry, then key order)
-Layout from_maps(int nsym, const std::vector<std::map<Key, Blk>>& ents) {
+Layout from_maps(int nsym, const int* per, const std::vector<std::map<Key, Blk>>& ents) {
     Layout L;
     L.nsym = nsym;
+    for (int i = 0; i < kMaxSym; ++i) L.per[i] = per[i];
     L.l.resize(ents.size());
     long pos = 0;
     for (size_t e = 0; e < ents.size(); ++e)
@@ -65,7 +66,7 @@ long entry_start(const Layout& L, int e) {
 
 Layout entry_layout(const Layout& L, int e) {
     Layout out;
-    out.nsym = L.nsym;
+    set_group(out, L);
     out.l.resize(1);
     const long base = entry_start(L, e);
     long pos = 0;
@@ -232,7 +233,7 @@ XObj BlockTape::comb(const std::vector<std::vector<Src>>& rows, const Layout* fi
                     else if (it->second.qout != b.qout || it->second.rows != b.rows || it->second.cols != b.cols)
                         throw InvalidArg("heff: sum of blocks with different shapes or outgoing charges");
                 }
-        lay = from_maps(nsym, ents);
+        lay = from_maps(nsym, per, ents);
     }
     XObj y;
     if (dest) {
@@ -306,7 +307,7 @@ XObj BlockTape::diag_part(const XObj& a) {
     for (size_t e = 0; e < ents.size(); ++e)
         for (const Blk& b : a.lay.l[e].b)
             if (b.qin == b.qout) ents[e].emplace(b.qin, b);
-    return onto(a, from_maps(nsym, ents));
+    return onto(a, from_maps(nsym, per, ents));
 }
 
 XObj BlockTape::apply_operator(const XObj& AA) {
@@ -420,6 +421,7 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     d = dphys;
     nmom = nmomenta;
     nsym = al[0].nsym;
+    for (int i = 0; i < kMaxSym; ++i) per[i] = al[0].per[i];
     for (int n = 0; n < N; ++n) {
         if ((int)al[n].l.size() != d || (int)ar[n].l.size() != d || (int)nl[n].l.size() != d)
             throw InvalidArg("heff: MPS layouts need nlist = d");
@@ -431,7 +433,12 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     ARl = ar;
     NLl = nl;
     Cl = cl;
-    K = Key(Kq, nsym);
+    {
+        // the excitation's charge, as a residue for Z(N) charges
+        int kq[kMaxSym] = {0, 0, 0, 0};
+        for (int i = 0; i < nsym; ++i) kq[i] = kred(Kq[i], per[i]);
+        K = Key(kq, nsym, per);
+    }
     const int D = op_sites == 2 ? d * d : d;
     H0.assign(H, H + (size_t)D * D);
     for (int i = 0; i < D; ++i) H0[i + (size_t)i * D] -= E0;
@@ -466,7 +473,7 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
             auto it = mr.find(qk);
             if (it != mr.end() && kv.second > 0 && it->second > 0) ent[0].emplace(kv.first, Blk{kv.first, qk, kv.second, it->second, 0});
         }
-        Layout L = from_maps(nsym, ent);
+        Layout L = from_maps(nsym, per, ent);
         mtot += L.nelem;
         xdims.push_back(L);
     }
@@ -482,7 +489,7 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     {
         std::vector<std::map<Key, Blk>> ents(N);
         Layout all;
-        all.nsym = nsym;
+        set_group(all, al[0]);
         all.l.resize(N);
         long pos = 0;
         for (int n = 0; n < N; ++n)

@@ -58,6 +58,7 @@ struct CoefSpec {
 struct BlockTape {
     imps_context ctx = nullptr;
     int nsym = 1, d = 1, nmom = 1;
+    int per[kMaxSym] = {0, 0, 0, 0};  // charge periods of the layouts (0: U(1), N > 1: Z(N))
     std::vector<double> H0;  // d^2 x d^2 column-major two-site operator used by apply_operator/redop
 
     long arena_elems = 0;

@@ -544,7 +544,7 @@ Layout bond_layout(const Layout& A, bool right) {
             else if (it->second != sz) throw InvalidArg("env: inconsistent sector sizes on a bond");
         }
     Layout L;
-    L.nsym = A.nsym;
+    set_group(L, A);
     L.l.resize(1);
     long pos = 0;
     for (auto& kv : m) {

@@ -30,17 +30,28 @@ constexpr int kTile = 32;  // output tile edge of the grouped GEMM kernel (one w
 
 struct InvalidArg : std::runtime_error { using std::runtime_error::runtime_error; };
 
-// Charge key: nsym U(1) charges compared lexicographically (std::vector<int> order, KeyTypes.hpp:19-58).
+// Residue of x modulo a period p > 1 in [0, p); p = 0 (U(1)) leaves x as it is.
+inline int kred(int x, int p) { return p > 1 ? ((x % p) + p) % p : x; }
+
+// Charge key: nsym abelian charges compared lexicographically (std::vector<int> order,
+// KeyTypes.hpp:19-58).  Each charge has a period (symobj.hpp:80-161): 0 for U(1), N > 1 for Z(N),
+// whose charges live in [0, N).  Sums and differences reduce with the left operand's periods, as
+// the reference's KeyType operators use the left key's group object (KeyTypes.hpp:133-151).
 struct Key {
     int n = 0;
     int q[kMaxSym] = {0, 0, 0, 0};
+    int per[kMaxSym] = {0, 0, 0, 0};
     Key() {}
     Key(const int* p, int nsym) : n(nsym) { for (int i = 0; i < nsym; ++i) q[i] = p[i]; }
+    Key(const int* p, int nsym, const int* periods) : n(nsym) {
+        for (int i = 0; i < nsym; ++i) { per[i] = periods ? periods[i] : 0; q[i] = p[i]; }
+    }
     bool operator<(const Key& o) const { for (int i = 0; i < n; ++i) if (q[i] != o.q[i]) return q[i] < o.q[i]; return false; }
     bool operator==(const Key& o) const { for (int i = 0; i < n; ++i) if (q[i] != o.q[i]) return false; return true; }
     bool operator!=(const Key& o) const { return !(*this == o); }
-    Key operator+(const Key& o) const { Key r(*this); for (int i = 0; i < n; ++i) r.q[i] += o.q[i]; return r; }
-    Key operator-(const Key& o) const { Key r(*this); for (int i = 0; i < n; ++i) r.q[i] -= o.q[i]; return r; }
+    Key operator+(const Key& o) const { Key r(*this); for (int i = 0; i < n; ++i) r.q[i] = kred(q[i] + o.q[i], per[i]); return r; }
+    Key operator-(const Key& o) const { Key r(*this); for (int i = 0; i < n; ++i) r.q[i] = kred(q[i] - o.q[i], per[i]); return r; }
+    bool canonical() const { for (int i = 0; i < n; ++i) if (per[i] > 1 && (q[i] < 0 || q[i] >= per[i])) return false; return true; }
 };
 
 struct Blk {
@@ -63,6 +74,7 @@ struct BList {
 
 struct Layout {
     int nsym = 1;
+    int per[kMaxSym] = {0, 0, 0, 0};  // charge periods (0: U(1), N > 1: Z(N))
     std::vector<BList> l;
     long nelem = 0;
     size_t nblocks() const { size_t n = 0; for (auto& x : l) n += x.b.size(); return n; }
@@ -75,6 +87,11 @@ inline Layout layout_from_c(const imps_layout* c, const char* what) {
     if (!c->nblk) throw InvalidArg(std::string(what) + ": nblk is NULL");
     Layout L;
     L.nsym = c->nsym;
+    for (int i = 0; i < c->nsym; ++i) {
+        const int p = c->periods ? c->periods[i] : 0;
+        if (p < 0) throw InvalidArg(std::string(what) + ": negative charge period");
+        L.per[i] = p > 1 ? p : 0;  // period 1 means U(1), as in symobj::init
+    }
     L.l.resize(c->nlist);
     long pos = 0;
     size_t k = 0;
@@ -83,8 +100,9 @@ inline Layout layout_from_c(const imps_layout* c, const char* what) {
         if (c->nblk[e] > 0 && (!c->qin || !c->qout || !c->rows || !c->cols)) throw InvalidArg(std::string(what) + ": NULL block arrays");
         for (int i = 0; i < c->nblk[e]; ++i, ++k) {
             Blk b;
-            b.qin = Key(c->qin + k * c->nsym, c->nsym);
-            b.qout = Key(c->qout + k * c->nsym, c->nsym);
+            b.qin = Key(c->qin + k * c->nsym, c->nsym, L.per);
+            b.qout = Key(c->qout + k * c->nsym, c->nsym, L.per);
+            if (!b.qin.canonical() || !b.qout.canonical()) throw InvalidArg(std::string(what) + ": a Z(N) charge outside [0, N)");
             b.rows = c->rows[k];
             b.cols = c->cols[k];
             if (b.rows < 0 || b.cols < 0) throw InvalidArg(std::string(what) + ": negative block size");
@@ -97,6 +115,18 @@ inline Layout layout_from_c(const imps_layout* c, const char* what) {
     }
     L.nelem = pos;
     return L;
+}
+
+inline bool same_group(const Layout& a, const Layout& b) {
+    if (a.nsym != b.nsym) return false;
+    for (int i = 0; i < a.nsym; ++i) if (a.per[i] != b.per[i]) return false;
+    return true;
+}
+
+// dst takes src's symmetry group (charge count and periods)
+inline void set_group(Layout& dst, const Layout& src) {
+    dst.nsym = src.nsym;
+    for (int i = 0; i < kMaxSym; ++i) dst.per[i] = src.per[i];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -177,7 +207,7 @@ inline void emit_stage(Program& P, std::vector<OutBlock>& obs, int out_slot) {
 }
 
 inline void require_same_nsym(const Layout& a, const Layout& b, const char* what) {
-    if (a.nsym != b.nsym) throw InvalidArg(std::string(what) + ": layouts disagree on nsym");
+    if (!same_group(a, b)) throw InvalidArg(std::string(what) + ": layouts disagree on the symmetry group (nsym or charge periods)");
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -421,7 +451,7 @@ inline Program build_prod(int kind, const Layout& X1, const Layout& X2, Layout& 
     P.x_elems = X1.nelem;
     P.op_elems = X2.nelem;
     out = Layout();
-    out.nsym = X1.nsym;
+    set_group(out, X1);
     long pos = 0;
     std::vector<OutBlock> obs;
     auto entry_offset = [](const Layout& L, size_t e) { return L.l[e].b.empty() ? 0L : L.l[e].b.front().off; };
@@ -579,7 +609,7 @@ inline Program build_redop(const double* H, int d, const Layout& A, int dir, Lay
             }
         }
     out = Layout();
-    out.nsym = A.nsym;
+    set_group(out, A);
     out.l.resize(D);
     long pos = 0;
     std::vector<OutBlock> obs;

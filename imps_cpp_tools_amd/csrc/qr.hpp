@@ -56,7 +56,7 @@ inline QrPlanData build_qr(const Layout& A, int dir) {
     for (size_t s = 0; s < A.l.size(); ++s)
         for (const Blk& b : A.l[s].b) secs[dir == IMPS_LEFT ? b.qout : b.qin].push_back(Part{(int)s, &b});
     QrPlanData P;
-    P.R.nsym = A.nsym;
+    set_group(P.R, A);
     P.R.l.resize(1);
     long rpos = 0;
     for (auto& kv : secs) {

@@ -87,8 +87,11 @@ def schmidt_values(lay: BlockLayout, data: np.ndarray) -> Dict[Key, np.ndarray]:
     return {q: np.linalg.svd(m, compute_uv=False) for q, m in diag_blocks(lay, data).items()}
 
 
-def _add(a: Key, b) -> Key:
-    return tuple(int(x) + int(y) for x, y in zip(a, b))
+def _add(a: Key, b, periods=None) -> Key:
+    """Charge sum, reduced per Z(N) charge (period N > 1) to [0, N)."""
+    if periods is None:
+        return tuple(int(x) + int(y) for x, y in zip(a, b))
+    return tuple((int(x) + int(y)) % int(p) if p > 1 else int(x) + int(y) for x, y, p in zip(a, b, periods))
 
 
 def _null_rows(A: np.ndarray) -> np.ndarray:
@@ -100,7 +103,7 @@ def _null_rows(A: np.ndarray) -> np.ndarray:
     return U[:, m:]
 
 
-def expand_from_h(H: np.ndarray, dq: np.ndarray, AL, C: Dict[Key, np.ndarray], AR, dm_max: int):
+def expand_from_h(H: np.ndarray, dq: np.ndarray, AL, C: Dict[Key, np.ndarray], AR, dm_max: int, periods=None):
     """One ExpandFromH step on the bond of C (host data from mps_blocks / diag_blocks).
 
     H: two-site Hamiltonian (d^2 x d^2, pair index s = s_l d + s_r); dq[s]: charge shift of
@@ -108,7 +111,7 @@ def expand_from_h(H: np.ndarray, dq: np.ndarray, AL, C: Dict[Key, np.ndarray], A
     right-orthonormal right tensor; C: {Q: block} of the bond.  Returns (dm, new_cols, new_rows, lam):
     dm = {Q: added states} (in ascending charge order), new_cols[s] = {q_l: columns appended to
     AL's block (q_l -> Q)}, new_rows[s] = {q_r_in: rows appended to AR's block (Q -> q_r)} and lam =
-    {Q: all singular values of the sector's matrix}."""
+    {Q: all singular values of the sector's matrix}.  periods: the symmetry group (None: U(1))."""
     d = len(AL)
     dq = [tuple(int(v) for v in row) for row in np.asarray(dq).reshape(d, -1)]
     # left and right bond sector sizes
@@ -140,7 +143,7 @@ def expand_from_h(H: np.ndarray, dq: np.ndarray, AL, C: Dict[Key, np.ndarray], A
                         continue
                     for ql, (qr, m) in psi[(tl, tr)].items():
                         # H conserves charge: the pair (sl, sr) maps ql to the same qr
-                        if _add(_add(ql, dq[sl]), dq[sr]) != qr or qr not in mr:
+                        if _add(_add(ql, dq[sl], periods), dq[sr], periods) != qr or qr not in mr:
                             continue
                         ent = phi.setdefault((sl, sr), {})
                         if ql in ent:
@@ -151,7 +154,7 @@ def expand_from_h(H: np.ndarray, dq: np.ndarray, AL, C: Dict[Key, np.ndarray], A
     centers: Dict[Key, Dict[str, Dict[int, Key]]] = {}
     for (sl, sr), ent in phi.items():
         for ql, (qr, _) in ent.items():
-            Q = _add(ql, dq[sl])
+            Q = _add(ql, dq[sl], periods)
             c = centers.setdefault(Q, {"l": {}, "r": {}})
             c["l"][sl] = ql
             c["r"][sr] = qr

@@ -132,16 +132,16 @@ def _mps_items(lay: BlockLayout, data) -> List[List[Tuple[Key, Key, np.ndarray]]
     return out
 
 
-def diag_from_items(nsym: int, items: List[Tuple[Key, np.ndarray]]) -> Tuple[BlockLayout, np.ndarray]:
+def diag_from_items(nsym: int, items: List[Tuple[Key, np.ndarray]], periods=None) -> Tuple[BlockLayout, np.ndarray]:
     items = sorted(items, key=lambda t: t[0])
     lay = BlockLayout(nsym, [len(items)], np.asarray([k for k, _ in items], np.int32).reshape(-1, nsym),
                       np.asarray([k for k, _ in items], np.int32).reshape(-1, nsym), [m.shape[0] for _, m in items],
-                      [m.shape[1] for _, m in items])
+                      [m.shape[1] for _, m in items], periods)
     data = np.concatenate([m.reshape(-1, order="F") for _, m in items]) if items else np.zeros(0)
     return lay, data
 
 
-def mps_from_items(nsym: int, items: List[List[Tuple[Key, Key, np.ndarray]]]) -> Tuple[BlockLayout, np.ndarray]:
+def mps_from_items(nsym: int, items: List[List[Tuple[Key, Key, np.ndarray]]], periods=None) -> Tuple[BlockLayout, np.ndarray]:
     nblk, qin, qout, rows, cols, parts = [], [], [], [], [], []
     for ent in items:
         ent = sorted(ent, key=lambda t: t[0])
@@ -150,7 +150,7 @@ def mps_from_items(nsym: int, items: List[List[Tuple[Key, Key, np.ndarray]]]) ->
             qin.append(ki), qout.append(ko), rows.append(m.shape[0]), cols.append(m.shape[1])
             parts.append(m.reshape(-1, order="F"))
     nb = sum(nblk)
-    lay = BlockLayout(nsym, nblk, np.asarray(qin, np.int32).reshape(nb, nsym), np.asarray(qout, np.int32).reshape(nb, nsym), rows, cols)
+    lay = BlockLayout(nsym, nblk, np.asarray(qin, np.int32).reshape(nb, nsym), np.asarray(qout, np.int32).reshape(nb, nsym), rows, cols, periods)
     return lay, (np.concatenate(parts) if parts else np.zeros(0))
 
 
@@ -193,9 +193,10 @@ def write_imps(path: str, lam: List[Dict[Key, np.ndarray]], C: List[Tuple[BlockL
                         _w_mat(f, m)
 
 
-def read_imps(path: str, nsym: int):
+def read_imps(path: str, nsym: int, periods=None):
     """loadIMPS: returns (lam, C, AL, AR) in write_imps' form (layouts rebuilt from the stored
-    sectors, nsym charges per key)."""
+    sectors, nsym charges per key; `periods`: the reader's symmetry group, which the file does not
+    hold)."""
     with open(path, "rb") as f:
         _r_tag(f, "BLOCKLAMARRAY")
         lam = []
@@ -214,7 +215,7 @@ def read_imps(path: str, nsym: int):
             for _ in range(_r_u32(f)):
                 k = _r_key(f)
                 items.append((k, _r_mat(f)))
-            C.append(diag_from_items(nsym, items))
+            C.append(diag_from_items(nsym, items, periods))
         arrs = []
         for _ in range(2):
             _r_tag(f, "RBLOCKMPSARRAY")
@@ -233,7 +234,7 @@ def read_imps(path: str, nsym: int):
                         ko = _r_key(f)
                         ent.append((ki, ko, _r_mat(f)))
                     items.append(ent)
-                arr.append(mps_from_items(nsym, items))
+                arr.append(mps_from_items(nsym, items, periods))
             arrs.append(arr)
     if any(len(k) != nsym for site in lam for k in site):
         raise ValueError("IMPS file: charge keys do not have nsym entries")

@@ -41,7 +41,7 @@ def left_nullspace_layout(lay: BlockLayout) -> BlockLayout:
         b += int(lay.nblk[e])
     nb = sum(nblk)
     return BlockLayout(lay.nsym, nblk, np.asarray(qin, np.int32).reshape(nb, lay.nsym), np.asarray(qout, np.int32).reshape(nb, lay.nsym),
-                       rows, cols)
+                       rows, cols, lay.periods)
 
 
 def left_nullspace(ctx: Context, A, lay: BlockLayout, seed: int = 0) -> Tuple[object, BlockLayout]:

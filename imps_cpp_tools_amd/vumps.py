@@ -32,9 +32,9 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
-from . import (BF16X3, FP32, FP64, LEFT, RIGHT, MAT_MPS, MATT_MPS, MPS_MAT, MPS_MATT, MPS_REDOP, REDOP_MPS, TM_LEFT_UNITY,
+from . import (BF16X3, FP32, FP64, LEFT, RIGHT, MAT_MPS, MATT_MPS, MPS_MAT, MPS_MATT, MPS_MPS, MPS_REDOP, REDOP_MPS, TM_LEFT_UNITY,
                TM_RIGHT_UNITY, BlockLayout, Context, EnvPlan, HACPlan, HCPlan, ProdPlan, QRPlan, RedOpPlan, TMPlan, eigsh,
-               make_dims, model_d, model_ham, model_nsym, mps_structure)
+               make_dims, model_d, model_ham, model_nsym, model_periods, mps_structure)
 from .streams import side_streams
 
 
@@ -92,8 +92,9 @@ class VUMPS:
         self.concurrent = concurrent
         self._side = None  # (contexts of the two H_C solves, thread pool), made on first use
         self.d, self.nsym = model_d(model), model_nsym(model)
+        self.periods = model_periods(model)
         self.bonds = make_dims(model, N, QN, keys0, sizes0)
-        self.blay = [BlockLayout.diag(self.nsym, k, s) for (k, s) in self.bonds]
+        self.blay = [BlockLayout.diag(self.nsym, k, s, self.periods) for (k, s) in self.bonds]
         self.slay = [mps_structure(model, N, QN, self.bonds[n], self.bonds[(n + 1) % N]) for n in range(N)]
         self.H = model_ham(model, params)
         self.eig_tol_max, self.eig_tol_min, self.env_tol, self.env_fac, self.krylov = eig_tol_max, eig_tol_min, env_tol, env_fac, krylov
@@ -557,12 +558,12 @@ class VUMPS:
         AL_r = mps_blocks(self.slay[r], host(self.AL[r]))
         Cb = diag_blocks(self.blay[b], host(self.C[l]))
         dq = i2k(self.model, N, self.QN)
-        add, new_cols, new_rows, _ = expand_from_h(self.H, dq, AL_l, Cb, AR_r, dm)
+        add, new_cols, new_rows, _ = expand_from_h(self.H, dq, AL_l, Cb, AR_r, dm, self.periods)
         if not add:
             return 0
         keys, sizes = grow_bond(self.bonds[b][0], self.bonds[b][1], add)
         self.bonds[b] = (keys, sizes)
-        self.blay[b] = BlockLayout.diag(self.nsym, keys, sizes)
+        self.blay[b] = BlockLayout.diag(self.nsym, keys, sizes, self.periods)
         for n in (l, r):
             self.slay[n] = mps_structure(self.model, N, self.QN, self.bonds[n], self.bonds[self._b(n + 1)])
         self.AL[l] = self._dev(flatten_mps(self.slay[l], append_columns(AL_l, new_cols)))
@@ -607,7 +608,7 @@ class VUMPS:
         Cn = {q: np.diag(S[q][:keep[q]] / nrm) for q in S if keep[q] > 0}
         qs = sorted(Cn)
         self.bonds[b] = (np.asarray(qs, np.int32).reshape(len(qs), -1), np.asarray([keep[q] for q in qs], np.int32))
-        self.blay[b] = BlockLayout.diag(self.nsym, self.bonds[b][0], self.bonds[b][1])
+        self.blay[b] = BlockLayout.diag(self.nsym, self.bonds[b][0], self.bonds[b][1], self.periods)
         for n in (l, r):
             self.slay[n] = mps_structure(self.model, N, self.QN, self.bonds[n], self.bonds[self._b(n + 1)])
         self.AL[l] = self._dev(flatten_mps(self.slay[l], AL_l))
@@ -641,27 +642,95 @@ class VUMPS:
                 self.expand_ct[b] += 1
 
     # ------------------------------------------------------------------ observables
-    # diagonal one-site operators of the models (Models.cpp:57-75 XXZ, :546-560 Fermi-Hubbard;
-    # local states of the Hubbard chain: empty, down, up, double)
-    DIAG_OPS = {0: {"sz": [0.5, -0.5]},
-                1: {"nup": [0.0, 0.0, 1.0, 1.0], "ndo": [0.0, 1.0, 0.0, 1.0], "sz": [0.0, -0.5, 0.5, 0.0],
-                    "n": [0.0, 1.0, 1.0, 2.0]}}
+    @staticmethod
+    def model_operators(model: int) -> Dict[str, np.ndarray]:
+        """The one-site operators of the models' GetObservables, as d x d matrices: XXZ and QIM
+        (XYZAbstractModel, Models.cpp:57-145: sx, syi, sz, rx = 2 sx, rz = 2 sz, sp = sx - syi,
+        sm = sx + syi), Fermi-Hubbard (:575-635: cup, cdo, cupd, cdod, nup, ndo, sz, n, rx, rz; local
+        states empty, down, up, double), quantum Potts (:432-470: X, Z) and Bose-Hubbard (:779-828:
+        b, b2, n, nmax)."""
+        t, d = model & 0xFF, model_d(model)
+        if t in (0, 2):
+            sx = np.array([[0.0, 0.5], [0.5, 0.0]])
+            syi = np.array([[0.0, -0.5], [0.5, 0.0]])
+            sz = np.diag([0.5, -0.5])
+            return {"sx": sx, "syi": syi, "sz": sz, "rx": 2 * sx, "rz": 2 * sz, "sp": sx - syi, "sm": sx + syi}
+        if t == 1:
+            cup, cdo = np.zeros((4, 4)), np.zeros((4, 4))
+            cup[0, 2] = cup[1, 3] = 1.0
+            cdo[0, 1], cdo[2, 3] = 1.0, -1.0
+            nup, ndo = np.diag([0.0, 0.0, 1.0, 1.0]), np.diag([0.0, 1.0, 0.0, 1.0])
+            rx = np.zeros((4, 4))
+            rx[0, 0], rx[1, 2], rx[2, 1], rx[3, 3] = 1.0, 1.0, 1.0, -1.0
+            return {"cup": cup, "cdo": cdo, "cupd": cup.T.copy(), "cdod": cdo.T.copy(), "nup": nup, "ndo": ndo,
+                    "sz": 0.5 * (nup - ndo), "n": nup + ndo, "rx": rx, "rz": np.diag([1.0, -1.0, 1.0, -1.0])}
+        if t == 3:
+            X, Z = np.zeros((d, d)), np.diag([d - 1.0] + [-1.0] * (d - 1))
+            X[d - 1, 0] = 1.0
+            for i in range(1, d):
+                X[i - 1, i] = 1.0
+            return {"X": X, "Z": Z}
+        if t == 4:
+            b = np.zeros((d, d))
+            for i in range(1, d):
+                b[i - 1, i] = np.sqrt(i)
+            nmax = np.zeros((d, d))
+            nmax[d - 1, d - 1] = 1.0
+            return {"b": b, "b2": b @ b, "n": np.diag(np.arange(d, dtype=float)), "nmax": nmax}
+        raise ValueError(f"unknown model {model}")
 
-    def observables(self, names) -> Dict[str, np.ndarray]:
-        """MeasureObservables (MPSBlockUtilities.hpp:1017-1073) for one-site operators that conserve
-        the charges (diagonal in the local basis): <O>_n = sum_s O_ss ||AC[n]_s||^2 / ||AC[n]||^2 with
-        AC[n] = C[n-1] AR[n], for every site n of the cell.  Charge-changing operators have zero
-        expectation value in these symmetric states and are not offered."""
-        ops = self.DIAG_OPS[self.model]
+    @staticmethod
+    def _gram(lay, data: np.ndarray) -> np.ndarray:
+        """G[s, t] = <A_s | A_t> over the blocks that A_s and A_t share (same ingoing and outgoing
+        charges); blocks of different charges are orthogonal, so a charge-changing operator meets
+        zeros, as the reference's trace of a charged BlockMat is zero (BlockObj.hpp:1340-1350)."""
+        from .expand import mps_blocks
+        blk = mps_blocks(lay, data)
+        n = len(blk)
+        G = np.zeros((n, n))
+        for a in range(n):
+            for b in range(a, n):
+                v = 0.0
+                for q, (qo, m) in blk[a].items():
+                    e = blk[b].get(q)
+                    if e is not None and e[0] == qo:
+                        v += float(np.vdot(m, e[1]))
+                G[a, b] = G[b, a] = v
+        return G
+
+    def observables(self, ops) -> Dict[str, np.ndarray]:
+        """MeasureObservables (MPSBlockUtilities.hpp:1017-1073) in the center gauge: for every site n
+        of the cell, <O>_n = sum_{s,t} O[s, t] <A_s | A_t> / sum_s <A_s | A_s> with A = AC[n] =
+        C[n-1] AR[n] for a one-site operator (d x d) and A = AC[n] AR[n+1] (pair index s = s_l d + s_r)
+        for a two-site one (d^2 x d^2, e.g. the Hamiltonian, whose values are the bond energies).
+        `ops`: names of the model's operators (model_operators) or a dict {name: matrix}.  Operators
+        that change the charges have zero expectation value in these symmetric states."""
+        table = self.model_operators(self.model)
+        if isinstance(ops, dict):
+            items = list(ops.items())
+        else:
+            items = []
+            for name in ops:
+                if name not in table:
+                    raise ValueError(f"operator {name} not defined (operators of this model: {', '.join(table)})")
+                items.append((name, table[name]))
         out = {}
-        for name in names:
-            if name not in ops:
-                raise ValueError(f"operator {name} not defined (charge-conserving one-site operators: {', '.join(ops)})")
+        grams = {}
+        for name, O in items:
+            O = np.asarray(O, np.float64)
+            sites = {self.d: 1, self.d * self.d: 2}.get(O.shape[0])
+            if O.shape != (O.shape[0], O.shape[0]) or sites is None:
+                raise ValueError(f"operator {name}: shape {O.shape}, expected d x d or d^2 x d^2 (d = {self.d})")
             vals = np.zeros(self.N)
             for n in range(self.N):
-                AC, _ = self._prod(MAT_MPS, self.blay[n], self.C[self._b(n - 1)], self.slay[n], self.AR[n], ("obsAC", n))
-                w = np.array([float(AC[a:b].square().sum().item()) for a, b in (self.slay[n].entry_range(e) for e in range(self.d))])
-                vals[n] = float(np.dot(ops[name], w) / w.sum())
+                if (sites, n) not in grams:
+                    AC, lay = self._prod(MAT_MPS, self.blay[n], self.C[self._b(n - 1)], self.slay[n], self.AR[n], ("obsAC", n))
+                    if sites == 2:
+                        m = self._b(n + 1)
+                        AC, lay = self._prod(MPS_MPS, lay, AC, self.slay[m], self.AR[m], ("obsAA", n))
+                    grams[(sites, n)] = self._gram(lay, AC.cpu().numpy())
+                G = grams[(sites, n)]
+                vals[n] = float(np.sum(O * G) / np.trace(G))
             out[name] = vals
         return out
 
@@ -683,7 +752,7 @@ class VUMPS:
         (blocks the file leaves out are zero); environments and plans are rebuilt."""
         from .expand import diag_blocks, flatten_diag, flatten_mps, mps_blocks
         from .imps_io import read_imps
-        _, C, AL, AR = read_imps(path, self.nsym)
+        _, C, AL, AR = read_imps(path, self.nsym, self.periods)
         N = self.N
         if not (len(C) == len(AL) == len(AR) == N):
             raise ValueError(f"state file holds a {len(AL)}-site cell, this VUMPS has {N} sites")
@@ -691,7 +760,7 @@ class VUMPS:
         for n in range(N):
             lay = C[n][0]
             bonds[self._b(n + 1)] = (np.array(lay.qin, np.int32), np.array(lay.rows, np.int32))
-        blay = [BlockLayout.diag(self.nsym, k, s) for (k, s) in bonds]
+        blay = [BlockLayout.diag(self.nsym, k, s, self.periods) for (k, s) in bonds]
         slay = [mps_structure(self.model, N, self.QN, bonds[n], bonds[self._b(n + 1)]) for n in range(N)]
 
         def embed(lay_new, lay_old, data):

@@ -7,12 +7,12 @@ Format (parser.cpp:4-46): every line or argument of the form `--key=value` sets 
 comments included, is ignored); files are read in order, then the command-line arguments, later
 settings replacing earlier ones.  Vectors are written `[a,b,c]`.
 
-Keys (VUMPS_main.cpp:51-69, Models.cpp:940-1025): model (XXZ | FHUB), N (>= 2), QN, m0 (initial
-dims per sector, GetDimMap), mmax (one value or one per bond), thresh (stop when max F is below),
-dmmax, exthresh, lamthresh, expand_pause, InvETol, InvEfac, save, folder, obs (charge-conserving
-one-site operators: XXZ sz; FHUB n, nup, ndo, sz); XXZ: J (1), Delta, hz (0);
-FHUB: t (1), U, V (0), mu (0), hz (0).  Extra keys of this runner: maxit (outer iterations, 1000),
-prec (fp64 | fp32 | bf16x3), seed.
+Keys (VUMPS_main.cpp:51-69, Models.cpp:881-1114): model (XXZ | QIM | QPOTTS | BHUB | FHUB), N (>= 2),
+QN, m0 (initial dims per sector, GetDimMap), mmax (one value or one per bond), thresh (stop when max F
+is below), dmmax, exthresh, lamthresh, expand_pause, InvETol, InvEfac, save, folder, obs (one-site
+operators of the model's GetObservables); XXZ: J (1), Delta, hz (0); QIM: J (1), hz; QPOTTS: q, h
+(0); BHUB: d, t (1), U, mu (0); FHUB: t (1), U, V (0), mu (0), hz (0).  Extra keys of this runner:
+maxit (outer iterations, 1000), prec (fp64 | fp32 | bf16x3), seed.
 """
 from __future__ import annotations
 
@@ -20,7 +20,20 @@ import os
 import sys
 from typing import Dict, Iterable, List, Optional
 
-MODELS = {"XXZ": 0, "FHUB": 1}
+MODELS = ("XXZ", "FHUB", "QIM", "QPOTTS", "BHUB")
+PARAM_NAMES = {"XXZ": ("J", "Delta", "hz"), "FHUB": ("t", "U", "V", "mu", "hz"), "QIM": ("J", "h"), "QPOTTS": ("h",),
+               "BHUB": ("t", "U", "mu")}
+
+
+def model_id(name: str, dim: Optional[int] = None) -> int:
+    """The library's model id of a CreateModel name (the local dimension q or d for QPOTTS / BHUB)."""
+    import imps_cpp_tools_amd as im
+    ids = {"XXZ": im.XXZ, "FHUB": im.FHUB, "QIM": im.QIM}
+    if name in ids:
+        return ids[name]
+    if dim is None:
+        raise ValueError(f"{name} needs its local dimension")
+    return im.QPOTTS(dim) if name == "QPOTTS" else im.BHUB(dim)
 
 
 def parse_lines(lines: Iterable[str], values: Optional[Dict[str, str]] = None) -> Dict[str, str]:
@@ -75,8 +88,19 @@ def settings(values: Dict[str, str]) -> Dict:
     N = _get(values, "N", int, required=True)
     if N < 2:
         raise ValueError("unit cell size must be at least 2")
+    dim = None
     if model == "XXZ":
         params = [_get(values, "J", float, 1.0), _get(values, "Delta", float, required=True), _get(values, "hz", float, 0.0)]
+        if _get(values, "d", int, 2) != 2:
+            raise ValueError("XXZ: only spin-1/2 (d = 2) is built")
+    elif model == "QIM":
+        params = [_get(values, "J", float, 1.0), _get(values, "hz", float, required=True)]
+    elif model == "QPOTTS":
+        dim = _get(values, "q", int, required=True)
+        params = [_get(values, "h", float, 0.0)]
+    elif model == "BHUB":
+        dim = _get(values, "d", int, required=True)
+        params = [_get(values, "t", float, 1.0), _get(values, "U", float, required=True), _get(values, "mu", float, 0.0)]
     else:
         params = [_get(values, "t", float, 1.0), _get(values, "U", float, required=True), _get(values, "V", float, 0.0),
                   _get(values, "mu", float, 0.0), _get(values, "hz", float, 0.0)]
@@ -86,7 +110,8 @@ def settings(values: Dict[str, str]) -> Dict:
     elif len(mmax) != N:
         raise ValueError("mmax needs one value or one per bond")
     return dict(
-        model=model, N=N, params=params, QN=_get(values, "QN", lambda s: _vector(s, int), required=True),
+        model=model, dim=dim, N=N, params=params,
+        QN=_get(values, "QN", lambda s: _vector(s, int), required=model not in ("QIM", "QPOTTS"), default=[]),
         m0=_get(values, "m0", lambda s: _vector(s, int), required=True), mmax=mmax,
         thresh=_get(values, "thresh", float, 1e-14), dmmax=_get(values, "dmmax", int, 20),
         exthresh=_get(values, "exthresh", float, 1e-5), lamthresh=_get(values, "lamthresh", float, 1e-12),
@@ -102,7 +127,7 @@ def run(cfg: Dict, ctx=None, log=print):
     bond growth up to mmax; returns the VUMPS object and the list of per-iteration stats."""
     import imps_cpp_tools_amd as im
     from .vumps import VUMPS
-    model = MODELS[cfg["model"]]
+    model = model_id(cfg["model"], cfg.get("dim"))
     keys0, sizes0 = im.dimmap(model, cfg["N"], cfg["QN"], cfg["m0"])
     prec = {"fp64": im.FP64, "fp32": im.FP32, "bf16x3": im.BF16X3}[cfg["prec"]]
     ctx = im.Context(0) if ctx is None else ctx
@@ -121,8 +146,8 @@ def run(cfg: Dict, ctx=None, log=print):
             log(f"{name}: {' '.join(f'{x:.12f}' for x in vals)}  mean: {vals.mean():.12f}")
     if cfg["save"]:
         os.makedirs(cfg["folder"], exist_ok=True)
-        name = "_".join([f"UMPS_{cfg['model']}"] + [f"{k}{p:g}" for k, p in zip(("J", "Delta", "hz") if cfg["model"] == "XXZ"
-                                                                                  else ("t", "U", "V", "mu", "hz"), cfg["params"])]
+        dimtag = [f"{'q' if cfg['model'] == 'QPOTTS' else 'd'}{cfg['dim']}"] if cfg.get("dim") else []
+        name = "_".join([f"UMPS_{cfg['model']}"] + dimtag + [f"{k}{p:g}" for k, p in zip(PARAM_NAMES[cfg["model"]], cfg["params"])]
                         + [f"N{cfg['N']}", f"chi{max(v.bond_dims())}"]) + ".bin"
         path = os.path.join(cfg["folder"], name)
         v.save(path)

@@ -50,15 +50,18 @@ typedef enum { IMPS_LEFT = 0, IMPS_RIGHT = 1 } imps_dir;  /* dirtype l / r (arma
 /* Block structure of a BlockMat (nlist = 1), BlockDiagMat (nlist = 1, qin == qout, square blocks),
  * MPSBlockMat (nlist = d) or RedOp (nlist = d*d, row-major grid, OperatorTypes.hpp:160-187).
  * Within each list entry blocks are in strictly ascending qin order (lexicographic over the nsym
- * charges), which is the std::map order of the reference. */
+ * charges), which is the std::map order of the reference.  `periods` is the symmetry group
+ * (symobj::init, symobj.hpp:80-161): per charge 0 or 1 for U(1), N > 1 for Z(N), whose charges
+ * must lie in [0, N); NULL means U(1) for every charge.  All layouts of one plan share the group. */
 typedef struct {
-    int nsym;          /* number of U(1) charges per sector key, 1..4 */
+    int nsym;          /* number of abelian charges per sector key, 1..4 */
     int nlist;         /* list entries */
     const int* nblk;   /* [nlist] blocks per entry */
     const int* qin;    /* [sum(nblk) * nsym] ingoing charges */
     const int* qout;   /* [sum(nblk) * nsym] outgoing charges */
     const int* rows;   /* [sum(nblk)] */
     const int* cols;   /* [sum(nblk)] */
+    const int* periods; /* [nsym] charge periods, or NULL (all U(1)) */
 } imps_layout;
 
 typedef struct imps_context_s* imps_context;
@@ -302,16 +305,31 @@ imps_status imps_heff_eigs(imps_plan plan, const double* p, int nev, double tol,
                            double* resid, int* napply);
 
 /* ---- bookkeeping (A1/A2) -------------------------------------------------------------------
- * model: 0 = XXZ spin-1/2 (one U(1)), 1 = Fermi-Hubbard in (N,M) representation (U(1)xU(1)).
- * imps_model_i2k: GetI2K (Models.cpp:262-295, :699-733); writes d*nsym charges, returns d or -1.
+ * model (CreateModel's types, Models.cpp:881-1114): IMPS_MODEL_XXZ spin-1/2 (one U(1));
+ * IMPS_MODEL_FHUB Fermi-Hubbard in (N,M) representation (U(1)xU(1)); IMPS_MODEL_QIM quantum Ising
+ * (Z(2)); IMPS_MODEL_QPOTTS(q) q-state quantum Potts (Z(q)); IMPS_MODEL_BHUB(d) Bose-Hubbard with
+ * local dimension d (U(1)).  An id is type | (d << 8); 2 <= d <= 16 for QPOTTS and BHUB.
+ * imps_model_nsym / imps_model_d: charges per key and local dimension, -1 for an unknown id.
+ * imps_model_periods: the symmetry group (GroupObj_.init of each model's Init): writes nsym
+ * periods (0 = U(1), N = Z(N)) for imps_layout.periods; returns nsym or -1.
+ * imps_model_i2k: GetI2K (Models.cpp:262-295, :372-376, :500-506, :699-733, :862-878); writes
+ * d*nsym charges, returns d or -1.
  * imps_model_validpath: validpath (ItoKey.hpp:126-165); writes N indices.
  * imps_model_make_dims: MakeDims (Models.cpp:13-32) from an explicit bond-0 dim map
  * (keys0 ascending); writes N bonds of nsec (keys, sizes). Returns nsec or -1.
- * imps_model_dimmap: GetDimMap (Models.cpp:248-260, :680-697) from the per-sector dims list.
- * imps_model_ham: local two-site Hamiltonian (Models.cpp:57-93, :543-572), dense d^2 x d^2
- * column-major; params XXZ {J, Delta, hz}, FHUB {t, U, V, mu, hz}. */
+ * imps_model_dimmap: GetDimMap (Models.cpp:248-260, :357-370, :478-498, :680-697, :845-860) from
+ * the per-sector dims list.  QN: XXZ {2 Sz per N sites}, FHUB {n, m}, BHUB {n}, QIM / QPOTTS unused.
+ * imps_model_ham: local two-site Hamiltonian (Models.cpp:57-93, :411-430, :543-572, :768-777),
+ * dense d^2 x d^2 column-major; params XXZ {J, Delta, hz}, FHUB {t, U, V, mu, hz}, QIM {J, h},
+ * QPOTTS {h}, BHUB {t, U, mu}. */
+#define IMPS_MODEL_XXZ 0
+#define IMPS_MODEL_FHUB 1
+#define IMPS_MODEL_QIM 2
+#define IMPS_MODEL_QPOTTS(q) (3 | ((q) << 8))
+#define IMPS_MODEL_BHUB(d) (4 | ((d) << 8))
 int imps_model_nsym(int model);
 int imps_model_d(int model);
+int imps_model_periods(int model, int* periods);
 int imps_model_i2k(int model, int N, const int* QN, int* i2k);
 int imps_model_validpath(int model, int N, const int* QN, int* path);
 int imps_model_dimmap(int model, int N, const int* QN, const int* dims, int ndims, int* keys, int* sizes);

@@ -12,7 +12,9 @@
 // bookkeeping known answers, isometry/gauge/Hermiticity/adjoint identities, GMRES residuals and
 // the geometric-sum identity of Excitations_helpers.hpp:855-877.
 //
-// Restrictions: abelian U(1)^nsym groups only (period 0, symobj.hpp:80-161 with periods {0,...}).
+// Symmetry groups: abelian, U(1) or Z(N) per charge (symobj.hpp:80-161).  The group is process
+// state here (g_per, set by orc_set_periods and by the model entry points), as every key of the
+// reference refers to its model's one group object (KeyTypes.hpp:19-58).
 
 #include <algorithm>
 #include <cassert>
@@ -35,8 +37,15 @@ namespace orc {
 
 using Key = std::vector<int>;  // std::vector<int> lexicographic order = KeyType order (KeyTypes.hpp:19-58)
 
-static Key kadd(const Key& a, const Key& b) { Key c(a); for (size_t i = 0; i < c.size(); ++i) c[i] += b[i]; return c; }
-static Key ksub(const Key& a, const Key& b) { Key c(a); for (size_t i = 0; i < c.size(); ++i) c[i] -= b[i]; return c; }
+// charge periods of the current group: 0 = U(1), N > 1 = Z(N) (symobj::init's Plus_ / Minus_:
+// the sum or difference taken modulo N into [0, N))
+static std::vector<int> g_per;
+static int kmod(int x, size_t i) {
+    const int p = i < g_per.size() ? g_per[i] : 0;
+    return p > 1 ? (x % p + p) % p : x;
+}
+static Key kadd(const Key& a, const Key& b) { Key c(a); for (size_t i = 0; i < c.size(); ++i) c[i] = kmod(c[i] + b[i], i); return c; }
+static Key ksub(const Key& a, const Key& b) { Key c(a); for (size_t i = 0; i < c.size(); ++i) c[i] = kmod(c[i] - b[i], i); return c; }
 
 // ---------------------------------------------------------------------------------------------
 // dense column-major matrix
@@ -670,15 +679,41 @@ static MPS mps_nullspace(const MPS& A, int dir) {
 // ---------------------------------------------------------------------------------------------
 // bookkeeping (Models.cpp; ItoKey.hpp)
 static int igcd(int u, int v) { u = std::abs(u); v = std::abs(v); while (v != 0) { int r = u % v; u = v; v = r; } return u; }
-// model 0 = XXZ (d=2, one U(1)), model 1 = FHUB in (N,M) representation (d=4, U(1)xU(1))
+// model ids: type | (d << 8); type 0 = XXZ (d=2, one U(1)), 1 = FHUB in (N,M) representation
+// (d=4, U(1)xU(1)), 2 = quantum Ising (d=2, Z(2)), 3 = quantum Potts (d=q, Z(q)), 4 = Bose-Hubbard
+// (local dimension d, U(1))
+static int mtype(int model) { return model & 0xff; }
+static int model_d(int model) {
+    const int t = mtype(model), d = model >> 8;
+    if (t == 0 || t == 2) return (d == 0 || d == 2) ? 2 : -1;
+    if (t == 1) return (d == 0 || d == 4) ? 4 : -1;
+    if (t == 3 || t == 4) return (d >= 2 && d <= 16) ? d : -1;
+    return -1;
+}
+static int model_nsym(int model) { return model_d(model) < 0 ? -1 : (mtype(model) == 1 ? 2 : 1); }
+// the model's group (GroupObj_.init in each Init: Models.cpp:200, 335, 393, 521, 749) made current
+static void use_group(int model) {
+    const int t = mtype(model), d = model_d(model);
+    if (d < 0) throw std::domain_error("unknown model");
+    g_per = t == 1 ? std::vector<int>{0, 0} : t == 2 ? std::vector<int>{2} : t == 3 ? std::vector<int>{d} : std::vector<int>{0};
+}
 static std::vector<std::pair<int, int>> getdQ(int model, int N, const int* QN) {
     std::vector<std::pair<int, int>> dQ;
-    if (model == 0) {  // Models.cpp:216-240
+    const int t = mtype(model);
+    if (t == 2 || t == 3) {  // QuantumIsingModel::GetdQ :351-355, QuantumPottsModel::GetdQ :472-476
+        dQ.push_back({1, 0});
+    } else if (t == 4) {     // BoseHubbardModel::GetdQ :830-843
+        const int n = QN[0];
+        if (n < 1 || n >= N * model_d(model)) throw std::domain_error("0 < n < N d");
+        const int div = igcd(N, n);
+        dQ.push_back({N / div, -n / div});
+    } else if (t == 0) {  // Models.cpp:216-240
         int m = QN[0];
         if (std::abs(m) >= N) throw std::domain_error("|m| < N");
         int div = igcd(N - m, N + m);
         dQ.push_back({2 * N / div, -(N + m) / div});
     } else {           // Models.cpp:642-662
+        if (t != 1) throw std::domain_error("unknown model");
         int n = QN[0], m = QN[1];
         if (n + m <= 0 || n + m >= 2 * N) throw std::domain_error("n range");
         if (n - m <= 0 || n - m >= 2 * N) throw std::domain_error("n range");
@@ -692,8 +727,16 @@ static std::vector<std::pair<int, int>> getdQ(int model, int N, const int* QN) {
 static std::vector<Key> get_i2k(int model, int N, const int* QN) {
     auto dQ = getdQ(model, N, QN);
     std::vector<Key> I2K;
-    if (model == 0) {  // Models.cpp:270-277: s = d-1 ... 0 pushed in that order
+    const int t = mtype(model), d = model_d(model);
+    if (t == 0) {  // Models.cpp:270-277: s = d-1 ... 0 pushed in that order
         for (int s = 1; s >= 0; --s) I2K.push_back({dQ[0].second + s * dQ[0].first});
+    } else if (t == 2) {  // QuantumIsingModel::GetI2K :372-376: {0}, {1}
+        I2K.push_back({0});
+        I2K.push_back({1});
+    } else if (t == 3) {  // QuantumPottsModel::GetI2K :500-506: {i}
+        for (int i = 0; i < d; ++i) I2K.push_back({i});
+    } else if (t == 4) {  // BoseHubbardModel::GetI2K :862-878: {dQ.second + i dQ.first}
+        for (int i = 0; i < d; ++i) I2K.push_back({dQ[0].second + i * dQ[0].first});
     } else {           // Models.cpp:712-718
         I2K.push_back({dQ[0].second, dQ[1].second + dQ[1].first});
         I2K.push_back({dQ[0].second + dQ[0].first, dQ[1].second});
@@ -704,9 +747,15 @@ static std::vector<Key> get_i2k(int model, int N, const int* QN) {
 }
 static std::map<Key, int> get_dimmap(int model, const std::vector<int>& dims, int N, const int* QN) {
     auto dQ = getdQ(model, N, QN);
-    const int Nsec = (int)dims.size();
+    int Nsec = (int)dims.size();
     std::map<Key, int> dm;
-    if (model == 0) {  // Models.cpp:248-260
+    const int t = mtype(model), d = model_d(model);
+    if (t == 2 || t == 3) {  // QIM :357-370 (2 sectors), QPOTTS :478-498 (q sectors); extra dims dropped
+        if (Nsec < d) throw std::domain_error("not enough symmetry sectors");
+        for (int i = 0; i < d; ++i) dm[{i}] = dims[i];
+    } else if (t == 4) {  // BHUB :845-860
+        for (int i = 0; i < Nsec; ++i) dm[{i * dQ[0].first}] = dims[i];
+    } else if (t == 0) {  // Models.cpp:248-260
         for (int i = 0; i < Nsec; ++i) dm[{dQ[0].first * (i - Nsec / 2)}] = dims[i];
     } else {           // Models.cpp:680-697 (N,M) representation
         for (int i = 0; i < Nsec; ++i)
@@ -770,7 +819,49 @@ static std::vector<double> dmul(const std::vector<double>& A, const std::vector<
 static std::vector<double> dtr(const std::vector<double>& A, int d) { std::vector<double> T(A.size()); for (int i = 0; i < d; ++i) for (int j = 0; j < d; ++j) T[j + i * d] = A[i + j * d]; return T; }
 static std::vector<double> axpy(double a, const std::vector<double>& x, const std::vector<double>& y) { std::vector<double> z(y); for (size_t i = 0; i < z.size(); ++i) z[i] += a * x[i]; return z; }
 static std::vector<double> model_ham(int model, const double* p) {
-    if (model == 0) {  // XXZ: p = {J, Delta, hz}; Jx = Jy = J, Jz = Delta
+    const int mt = mtype(model);
+    if (mt == 2) {  // QIM (Models.cpp:325-330 -> XYZAbstractModel::MakeOps :57-93 with Jx = J, Jy = Jz = 0, hz = h)
+        const int d = 2;
+        std::vector<double> sx(4, 0), sz(4, 0), id(4, 0);
+        sx[0 + 1 * d] = 0.5; sx[1 + 0 * d] = 0.5;
+        sz[0] = 0.5; sz[1 + 1 * d] = -0.5;
+        id[0] = 1; id[3] = 1;
+        std::vector<double> H((size_t)16, 0.0);
+        H = axpy(-p[0], kron(sx, sx, d), H);
+        H = axpy(-0.5 * p[1], kron(sz, id, d), H);
+        H = axpy(-0.5 * p[1], kron(id, sz, d), H);
+        return H;
+    }
+    if (mt == 3) {  // QPOTTS, QuantumPottsModel::MakeOps :411-430; p = {h}
+        const int q = model_d(model);
+        std::vector<double> X((size_t)q * q, 0), Z((size_t)q * q, 0), id((size_t)q * q, 0);
+        X[(q - 1) + 0 * q] = 1;
+        Z[0] = q - 1;
+        for (int i = 1; i < q; ++i) { X[(i - 1) + i * q] = 1; Z[i + i * q] = -1; }
+        for (int i = 0; i < q; ++i) id[i + i * q] = 1;
+        std::vector<std::vector<double>> MXq{X};
+        for (int i = 1; i < q - 1; ++i) MXq.push_back(dmul(MXq.back(), X, q));
+        std::vector<double> H((size_t)q * q * q * q, 0.0);
+        H = axpy(-0.5 * p[0], kron(Z, id, q), H);
+        H = axpy(-0.5 * p[0], kron(id, Z, q), H);
+        for (int i = 0; i < q - 1; ++i) H = axpy(-1.0, kron(MXq[i], MXq[q - 2 - i], q), H);
+        return H;
+    }
+    if (mt == 4) {  // BHUB, BoseHubbardModel::MakeOps :768-777; p = {t, U, mu}
+        const int d = model_d(model);
+        std::vector<double> b((size_t)d * d, 0), n((size_t)d * d, 0), id((size_t)d * d, 0);
+        for (int i = 1; i < d; ++i) { b[(i - 1) + i * d] = std::sqrt((double)i); n[i + i * d] = i; }
+        for (int i = 0; i < d; ++i) id[i + i * d] = 1;
+        auto onsite = axpy(-p[2], n, dmul(axpy(0.5 * p[1], n, std::vector<double>((size_t)d * d, 0.0)), axpy(-1.0, id, n), d));
+        auto bt = dtr(b, d);
+        std::vector<double> H((size_t)d * d * d * d, 0.0);
+        H = axpy(-p[0], kron(b, bt, d), H);
+        H = axpy(-p[0], kron(bt, b, d), H);
+        H = axpy(0.5, kron(onsite, id, d), H);
+        H = axpy(0.5, kron(id, onsite, d), H);
+        return H;
+    }
+    if (mt == 0) {  // XXZ: p = {J, Delta, hz}; Jx = Jy = J, Jz = Delta
         const int d = 2;
         std::vector<double> sx(4, 0), syi(4, 0), sz(4, 0), id(4, 0);
         sx[0 + 1 * d] = 0.5; sx[1 + 0 * d] = 0.5;
@@ -1095,20 +1186,27 @@ double orc_gradnorm_left(void* AL, void* C, void* EHL, void* EHR, void* HL, void
     } catch (const std::exception& e) { g_err = e.what(); return -1; }
 }
 // ---- bookkeeping ----
+void orc_set_periods(int nsym, const int* periods) {
+    g_per.assign(periods, periods + nsym);
+    for (auto& p : g_per) if (p == 1) p = 0;
+}
+int orc_model_d(int model) { return model_d(model); }
+int orc_model_nsym(int model) { return model_nsym(model); }
 int orc_i2k(int model, int N, const int* QN, int* out) {
-    try { auto I = get_i2k(model, N, QN); int k = 0; for (auto& q : I) for (int v : q) out[k++] = v; return (int)I.size(); }
+    try { use_group(model); auto I = get_i2k(model, N, QN); int k = 0; for (auto& q : I) for (int v : q) out[k++] = v; return (int)I.size(); }
     catch (const std::exception& e) { g_err = e.what(); return -1; }
 }
 int orc_validpath(int model, int N, const int* QN, int* path) {
-    try { auto p = validpath(N, get_i2k(model, N, QN)); for (size_t i = 0; i < p.size(); ++i) path[i] = p[i]; return (int)p.size(); }
+    try { use_group(model); auto p = validpath(N, get_i2k(model, N, QN)); for (size_t i = 0; i < p.size(); ++i) path[i] = p[i]; return (int)p.size(); }
     catch (const std::exception& e) { g_err = e.what(); return -1; }
 }
 // GetDimMap + MakeDims: writes N bonds of (keys, sizes); returns sectors per bond
 int orc_make_dims_model(int model, const int* dims, int nsec, int N, const int* QN, int* keys, int* sizes) {
     try {
+        use_group(model);
         auto dm0 = get_dimmap(model, std::vector<int>(dims, dims + nsec), N, QN);
         auto dv = make_dims(dm0, N, get_i2k(model, N, QN));
-        int k = 0, ns = model == 0 ? 1 : 2, b = 0;
+        int k = 0, b = 0;
         for (auto& bond : dv) for (auto& kv : bond) { for (int v : kv.first) keys[k++] = v; sizes[b++] = kv.second; }
         return (int)dv[0].size();
     } catch (const std::exception& e) { g_err = e.what(); return -1; }
@@ -1116,6 +1214,7 @@ int orc_make_dims_model(int model, const int* dims, int nsec, int N, const int* 
 // MakeDims from an explicit bond-0 dim map
 int orc_make_dims(int model, int N, const int* QN, int nsym, int nsec, const int* keys0, const int* sizes0, int* keys, int* sizes) {
     try {
+        use_group(model);
         std::map<Key, int> dm0;
         for (int i = 0; i < nsec; ++i) dm0[Key(keys0 + i * nsym, keys0 + (i + 1) * nsym)] = sizes0[i];
         auto dv = make_dims(dm0, N, get_i2k(model, N, QN));
@@ -1126,6 +1225,7 @@ int orc_make_dims(int model, int N, const int* QN, int nsym, int nsec, const int
 }
 void* orc_mps_structure(int model, int N, const int* QN, int nsym, int nl, const int* kl, const int* sl, int nr, const int* kr, const int* sr) {
     ORC_TRY(
+        use_group(model);
         std::map<Key, int> ml, mr;
         for (int i = 0; i < nl; ++i) ml[Key(kl + i * nsym, kl + (i + 1) * nsym)] = sl[i];
         for (int i = 0; i < nr; ++i) mr[Key(kr + i * nsym, kr + (i + 1) * nsym)] = sr[i];
@@ -1133,6 +1233,7 @@ void* orc_mps_structure(int model, int N, const int* QN, int nsym, int nl, const
     )
 }
 void orc_model_ham(int model, const double* params, double* H) {
+    if (model_d(model) < 0) return;
     auto h = model_ham(model, params);
     std::copy(h.begin(), h.end(), H);
 }

@@ -68,6 +68,9 @@ def lib():
             "orc_make_dims": (I, [I, I, ip, I, I, ip, ip, ip, ip]),
             "orc_mps_structure": (P, [I, I, ip, I, I, ip, ip, I, ip, ip]),
             "orc_model_ham": (None, [I, dp, dp]),
+            "orc_set_periods": (None, [I, ip]),
+            "orc_model_d": (I, [I]),
+            "orc_model_nsym": (I, [I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -87,7 +90,9 @@ def _dp(a):
 
 @dataclass
 class OB:
-    """Block-sparse object: structure arrays + flat data."""
+    """Block-sparse object: structure arrays + flat data; `periods` is the symmetry group (per
+    charge 0 = U(1), N = Z(N); None = U(1)), made the oracle's current group by every operation on
+    the object."""
     nsym: int
     nblk: np.ndarray
     qin: np.ndarray
@@ -95,6 +100,7 @@ class OB:
     rows: np.ndarray
     cols: np.ndarray
     data: np.ndarray
+    periods: object = None
 
     @staticmethod
     def like(layout, data=None) -> "OB":
@@ -103,13 +109,14 @@ class OB:
         assert dat.size == n
         return OB(layout.nsym, np.asarray(layout.nblk, np.int32), np.asarray(layout.qin, np.int32).reshape(-1, layout.nsym),
                   np.asarray(layout.qout, np.int32).reshape(-1, layout.nsym), np.asarray(layout.rows, np.int32),
-                  np.asarray(layout.cols, np.int32), dat)
+                  np.asarray(layout.cols, np.int32), dat, getattr(layout, "periods", None))
 
     @property
     def nelem(self):
         return self.data.size
 
     def handle(self):
+        set_group(self.nsym, self.periods)
         nb = np.ascontiguousarray(self.nblk, np.int32)
         qi = np.ascontiguousarray(self.qin, np.int32).reshape(-1)
         qo = np.ascontiguousarray(self.qout, np.int32).reshape(-1)
@@ -155,10 +162,37 @@ class _H:
         cols = np.zeros(max(nb, 1), np.int32)
         data = np.zeros(max(n, 1))
         L.orc_export(self.h, _ip(qin), _ip(qout), _ip(rows), _ip(cols), _dp(data))
-        return OB(ns, nblk, qin[:nb * ns].reshape(nb, ns), qout[:nb * ns].reshape(nb, ns), rows[:nb], cols[:nb], data[:n])
+        return OB(ns, nblk, qin[:nb * ns].reshape(nb, ns), qout[:nb * ns].reshape(nb, ns), rows[:nb], cols[:nb], data[:n], _group[1])
 
 
 _alive = []
+_group = [1, None]  # (nsym, periods) of the oracle's current group
+
+
+def set_group(nsym: int, periods=None):
+    """Make (nsym, periods) the oracle's current symmetry group (None: U(1) for every charge)."""
+    per = np.zeros(nsym, np.int32) if periods is None else np.ascontiguousarray(periods, np.int32).reshape(-1)
+    lib().orc_set_periods(nsym, _ip(per))
+    _group[0], _group[1] = nsym, (None if periods is None else per.copy())
+
+
+def model_periods(model):
+    """The model's group as the oracle restates it (0 = U(1), N = Z(N))."""
+    t, d = model & 0xFF, model_d(model)
+    return np.array({1: [0, 0], 2: [2], 3: [d]}.get(t, [0]), np.int32)
+
+
+def model_d(model):
+    return lib().orc_model_d(model)
+
+
+def model_nsym(model):
+    return lib().orc_model_nsym(model)
+
+
+def _qn(QN):
+    q = np.asarray(list(QN), np.int32).reshape(-1)
+    return q if q.size else np.zeros(1, np.int32)
 
 
 def _h(x: OB):
@@ -304,9 +338,9 @@ def gradnorm(direction, A: OB, C: OB, EHL, EHR, HL, HR) -> float:
 # ---- bookkeeping ----
 @_op
 def i2k(model, N, QN):
-    d, ns = (2, 1) if model == 0 else (4, 2)
+    d, ns = model_d(model), model_nsym(model)
     out = np.zeros(d * ns, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     assert lib().orc_i2k(model, N, _ip(q), _ip(out)) == d
     return out.reshape(d, ns)
 
@@ -314,19 +348,19 @@ def i2k(model, N, QN):
 @_op
 def validpath(model, N, QN):
     out = np.zeros(N, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     assert lib().orc_validpath(model, N, _ip(q), _ip(out)) == N
     return out
 
 
 @_op
 def make_dims_model(model, dims, N, QN):
-    ns = 1 if model == 0 else 2
-    nsec = len(dims) if model == 0 else len(dims) ** 2
+    ns = model_nsym(model)
+    nsec = len(dims) ** 2 if model == 1 else len(dims)
     keys = np.zeros(N * nsec * ns, np.int32)
     sizes = np.zeros(N * nsec, np.int32)
     dd = np.asarray(dims, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     n = lib().orc_make_dims_model(model, _ip(dd), len(dd), N, _ip(q), _ip(keys), _ip(sizes))
     assert n > 0, lib().orc_last_error()
     return [(keys.reshape(N, n, ns)[k], sizes.reshape(N, n)[k]) for k in range(N)]
@@ -334,13 +368,13 @@ def make_dims_model(model, dims, N, QN):
 
 @_op
 def make_dims(model, N, QN, keys0, sizes0):
-    ns = 1 if model == 0 else 2
+    ns = model_nsym(model)
     k0 = np.ascontiguousarray(keys0, np.int32).reshape(-1)
     s0 = np.ascontiguousarray(sizes0, np.int32)
     n0 = len(s0)
     keys = np.zeros(N * n0 * ns, np.int32)
     sizes = np.zeros(N * n0, np.int32)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
     n = lib().orc_make_dims(model, N, _ip(q), ns, n0, _ip(k0), _ip(s0), _ip(keys), _ip(sizes))
     assert n == n0, lib().orc_last_error()
     return [(keys.reshape(N, n, ns)[k], sizes.reshape(N, n)[k]) for k in range(N)]
@@ -348,17 +382,18 @@ def make_dims(model, N, QN, keys0, sizes0):
 
 @_op
 def mps_structure(model, N, QN, left, right) -> OB:
-    ns = 1 if model == 0 else 2
+    ns = model_nsym(model)
     kl, sl = (np.ascontiguousarray(a, np.int32) for a in left)
     kr, sr = (np.ascontiguousarray(a, np.int32) for a in right)
-    q = np.asarray(QN, np.int32)
+    q = _qn(QN)
+    set_group(ns, model_periods(model))
     return _H(lib().orc_mps_structure(model, N, _ip(q), ns, len(sl), _ip(kl.reshape(-1)), _ip(sl), len(sr),
                                       _ip(kr.reshape(-1)), _ip(sr))).get()
 
 
 @_op
 def model_ham(model, params):
-    d = 2 if model == 0 else 4
+    d = model_d(model)
     p = np.asarray(params, np.float64)
     H = np.zeros(d ** 4)
     lib().orc_model_ham(model, _dp(p), _dp(H))

@@ -1,7 +1,8 @@
 """Synthetic, seeded block-sparse states for the tests, built with the oracle (the checker).
 
 Sector profiles follow the measurement configurations: XXZ U(1) with charges spaced by 2 and
-Gaussian-in-charge sizes; Fermi-Hubbard U(1)xU(1) (N, M) sectors on a checkerboard.
+Gaussian-in-charge sizes; Fermi-Hubbard U(1)xU(1) (N, M) sectors on a checkerboard.  The Z(q)
+models (quantum Ising, Potts) and the Bose-Hubbard chain take explicit per-sector dims (GetDimMap).
 """
 from __future__ import annotations
 
@@ -18,7 +19,15 @@ from oracle import pyoracle as orc  # noqa: E402
 from imps_cpp_tools_amd.workloads import fhub_profile, xxz_profile  # noqa: E402
 from oracle.pyoracle import OB  # noqa: E402
 
-XXZ, FHUB = 0, 1
+XXZ, FHUB, QIM = 0, 1, 2
+
+
+def QPOTTS(q):
+    return 3 | (q << 8)
+
+
+def BHUB(d):
+    return 4 | (d << 8)
 
 
 def xxz_bond0(nsec: int, peak: int, lo: int = 1, total: int = None):
@@ -29,17 +38,18 @@ def fhub_bond0(nside: int, peak: int, lo: int = 1, total: int = None):
     return fhub_profile(nside, peak, lo, total)
 
 
-def diag_layout(nsym, keys, sizes, data=None):
+def diag_layout(nsym, keys, sizes, data=None, periods=None):
     keys = np.asarray(keys, np.int32).reshape(-1, nsym)
     sizes = np.asarray(sizes, np.int32)
-    lay = OB(nsym, np.array([len(sizes)], np.int32), keys, keys, sizes, sizes, np.zeros(int((sizes.astype(np.int64) ** 2).sum())))
+    lay = OB(nsym, np.array([len(sizes)], np.int32), keys, keys, sizes, sizes, np.zeros(int((sizes.astype(np.int64) ** 2).sum())),
+             periods)
     if data is not None:
         lay.data = np.ascontiguousarray(data, np.float64)
     return lay
 
 
 def random_like(ob: OB, rng, sym=False, shift=0.0) -> OB:
-    out = OB(ob.nsym, ob.nblk, ob.qin, ob.qout, ob.rows, ob.cols, rng.standard_normal(ob.nelem))
+    out = OB(ob.nsym, ob.nblk, ob.qin, ob.qout, ob.rows, ob.cols, rng.standard_normal(ob.nelem), ob.periods)
     if sym:
         for _, qi, qo, m in out.blocks():
             m[...] = 0.5 * (m + m.T) + shift * np.eye(m.shape[0])
@@ -51,8 +61,8 @@ class Cell:
 
     def __init__(self, model, N, QN, keys0, sizes0, seed):
         self.model, self.N, self.QN = model, N, list(QN)
-        self.nsym = 1 if model == XXZ else 2
-        self.d = 2 if model == XXZ else 4
+        self.nsym, self.d = orc.model_nsym(model), orc.model_d(model)
+        self.periods = orc.model_periods(model)
         self.bonds = orc.make_dims(model, N, QN, keys0, sizes0)
         rng = np.random.default_rng(seed)
         self.rng = rng
@@ -70,7 +80,7 @@ class Cell:
 
     def diag(self, bond, data=None):
         k, s = self.bonds[bond % self.N]
-        return diag_layout(self.nsym, k, s, data)
+        return diag_layout(self.nsym, k, s, data, self.periods)
 
     def bond_offsets(self, bond):
         k, s = self.bonds[bond % self.N]

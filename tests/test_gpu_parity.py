@@ -5,12 +5,14 @@ Tolerances (norm-wise relative error ||y - y_oracle|| / ||y_oracle||):
   FP32   : 1e-5   (FP32 MFMA, FP32 accumulation over K <= ~500)
   BF16X3 : 2e-4   (split-BF16 hi*hi + hi*lo + lo*hi, each product accurate to ~2^-16 relative)
 Block structures are exact by construction: every output block of the layout is compared.
+Cells: XXZ U(1), Fermi-Hubbard U(1)xU(1), and the Z(q) / Bose-Hubbard structures (quantum Ising
+Z(2), 3-state Potts Z(3), whose blocks wrap modulo q, Bose-Hubbard d = 4 U(1) with an empty slice).
 """
 import numpy as np
 import pytest
 
 import imps_cpp_tools_amd as im
-from states import FHUB, XXZ, Cell, fhub_bond0, random_like, xxz_bond0
+from states import BHUB, FHUB, QIM, QPOTTS, XXZ, Cell, fhub_bond0, random_like, xxz_bond0
 from oracle import pyoracle as orc
 
 torch = pytest.importorskip("torch")
@@ -33,7 +35,7 @@ def dev(a):
 
 
 def lay(ob):
-    return im.BlockLayout(ob.nsym, ob.nblk, ob.qin, ob.qout, ob.rows, ob.cols)
+    return im.BlockLayout(ob.nsym, ob.nblk, ob.qin, ob.qout, ob.rows, ob.cols, ob.periods)
 
 
 def relerr(got, ref):
@@ -55,15 +57,24 @@ def cell(name):
         elif name == "xxz_big":  # TM microbenchmark shapes: 17 sectors, sizes up to 64
             k, s = xxz_bond0(17, 64, 2)
             CELLS[name] = Cell(XXZ, 2, [0], k, s, seed=1002)
+        elif name in ZCELLS:
+            model, QN, dims = ZCELLS[name]
+            b = orc.make_dims_model(model, dims, 2, QN)
+            CELLS[name] = Cell(model, 2, QN, b[0][0], b[0][1], seed=103)
     return CELLS[name]
 
 
+ZCELLS = {"qim": (QIM, [], [40, 36]), "qpotts3": (QPOTTS(3), [], [20, 26, 17]), "bhub4": (BHUB(4), [2], [9, 14, 11, 6])}
+HAM = {XXZ: [1.0, -1.0, 0.1], FHUB: [1.0, 4.0, 0.0, 0.0, 0.0], QIM: [1.0, 0.7], QPOTTS(3): [0.8], BHUB(4): [1.0, 3.0, 0.4]}
+NAMES = ["xxz", "fhub", "qim", "qpotts3", "bhub4"]
+
+
 def ham(c):
-    return orc.model_ham(c.model, [1.0, -1.0, 0.1] if c.model == XXZ else [1.0, 4.0, 0.0, 0.0, 0.0])
+    return orc.model_ham(c.model, HAM[c.model])
 
 
 @pytest.mark.parametrize("prec", PRECS)
-@pytest.mark.parametrize("name", ["xxz", "fhub", "xxz_big"])
+@pytest.mark.parametrize("name", NAMES + ["xxz_big"])
 @pytest.mark.parametrize("direction", [0, 1])
 @pytest.mark.parametrize("mixed", [False, True])
 def test_tm_parity(ctx, prec, name, direction, mixed):
@@ -86,7 +97,7 @@ def test_tm_parity(ctx, prec, name, direction, mixed):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-@pytest.mark.parametrize("name", ["xxz", "fhub"])
+@pytest.mark.parametrize("name", NAMES)
 def test_hac_parity(ctx, prec, name):
     c = cell(name)
     H = ham(c)
@@ -109,7 +120,7 @@ def test_hac_parity(ctx, prec, name):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-@pytest.mark.parametrize("name", ["xxz", "fhub"])
+@pytest.mark.parametrize("name", NAMES)
 def test_hc_parity(ctx, prec, name):
     """H_C through the reduced form equals the reference's per-nonzero 5-factor chain."""
     c = cell(name)
@@ -162,7 +173,7 @@ def test_hac_ksplit_variants(ctx, prec, ksplit, monkeypatch):
 
 
 @pytest.mark.parametrize("prec", [im.FP64, im.FP32])
-@pytest.mark.parametrize("name", ["xxz", "fhub"])
+@pytest.mark.parametrize("name", NAMES)
 @pytest.mark.parametrize("direction", [0, 1])
 def test_redop_parity(ctx, prec, name, direction):
     c = cell(name)

@@ -233,8 +233,14 @@ def test_vumps_from_config_file(tmp_path):
     assert GOLD["bethe_ansatz"] <= hist[-1].energy <= GOLD["bethe_ansatz"] + 1e-3
     sz = v.observables(["sz"])["sz"]
     assert abs(sz.sum()) <= max(1e-10, 10.0 * float(np.max(v.prec_n)))  # QN = [0]: no total Sz (to the gauge error)
+    # charge-changing operators share no blocks with the symmetric state: exactly zero; the
+    # two-site Hamiltonian as an observable gives the bond energies, whose mean is the energy
+    o = v.observables(["sx", "sp", "syi"])
+    assert all(np.all(o[k] == 0.0) for k in o)
+    eb = v.observables({"h": v.H})["h"]
+    assert abs(eb.mean() - v.energy()) <= max(1e-10, 10.0 * float(np.max(v.prec_n)))
     with pytest.raises(ValueError):
-        v.observables(["sx"])
+        v.observables(["sy"])
     files = list((tmp_path / "data").glob("UMPS_XXZ_*_N2_chi24.bin"))
     assert len(files) == 1
     w = _small_xxz()

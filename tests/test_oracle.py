@@ -4,7 +4,8 @@ the reference prints as self-checks (isometry, gauge, GMRES residual, geometric-
 import numpy as np
 import pytest
 
-from states import (FHUB, XXZ, Cell, dense_mat, dense_mps, diag_layout, extract, fhub_bond0, random_like, xxz_bond0)
+from states import (BHUB, FHUB, QIM, QPOTTS, XXZ, Cell, dense_mat, dense_mps, diag_layout, extract, fhub_bond0, random_like,
+                    xxz_bond0)
 from oracle import pyoracle as orc
 from oracle.pyoracle import OB
 
@@ -72,17 +73,27 @@ def test_mps_structure_matches_i2k_rule():
 
 
 # ---------------------------------------------------------------- dense brute force
-@pytest.fixture(scope="module", params=["xxz", "fhub"])
+# Z(q) and Bose-Hubbard cells: (model, QN, GetDimMap dims); the Bose-Hubbard chain at one boson
+# per site with local dimension 4 (charges shift by I2K = [-1, 0, 1, 2])
+ZCELLS = {"qim": (QIM, [], [5, 4]), "qpotts3": (QPOTTS(3), [], [3, 4, 2]), "bhub4": (BHUB(4), [2], [2, 3, 2])}
+HAM_PARAMS = {XXZ: [1.0, -1.0, 0.3], FHUB: [1.0, 4.0, 0.2, 0.1, 0.05], QIM: [1.0, 0.7], QPOTTS(3): [0.8], BHUB(4): [1.0, 3.0, 0.4]}
+
+
+@pytest.fixture(scope="module", params=["xxz", "fhub", "qim", "qpotts3", "bhub4"])
 def cell(request):
     if request.param == "xxz":
         k, s = xxz_bond0(7, 6)
         return Cell(XXZ, 2, [0], k, s, seed=11)
-    k, s = fhub_bond0(3, 4)
-    return Cell(FHUB, 2, [2, 0], k, s, seed=12)
+    if request.param == "fhub":
+        k, s = fhub_bond0(3, 4)
+        return Cell(FHUB, 2, [2, 0], k, s, seed=12)
+    model, QN, dims = ZCELLS[request.param]
+    b = orc.make_dims_model(model, dims, 2, QN)
+    return Cell(model, 2, QN, b[0][0], b[0][1], seed=13)
 
 
 def _ham(c):
-    return orc.model_ham(c.model, [1.0, -1.0, 0.3] if c.model == XXZ else [1.0, 4.0, 0.2, 0.1, 0.05])
+    return orc.model_ham(c.model, HAM_PARAMS[c.model])
 
 
 def test_isometry_and_qr(cell):
@@ -145,18 +156,18 @@ def test_tm_charged_blockmat_dense(cell):
     A = random_like(c.struct[n], rng)
     B = random_like(c.struct[n], rng)
     Ad, Bd = dense_mps(A, lo, Dl, ro, Dr), dense_mps(B, lo, Dl, ro, Dr)
-    K = np.array([2] if c.model == XXZ else [1, 1])
+    K = np.array([2] if c.model == XXZ else [1, 1] if c.model == FHUB else [1])
 
     def charged(bond):
         k, s = c.bonds[bond % c.N]
         kk = [tuple(x) for x in k]
         qi, qo, r, cc = [], [], [], []
         for a, sa in zip(k, s):
-            t = tuple(a + K)
+            t = tuple(int(v) % int(p) if p > 1 else int(v) for v, p in zip(a + K, c.periods))  # Z(q) charges wrap
             if t in kk:
                 qi.append(a), qo.append(np.array(t)), r.append(sa), cc.append(s[kk.index(t)])
         return OB(c.nsym, np.array([len(r)], np.int32), np.array(qi, np.int32), np.array(qo, np.int32),
-                  np.array(r, np.int32), np.array(cc, np.int32), np.zeros(int(np.dot(r, cc))))
+                  np.array(r, np.int32), np.array(cc, np.int32), np.zeros(int(np.dot(r, cc))), c.periods)
 
     x = random_like(charged(n), rng)
     ylay = charged(n + 1)

@@ -6,6 +6,8 @@ import tempfile
 
 import pytest
 
+import imps_cpp_tools_amd as im
+
 from imps_cpp_tools_amd import vumps_main as vm
 
 CFG = """# model and parameters
@@ -54,6 +56,22 @@ def test_settings_defaults_and_vectors():
     x = vm.settings(vm.parse_lines(["--model=XXZ", "--Delta=-1", "--N=2", "--QN=[0]", "--m0=[2,4,2]", "--mmax=[16,32]", "--hz=0.5"]))
     assert x["params"] == [1.0, -1.0, 0.5] and x["mmax"] == [16, 32]
     assert vm.settings(vm.parse_lines(CFG.splitlines() + ["--obs=[n,sz]"]))["obs"] == ["n", "sz"]
+
+
+def test_settings_of_the_z_q_and_boson_models():
+    """CreateModel's keys and defaults (Models.cpp:976-1109): QIM J (1), hz (required); QPOTTS q
+    (required), h (0); BHUB d (required), t (1), U (required), mu (0).  QIM and QPOTTS take no QN."""
+    q = vm.settings(vm.parse_lines(["--model=QIM", "--hz=1.5", "--N=2", "--m0=[8,8]", "--mmax=[16]"]))
+    assert q["params"] == [1.0, 1.5] and q["QN"] == [] and vm.model_id(q["model"], q["dim"]) == im.QIM
+    p = vm.settings(vm.parse_lines(["--model=QPOTTS", "--q=3", "--N=2", "--m0=[4,3,3]", "--mmax=[10]"]))
+    assert p["params"] == [0.0] and p["dim"] == 3 and vm.model_id(p["model"], p["dim"]) == im.QPOTTS(3)
+    b = vm.settings(vm.parse_lines(["--model=BHUB", "--d=4", "--U=10", "--N=2", "--QN=[2]", "--m0=[1,2,1]", "--mmax=[8]"]))
+    assert b["params"] == [1.0, 10.0, 0.0] and vm.model_id(b["model"], b["dim"]) == im.BHUB(4)
+    for bad in (["--model=QIM", "--N=2", "--m0=[2,2]", "--mmax=[4]"],              # hz missing
+                ["--model=QPOTTS", "--N=2", "--m0=[2,2,2]", "--mmax=[4]"],          # q missing
+                ["--model=BHUB", "--d=3", "--N=2", "--QN=[2]", "--m0=[2]", "--mmax=[4]"]):  # U missing
+        with pytest.raises(ValueError):
+            vm.settings(vm.parse_lines(bad))
 
 
 @pytest.mark.parametrize("bad", [["--model=QIM"], ["--model=XXZ", "--N=2", "--QN=[0]", "--m0=[1]", "--mmax=[8]"],
