@@ -374,13 +374,20 @@ class BlockOps:
         Vc = t.empty_like(V)  # conjugated basis: the inner products read it without conjugating V per step
         V[0] = r / rnorm
         Vc[0] = V[0].conj()
-        Hm = np.zeros((kry + 1, kry + 1), complex)
-        c = np.zeros(kry + 1, complex)
-        s = np.zeros(kry + 1, complex)
-        g = np.zeros(kry + 1, complex)
+        # Hessenberg matrix and rotations sized by the basis capacity, grown with it
+        Hm = np.zeros((cap, cap), complex)
+        c = np.zeros(cap, complex)
+        s = np.zeros(cap, complex)
+        g = np.zeros(cap, complex)
         g[0] = rnorm
         kdim, flag = 0, -1
         for j in range(kry):
+            if j + 2 > Hm.shape[0]:  # row j + 1 is written in this step
+                n2 = min(kry + 1, 2 * Hm.shape[0])
+                H2 = np.zeros((n2, n2), complex)
+                H2[:Hm.shape[0], :Hm.shape[1]] = Hm
+                Hm = H2
+                c, s, g = (np.concatenate([a, np.zeros(n2 - a.size, complex)]) for a in (c, s, g))
             w = afun(V[j])
             Vj, Vcj = V[:j + 1], Vc[:j + 1]
             h1 = t.mv(Vcj, w)
@@ -402,7 +409,8 @@ class BlockOps:
                 flag = -2
                 break
             if j + 1 >= V.shape[0]:
-                grown = t.empty((min(kry + 1, 2 * V.shape[0]), dim), dtype=b.dtype, device=b.device)
+                n2 = min(kry + 1, 2 * V.shape[0])
+                grown = t.empty((n2, dim), dtype=b.dtype, device=b.device)
                 grown[:V.shape[0]] = V
                 V = grown
                 grown = t.empty_like(V)
@@ -434,10 +442,10 @@ class BlockOps:
     def _p(self, z, lay) -> DObj:
         return DObj(lay, self.torch.stack((z.real, z.imag)).contiguous())
 
-    def _tr(self, a: DObj, b: DObj) -> complex:
+    def _tr(self, a: DObj, b: DObj, device: bool = False):
         """trace(a * b) summed over sectors (a real or complex, b real or complex): sum over the
-        matching blocks of A_ij B_ji, as one gather of both operands (cached element pairs) and
-        one host transfer."""
+        matching blocks of A_ij B_ji, as one gather of both operands (cached element pairs); a
+        Python complex after one host transfer, or (device=True) a 0-d complex device tensor."""
         t = self.torch
         key = ("tr", _lkey(a.lay), _lkey(b.lay))
         m = self._maps.get(key)
@@ -467,6 +475,8 @@ class BlockOps:
             re, im = (A * B[0]).sum(), (A * B[1]).sum()
         else:
             re, im = (A * B).sum(), t.zeros((), dtype=A.dtype, device=A.device)
+        if device:
+            return t.complex(re, im)
         v = t.stack((re, im)).cpu().numpy()
         return complex(float(v[0]), float(v[1]))
 
@@ -490,8 +500,8 @@ class BlockOps:
             xin = self._p(z, lay)
             tx = self._c(self.onto(T(xin), lay))
             out = t.sub(z, tx, alpha=fac)  # z - fac T z
-            if proj:
-                tr = self._tr(xin, rin) if direction == LEFT else self._tr(rin, xin)
+            if proj:  # the trace stays on the device: no host round trip per GMRES step
+                tr = self._tr(xin, rin, device=True) if direction == LEFT else self._tr(rin, xin, device=True)
                 out = out + tr * routc
             return out
         z0 = self.torch.zeros_like(b) if y0 is None else self._c(y0)
