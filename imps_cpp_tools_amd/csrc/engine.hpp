@@ -49,6 +49,7 @@ struct PanelLaunch {
     long xs_stride = 0;
     const double* xd = nullptr;  // FP64 input read directly for plane-N chunks (nullptr: from the split copy)
     long xd_stride = 0;          // elements per vector
+    bool xt_direct = false;      // with xd: plane-T chunks read the input directly too (then no split copy is made)
     void* ws = nullptr;
     long ws_stride = 0;
     long plx = 0, plw = 0;

@@ -48,15 +48,22 @@ inline long split_bytes(int prec, long len) { return (prec == IMPS_BF16X3 ? 4 : 
 // PCF_RAW_XN: the raw factor is an input-vector block read k-contiguously (plane N).  In FP64 plane N
 // equals the input itself, so such chunks read the caller's vector directly when it is passed to the
 // kernel (no copy of plane N), else the split copy's plane N.
-enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_RAW_W = 4, PCF_RAW_XN = 8 };
+// PCF_RAW_XT: the raw factor is an input-vector block read along its stored columns (plane T of the
+// split copy).  When the apply passes the caller's vector for these chunks too (few vectors), they
+// read it directly, element (idx, k) at xoff + idx + k * xks: each lane loads its k one by one, and
+// the sixteen lanes of one k read sixteen consecutive elements; no split copy is made then.  FP32
+// and BF16X3 convert the FP64 elements as the split copy does, plane-N chunks included.
+enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_RAW_W = 4, PCF_RAW_XN = 8, PCF_RAW_XT = 16 };
 
-struct PChunk {       // 32 bytes
+struct PChunk {       // 40 bytes
     long raw_off;     // element index, within the (hi) N|T planes, of raw(idx0, k0)
     int pan;          // fragment index of the panel fragment (rg0, kc)
     int pan_sub;      // fragment index step to the second 16-row group (-1 if the panel has none)
     int s_idx;        // raw element stride along the output-side index (k is contiguous)
     int flags;
     int krem;         // valid k in this chunk and after it (K - k0)
+    int xoff;         // PCF_RAW_XT: element offset in the input vector of raw(idx0, k0)
+    int xks;          // PCF_RAW_XT: element stride along k in the input vector (the block's row count)
     int pad;
 };
 
@@ -200,9 +207,13 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                     c.pan = (int)(pd.frag0 + (long)rg0 * pd.nkc + kc);
                     c.pan_sub = rg0 + 1 < pd.nrg ? pd.nkc : -1;  // -1: no second row group
                     c.s_idx = s_idx;
-                    c.flags = (pa ? PCF_PANEL_IS_A : 0) | (rslot == SLOT_W ? PCF_RAW_W : 0) |
-                              (rslot == SLOT_X && kmajor && prec == IMPS_FP64 ? PCF_RAW_XN : 0);
+                    const bool fx = rslot == SLOT_X;
+                    c.flags = (pa ? PCF_PANEL_IS_A : 0) | (rslot == SLOT_W ? PCF_RAW_W : 0) | (fx && kmajor ? PCF_RAW_XN : 0) |
+                              (fx && !kmajor ? PCF_RAW_XT : 0);
                     c.krem = s.K - kc * KC;
+                    // idx-major: S(idx, k) of the stored block (Rb rows, column-major) at boff + idx + k * Rb
+                    c.xoff = fx && !kmajor ? (int)(boff + idx0 + (long)kc * KC * Rb) : 0;
+                    c.xks = fx && !kmajor ? Rb : 0;
                     c.pad = 0;
                     chs.push_back(c);
                 }

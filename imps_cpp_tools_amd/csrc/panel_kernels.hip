@@ -241,10 +241,56 @@ struct Frag {
 
 // Issue every load of one chunk: panel fragments (row groups rg0, rg0 + 1) and the raw fragments
 // of output-side indices idx0 + [0, 16) and idx0 + [16, 32).
+// Raw fragments of an input-vector chunk read from the FP64 input itself (rxn): lane (li, lk) takes
+// raw(idx0 + li (+ 16), k0 + RUN lk + q) for q < RUN.  PCF_RAW_XT: eight-byte loads along the stored
+// columns, each zero (no access) outside the block; PCF_RAW_XN: the lane's k-run is contiguous (16-
+// byte loads; k past K inside a run reads finite neighbouring data that meets zero panel entries, a
+// run wholly past K reads zeros).  FP32 and BF16X3 convert as the split copy does.
+template <int PREC>
+__device__ __forceinline__ void load_raw_x(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rxn, int nidx, int li, int lk) {
+    constexpr int RUN = PTraits<PREC>::RUN;
+    const bool xt = (c.flags & PCF_RAW_XT) != 0;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const int idx = li + 16 * s;
+        double d[RUN];
+        if (xt) {
+#pragma unroll
+            for (int q = 0; q < RUN; ++q) {
+                const int k = RUN * lk + q;
+                const unsigned off = (idx < nidx && k < c.krem) ? (unsigned)(c.xoff + idx + k * c.xks) * 8u : kOOB;
+                d[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rxn, off, 0, 0));
+            }
+        } else {
+            const bool ok = idx < nidx && RUN * lk < c.krem;
+            const unsigned off = ok ? (unsigned)(c.raw_off + (long)idx * c.s_idx + RUN * lk) * 8u : kOOB;
+#pragma unroll
+            for (int q = 0; q < RUN; q += 2) {
+                const pd2_t v = __builtin_bit_cast(pd2_t, ld16(rxn, ok ? off + 8u * q : kOOB));
+                d[q] = v[0];
+                d[q + 1] = v[1];
+            }
+        }
+        if constexpr (PREC == IMPS_FP64) {
+            F.r[s].v = pd4_t{d[0], d[1], d[2], d[3]};
+        } else if constexpr (PREC == IMPS_FP32) {
+            F.r[s].v = pf4_t{(float)d[0], (float)d[1], (float)d[2], (float)d[3]};
+        } else {
+#pragma unroll
+            for (int q = 0; q < RUN; ++q) {
+                __bf16 h, l;
+                split_bf16((float)d[q], h, l);
+                F.r[s].hi[q] = h;
+                F.r[s].lo[q] = l;
+            }
+        }
+    }
+}
+
 template <int PREC>
 __device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
                                            __amdgpu_buffer_rsrc_t rxn, __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx,
-                                           int li, int lk, int lane) {
+                                           int li, int lk, int lane, bool xdirect) {
     using TR = PTraits<PREC>;
     constexpr int RUN = TR::RUN;
     constexpr unsigned FB = PREC == IMPS_FP32 ? 64 * 16 : 64 * 32;  // bytes of one panel fragment
@@ -254,8 +300,14 @@ __device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amd
     const unsigned p1 = c.pan_sub >= 0 ? (unsigned)(c.pan + c.pan_sub) * FB + LB * lane : kOOB;
     F.p[0] = load_frg<PREC>(rp, p0, 1024u);
     F.p[1] = load_frg<PREC>(rp, p1, 1024u);
+    // every input-vector chunk from the input itself (few vectors: no split copy), or in FP64 the
+    // plane-N ones (their elements are the input's own)
+    if (xdirect && (c.flags & (PCF_RAW_XT | (PREC == IMPS_FP64 ? 0 : PCF_RAW_XN)))) {
+        load_raw_x(F, c, rxn, nidx, li, lk);
+        return;
+    }
     const bool w = (c.flags & PCF_RAW_W) != 0;
-    const __amdgpu_buffer_rsrc_t rs = w ? rw : ((c.flags & PCF_RAW_XN) ? rxn : rx);
+    const __amdgpu_buffer_rsrc_t rs = w ? rw : ((PREC == IMPS_FP64 && (c.flags & PCF_RAW_XN)) ? rxn : rx);
     const unsigned lo = w ? wlo : xlo;
     const bool kok = RUN * lk < c.krem;
     const unsigned e0 = (unsigned)(c.raw_off + (long)li * c.s_idx + RUN * lk) * EB;
@@ -418,6 +470,7 @@ struct PArgs {
     long xs_stride;  // bytes per vector
     const char* xd;  // FP64 input vectors read directly by plane-N chunks, or nullptr
     long xd_stride;  // bytes per vector
+    int xt;          // with xd: plane-T chunks (PCF_RAW_XT) read the input directly too (no split copy)
     const char* ws;  // split copy of the workspace
     long ws_stride;
     long plx, plw;   // plane lengths (elements)
@@ -454,24 +507,25 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
         const __amdgpu_buffer_rsrc_t rw = rsrc(A.ws + A.ws_stride * v, A.ws_stride);
         const __amdgpu_buffer_rsrc_t rp = rsrc(A.pstore, A.pstore_bytes);
         const unsigned xlo = (unsigned)(2 * A.plx) * EB, wlo = (unsigned)(2 * A.plw) * EB;  // lo planes (BF16X3)
+        const bool xdirect = A.xd != nullptr && A.xt != 0;
         const int nrow = t.M - t.m0, ncol = t.N - t.n0;  // raw-side extent: columns if the panel is A, rows otherwise
         const int cend = t.ch0 + t.nch;
         {
             int c = t.ch0 + part;
             Frag<PREC> F0, F1;
             PChunk d0 = A.chunks[c];
-            load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+            load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane, xdirect);
             while (true) {
                 // prefetch the next chunk (the current one again at the end: same count of loads)
                 const int c1 = c + S < cend ? c + S : c;
                 const PChunk d1 = A.chunks[c1];
-                load_chunk<PREC>(F1, d1, rp, rx, rxn, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                load_chunk<PREC>(F1, d1, rp, rx, rxn, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane, xdirect);
                 compute_chunk_n<PREC>(acc, F0, (d0.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c1;
                 const int c2 = c + S < cend ? c + S : c;
                 d0 = A.chunks[c2];
-                load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane, xdirect);
                 compute_chunk_n<PREC>(acc, F1, (d1.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c2;
@@ -556,8 +610,12 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_shared_kernel(const P
     };
     // raw fragments of output-side indices idx0 + [0, 16) and idx0 + [16, 32) (zeros when inactive)
     auto load_raw = [&](Frag<PREC>& F, const PChunk& c) {
+        if (A.xd && A.xt && (c.flags & (PCF_RAW_XT | (PREC == IMPS_FP64 ? 0 : PCF_RAW_XN)))) {
+            load_raw_x(F, c, rxn, active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0, li, lk);
+            return;
+        }
         const bool w = (c.flags & PCF_RAW_W) != 0;
-        const __amdgpu_buffer_rsrc_t rs = w ? rw : ((c.flags & PCF_RAW_XN) ? rxn : rx);
+        const __amdgpu_buffer_rsrc_t rs = w ? rw : ((PREC == IMPS_FP64 && (c.flags & PCF_RAW_XN)) ? rxn : rx);
         const unsigned lo = w ? wlo : xlo;
         const int nidx = active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0;
         const bool kok = RUN * lk < c.krem;
@@ -724,6 +782,7 @@ hipError_t launch_panel_stage(int prec, const DevPStage& st, int ksplit, bool sh
     a.xs_stride = L.xs_stride;
     a.xd = (const char*)L.xd;
     a.xd_stride = L.xd_stride * (long)sizeof(double);
+    a.xt = L.xt_direct ? 1 : 0;
     a.ws = (const char*)L.ws;
     a.ws_stride = L.ws_stride;
     a.plx = L.plx;

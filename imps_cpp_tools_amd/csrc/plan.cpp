@@ -165,6 +165,9 @@ void upload_panel_program(imps_plan_s* P) {
 
 }  // namespace
 
+// FP64 applies of at most this many vectors read every raw block from x itself (no split copy)
+constexpr int kDirectTMaxVectors = 2;
+
 // true when the element ranges [a, a + na) and [b, b + nb) share memory
 static bool overlaps(const double* a, long na, const double* b, long nb) {
     const uintptr_t a0 = (uintptr_t)a, a1 = a0 + sizeof(double) * (size_t)na;
@@ -257,11 +260,18 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
             hip_check(launch_zero((double*)P->d_ws.p, wsb * nvec / 8, s), "zero workspace");
             P->ws_ready = nvec;
         }
-        // FP64: plane-N chunks read x itself unless y overlaps it (in-place applies keep the copy)
-        const bool direct = P->prec == IMPS_FP64 && !overlaps(x, P->prog.x_elems * nvec, y, P->prog.y_elems * nvec);
-        hip_check(launch_split(P->prec, P->d_xunits, (int)Q.xunits.size(), P->d_xspans, x, P->prog.x_elems, P->d_xs.p, xsb, Q.plx, nvec, s,
-                               !direct),
-                  "split copy");
+        // Unless y overlaps x (in-place applies keep the split copy): FP64 plane-N chunks read x
+        // itself; at one or two vectors every input-vector chunk reads x (plane-T ones along its
+        // stored columns; FP32 / BF16X3 convert on load) and no split copy is made (measured at
+        // D = 1024 H_AC in FP64: 13.7 us per single-vector apply against 13.4 + 5 us with the copy;
+        // with 16 vectors the copy pays: 60 + 9.5 us against 68 us)
+        const bool apart = !overlaps(x, P->prog.x_elems * nvec, y, P->prog.y_elems * nvec);
+        const bool xt = apart && nvec <= kDirectTMaxVectors;
+        const bool direct = apart && (P->prec == IMPS_FP64 || xt);
+        if (!xt)
+            hip_check(launch_split(P->prec, P->d_xunits, (int)Q.xunits.size(), P->d_xspans, x, P->prog.x_elems, P->d_xs.p, xsb, Q.plx, nvec, s,
+                                   !direct),
+                      "split copy");
         PanelLaunch L;
         L.pstore = P->d_pstore.p;
         L.pstore_bytes = Q.nfrag * panel_frag_bytes(P->prec);
@@ -269,6 +279,7 @@ void run(imps_plan_s* P, const double* x, double* y, int nvec, const double* op_
         L.xs_stride = xsb;
         L.xd = direct ? x : nullptr;
         L.xd_stride = P->prog.x_elems;
+        L.xt_direct = xt;
         L.ws = P->d_ws.p;
         L.ws_stride = wsb;
         L.plx = Q.plx;
@@ -315,11 +326,15 @@ void run_static(imps_plan_s* P, const double* x, double* y, int nvec, void* scra
     hip_check(hipSetDevice(P->ctx->device), "hipSetDevice");
     hipStream_t s = P->ctx->stream;
     const PanelProgram& Q = P->pprog;
-    hip_check(launch_zero((double*)scratch, xsb * nvec / 8, s), "clear scratch");
-    const bool direct = P->prec == IMPS_FP64 && !overlaps(x, P->prog.x_elems * nvec, y, P->prog.y_elems * nvec);
-    hip_check(launch_split(P->prec, P->d_xunits, (int)Q.xunits.size(), P->d_xspans, x, P->prog.x_elems, scratch, xsb, Q.plx, nvec, s,
-                           !direct),
-              "split copy");
+    const bool apart = !overlaps(x, P->prog.x_elems * nvec, y, P->prog.y_elems * nvec);  // as in run()
+    const bool xt = apart && nvec <= kDirectTMaxVectors;
+    const bool direct = apart && (P->prec == IMPS_FP64 || xt);
+    if (!xt) {
+        hip_check(launch_zero((double*)scratch, xsb * nvec / 8, s), "clear scratch");
+        hip_check(launch_split(P->prec, P->d_xunits, (int)Q.xunits.size(), P->d_xspans, x, P->prog.x_elems, scratch, xsb, Q.plx, nvec, s,
+                               !direct),
+                  "split copy");
+    }
     PanelLaunch L;
     L.pstore = P->d_pstore.p;
     L.pstore_bytes = Q.nfrag * panel_frag_bytes(P->prec);
@@ -327,6 +342,7 @@ void run_static(imps_plan_s* P, const double* x, double* y, int nvec, void* scra
     L.xs_stride = xsb;
     L.xd = direct ? x : nullptr;
     L.xd_stride = P->prog.x_elems;
+    L.xt_direct = xt;
     L.ws = nullptr;
     L.ws_stride = 0;
     L.plx = Q.plx;
