@@ -187,7 +187,8 @@ def main():
     def roofline(name):
         r = results[name]
         ks, sh = r["schedule"]["batched"]
-        kname = f"panel_gemm_shared_kernel<{pidx[name]}>" if sh else f"panel_gemm_kernel<{pidx[name]}, {ks}>"  # rocprof's template names
+        # rocprof's template names; batched applies read the split copy (direct input reads: false)
+        kname = f"panel_gemm_shared_kernel<{pidx[name]}, false>" if sh else f"panel_gemm_kernel<{pidx[name]}, {ks}, false>"
         ach = r["batched"]["kernel_tflops"]
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[name], "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS[name],
                 "traffic": committed_traffic(kname, NV), "kernel": kname,

@@ -239,8 +239,6 @@ struct Frag {
     typename PTraits<PREC>::Frg p[2], r[2];
 };
 
-// Issue every load of one chunk: panel fragments (row groups rg0, rg0 + 1) and the raw fragments
-// of output-side indices idx0 + [0, 16) and idx0 + [16, 32).
 // Raw fragments of an input-vector chunk read from the FP64 input itself (rxn): lane (li, lk) takes
 // raw(idx0 + li (+ 16), k0 + RUN lk + q) for q < RUN.  PCF_RAW_XT: eight-byte loads along the stored
 // columns, each zero (no access) outside the block; PCF_RAW_XN: the lane's k-run is contiguous (16-
@@ -287,24 +285,22 @@ __device__ __forceinline__ void load_raw_x(Frag<PREC>& F, const PChunk& c, __amd
     }
 }
 
-template <int PREC>
-__device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
-                                           __amdgpu_buffer_rsrc_t rxn, __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx,
-                                           int li, int lk, int lane, bool xdirect) {
-    using TR = PTraits<PREC>;
-    constexpr int RUN = TR::RUN;
-    constexpr unsigned FB = PREC == IMPS_FP32 ? 64 * 16 : 64 * 32;  // bytes of one panel fragment
-    constexpr unsigned LB = PREC == IMPS_FP64 ? 32 : 16;            // bytes of one lane's run (hi plane)
+// The raw fragments of one chunk: output-side indices idx0 + [0, 16) and idx0 + [16, 32) (nidx of
+// them valid).  XD: the apply reads every input-vector chunk from the input itself (instantiated
+// apart, so that the split-copy kernels carry none of its registers); otherwise from the split copy
+// (rx), the workspace's (rw) or, FP64 plane-N chunks, the input (rxn).  Resources are passed and
+// selected by value: a selected reference to one (a lambda capture) makes the compiler keep the
+// candidates in scratch memory and load the chosen one.
+template <int PREC, bool XD>
+__device__ __forceinline__ void load_raw_frags(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rxn,
+                                               __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx, int li, int lk) {
+    constexpr int RUN = PTraits<PREC>::RUN;
     constexpr unsigned EB = PREC == IMPS_FP64 ? 8 : PREC == IMPS_FP32 ? 4 : 2;
-    const unsigned p0 = (unsigned)c.pan * FB + LB * lane;
-    const unsigned p1 = c.pan_sub >= 0 ? (unsigned)(c.pan + c.pan_sub) * FB + LB * lane : kOOB;
-    F.p[0] = load_frg<PREC>(rp, p0, 1024u);
-    F.p[1] = load_frg<PREC>(rp, p1, 1024u);
-    // every input-vector chunk from the input itself (few vectors: no split copy), or in FP64 the
-    // plane-N ones (their elements are the input's own)
-    if (xdirect && (c.flags & (PCF_RAW_XT | (PREC == IMPS_FP64 ? 0 : PCF_RAW_XN)))) {
-        load_raw_x(F, c, rxn, nidx, li, lk);
-        return;
+    if constexpr (XD) {
+        if (c.flags & (PCF_RAW_XT | (PREC == IMPS_FP64 ? 0 : PCF_RAW_XN))) {
+            load_raw_x(F, c, rxn, nidx, li, lk);
+            return;
+        }
     }
     const bool w = (c.flags & PCF_RAW_W) != 0;
     const __amdgpu_buffer_rsrc_t rs = w ? rw : ((PREC == IMPS_FP64 && (c.flags & PCF_RAW_XN)) ? rxn : rx);
@@ -314,6 +310,20 @@ __device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amd
     const unsigned es = (unsigned)(16 * c.s_idx) * EB;
     F.r[0] = load_frg<PREC>(rs, (kok && li < nidx) ? e0 : kOOB, lo);
     F.r[1] = load_frg<PREC>(rs, (kok && 16 + li < nidx) ? e0 + es : kOOB, lo);
+}
+
+// Issue every load of one chunk: panel fragments (row groups rg0, rg0 + 1) and the raw fragments.
+template <int PREC, bool XD>
+__device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
+                                           __amdgpu_buffer_rsrc_t rxn, __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx,
+                                           int li, int lk, int lane) {
+    constexpr unsigned FB = PREC == IMPS_FP32 ? 64 * 16 : 64 * 32;  // bytes of one panel fragment
+    constexpr unsigned LB = PREC == IMPS_FP64 ? 32 : 16;            // bytes of one lane's run (hi plane)
+    const unsigned p0 = (unsigned)c.pan * FB + LB * lane;
+    const unsigned p1 = c.pan_sub >= 0 ? (unsigned)(c.pan + c.pan_sub) * FB + LB * lane : kOOB;
+    F.p[0] = load_frg<PREC>(rp, p0, 1024u);
+    F.p[1] = load_frg<PREC>(rp, p1, 1024u);
+    load_raw_frags<PREC, XD>(F, c, rx, rxn, rw, xlo, wlo, nidx, li, lk);
 }
 
 // acc[a][b] += (col-side fragment b)(row-side fragment a), the transposed sub-tile (a, b), for
@@ -480,7 +490,7 @@ struct PArgs {
     int nvec, nvg;
 };
 
-template <int PREC, int S>
+template <int PREC, int S, bool XD>
 __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A) {
     using Acc = typename PTraits<PREC>::Acc;
     constexpr int V = kPWaves / S;  // vectors per workgroup
@@ -507,25 +517,24 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
         const __amdgpu_buffer_rsrc_t rw = rsrc(A.ws + A.ws_stride * v, A.ws_stride);
         const __amdgpu_buffer_rsrc_t rp = rsrc(A.pstore, A.pstore_bytes);
         const unsigned xlo = (unsigned)(2 * A.plx) * EB, wlo = (unsigned)(2 * A.plw) * EB;  // lo planes (BF16X3)
-        const bool xdirect = A.xd != nullptr && A.xt != 0;
         const int nrow = t.M - t.m0, ncol = t.N - t.n0;  // raw-side extent: columns if the panel is A, rows otherwise
         const int cend = t.ch0 + t.nch;
         {
             int c = t.ch0 + part;
             Frag<PREC> F0, F1;
             PChunk d0 = A.chunks[c];
-            load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane, xdirect);
+            load_chunk<PREC, XD>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
             while (true) {
                 // prefetch the next chunk (the current one again at the end: same count of loads)
                 const int c1 = c + S < cend ? c + S : c;
                 const PChunk d1 = A.chunks[c1];
-                load_chunk<PREC>(F1, d1, rp, rx, rxn, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane, xdirect);
+                load_chunk<PREC, XD>(F1, d1, rp, rx, rxn, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
                 compute_chunk_n<PREC>(acc, F0, (d0.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c1;
                 const int c2 = c + S < cend ? c + S : c;
                 d0 = A.chunks[c2];
-                load_chunk<PREC>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane, xdirect);
+                load_chunk<PREC, XD>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
                 compute_chunk_n<PREC>(acc, F1, (d1.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c2;
@@ -569,7 +578,7 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
 // workgroup into a double-buffered LDS copy (16-byte units, fragment f at units [f*HU, (f+1)*HU))
 // and every wave reads them from there; each wave loads only its own vector's raw fragments.
 // Every wave, including those past the last vector, takes part in the loads and barriers.
-template <int PREC>
+template <int PREC, bool XD>
 __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_shared_kernel(const PArgs A) {
     using TR = PTraits<PREC>;
     using Acc = typename TR::Acc;
@@ -610,19 +619,7 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_shared_kernel(const P
     };
     // raw fragments of output-side indices idx0 + [0, 16) and idx0 + [16, 32) (zeros when inactive)
     auto load_raw = [&](Frag<PREC>& F, const PChunk& c) {
-        if (A.xd && A.xt && (c.flags & (PCF_RAW_XT | (PREC == IMPS_FP64 ? 0 : PCF_RAW_XN)))) {
-            load_raw_x(F, c, rxn, active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0, li, lk);
-            return;
-        }
-        const bool w = (c.flags & PCF_RAW_W) != 0;
-        const __amdgpu_buffer_rsrc_t rs = w ? rw : ((PREC == IMPS_FP64 && (c.flags & PCF_RAW_XN)) ? rxn : rx);
-        const unsigned lo = w ? wlo : xlo;
-        const int nidx = active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0;
-        const bool kok = RUN * lk < c.krem;
-        const unsigned e0 = (unsigned)(c.raw_off + (long)li * c.s_idx + RUN * lk) * EB;
-        const unsigned es = (unsigned)(16 * c.s_idx) * EB;
-        F.r[0] = load_frg<PREC>(rs, (kok && li < nidx) ? e0 : kOOB, lo);
-        F.r[1] = load_frg<PREC>(rs, (kok && 16 + li < nidx) ? e0 + es : kOOB, lo);
+        load_raw_frags<PREC, XD>(F, c, rx, rxn, rw, xlo, wlo, active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0, li, lk);
     };
     // panel fragments of the chunk in LDS buffer b, in the lane layout of load_frg
     auto read_panels = [&](Frag<PREC>& F, int b) {
@@ -705,13 +702,14 @@ hipError_t launch_ps(const DevPStage& st, PArgs a, bool shared, hipStream_t stre
     const long nblk = (long)st.ntasks * a.nvg;
     if (nblk > 0x7fffffffL) return hipErrorInvalidValue;
     if constexpr (S == 1) {
-        if (shared)
-            hipLaunchKernelGGL((panel_gemm_shared_kernel<PREC>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
-        else
-            hipLaunchKernelGGL((panel_gemm_kernel<PREC, S>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
-    } else {
-        hipLaunchKernelGGL((panel_gemm_kernel<PREC, S>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+        if (shared) {
+            if (a.xt) hipLaunchKernelGGL((panel_gemm_shared_kernel<PREC, true>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+            else hipLaunchKernelGGL((panel_gemm_shared_kernel<PREC, false>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+            return hipGetLastError();
+        }
     }
+    if (a.xt) hipLaunchKernelGGL((panel_gemm_kernel<PREC, S, true>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
+    else hipLaunchKernelGGL((panel_gemm_kernel<PREC, S, false>), dim3((unsigned)nblk), dim3(64 * kPWaves), 0, stream, a);
     return hipGetLastError();
 }
 
