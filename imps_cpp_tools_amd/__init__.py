@@ -113,6 +113,8 @@ def lib():
             "imps_heff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I, I]),
             "imps_opeff_plan": (I, [P, ctypes.POINTER(P), I, I, LP, LP, LP, LP, dp, D, ip, I, I]),
             "imps_heff_set_state": (I, [P, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), D]),
+            "imps_heff_set_state_lr": (I, [P, ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), ctypes.POINTER(V), LP, V, LP, V,
+                                           LP, V, D]),
             "imps_heff_check": (I, [P, dp, dp, ip]),
             "imps_heff_apply": (I, [P, dp, V, V, ip, dp]),
             "imps_heff_eigs": (I, [P, dp, I, D, I, I, dp, V, dp, ip]),
@@ -680,15 +682,31 @@ class HeffPlan(Plan):
         self.xdims = self._out_layout(AL[0].nsym, AL[0].periods)
         self.mtot = self.xdims.nelem
 
-    def set_state(self, AL, AR, NL, C, inv_tol: float = 1e-14):
+    def set_state(self, AL, AR, NL, C, inv_tol: float = 1e-14, L=None, LM=None, RM=None):
+        """Copies the state and computes the constants.  With AL C = C AR the bond objects are taken in
+        closed form; for a transformed or shifted AR pass L, LM and RM as (layout, device tensor)
+        pairs (imps_heff_set_state_lr; `lr_overlaps` and `apply_transformation` in
+        imps_cpp_tools_amd.transform make them)."""
         lays = self._keep_layouts
 
         def arr(ts, ls, what):
             if len(ts) != self.N:
                 raise ValueError(f"heff {what}: {len(ts)} tensors for a cell of {self.N}")
             return (ctypes.c_void_p * self.N)(*[self._p(t, l.nelem, f"{what}[{k}]") for k, (t, l) in enumerate(zip(ts, ls))])
-        _check(lib().imps_heff_set_state(self.h, arr(AL, lays[0], "AL"), arr(AR, lays[1], "AR"), arr(NL, lays[2], "NL"),
-                                         arr(C, lays[3], "C"), float(inv_tol)))
+        a = (arr(AL, lays[0], "AL"), arr(AR, lays[1], "AR"), arr(NL, lays[2], "NL"), arr(C, lays[3], "C"))
+        given = [x is not None for x in (L, LM, RM)]
+        if not any(given):
+            _check(lib().imps_heff_set_state(self.h, *a, float(inv_tol)))
+            return
+        if not all(given):
+            raise ValueError("heff: L, LM and RM are given together")
+        cl = [x[0].c() for x in (L, LM, RM)]
+        bond = []
+        for c, (lay, t), what in zip(cl, (L, LM, RM), ("L", "LM", "RM")):
+            if lay.nlist != 1:
+                raise ValueError(f"heff {what}: a bond object has nlist = 1")
+            bond += [ctypes.byref(c), self._p(t, lay.nelem, what)]
+        _check(lib().imps_heff_set_state_lr(self.h, *a, *bond, float(inv_tol)))
 
     def check(self):
         """(tr(HL R), tr(L HR), environment-solver applications) of HeffConstants' check."""

@@ -539,6 +539,27 @@ imps_status imps_heff_set_state(imps_plan P, const double* const* AL, const doub
     });
 }
 
+imps_status imps_heff_set_state_lr(imps_plan P, const double* const* AL, const double* const* AR, const double* const* NL,
+                                   const double* const* C, const imps_layout* Llay, const double* L, const imps_layout* LMlay,
+                                   const double* LM, const imps_layout* RMlay, const double* RM, double inv_tol) {
+    return guarded([&] {
+        check_kind(P, K_HEFF);
+        if (!AL || !AR || !NL || !C) throw InvalidArg("tensor arrays are NULL");
+        HeffEngine::BondInput b;
+        b.Llay = layout_from_c(Llay, "heff L");
+        b.LMlay = layout_from_c(LMlay, "heff LM");
+        b.RMlay = layout_from_c(RMlay, "heff RM");
+        b.L = L;
+        b.LM = LM;
+        b.RM = RM;
+        const int N = P->heff->N;
+        P->heff->set_state(std::vector<const double*>(AL, AL + N), std::vector<const double*>(AR, AR + N),
+                           std::vector<const double*>(NL, NL + N), std::vector<const double*>(C, C + N), inv_tol, &b);
+        P->op_set = true;
+        return IMPS_OK;
+    });
+}
+
 imps_status imps_heff_apply(imps_plan P, const double* p, const double* x, double* y, int* gmres_steps, double* gmres_relres) {
     return guarded([&] {
         check_kind(P, K_HEFF);

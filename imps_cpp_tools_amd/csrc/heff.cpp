@@ -58,6 +58,43 @@ Layout from_maps(int nsym, const int* per, const std::vector<std::map<Key, Blk>>
     return L;
 }
 
+// Diagonal layout of one bond of an MPS layout: one square block per charge, from the blocks' right
+// (outgoing) or left (ingoing) side
+Layout bond_layout(const Layout& A, bool right_side) {
+    std::vector<std::map<Key, Blk>> ent(1);
+    for (const BList& e : A.l)
+        for (const Blk& b : e.b) {
+            const Key q = right_side ? b.qout : b.qin;
+            const int m = right_side ? b.cols : b.rows;
+            auto it = ent[0].find(q);
+            if (it == ent[0].end()) ent[0].emplace(q, Blk{q, q, m, m, 0});
+            else if (it->second.rows != m) throw InvalidArg("heff: a bond sector with two dimensions");
+        }
+    return from_maps(A.nsym, A.per, ent);
+}
+
+// Charge-conserving blocks (q, q) between two bonds: rows from bond a, columns from bond b
+Layout cross_layout(const Layout& a, const Layout& b) {
+    std::vector<std::map<Key, Blk>> ent(1);
+    for (const Blk& x : a.l[0].b)
+        if (const Blk* y = b.l[0].find(x.qin)) ent[0].emplace(x.qin, Blk{x.qin, x.qin, x.rows, y->rows, 0});
+    return from_maps(a.nsym, a.per, ent);
+}
+
+// Host copy of `src` (layout sl) onto layout dl by charge: every block of sl must exist in dl with
+// the same shape; blocks of dl without a source stay zero
+std::vector<double> map_blocks(const Layout& sl, const std::vector<double>& src, const Layout& dl, const char* what) {
+    std::vector<double> out(dl.nelem, 0.0);
+    if (sl.l.size() != 1) throw InvalidArg(std::string("heff: ") + what + " needs nlist = 1");
+    for (const Blk& b : sl.l[0].b) {
+        const Blk* t = dl.l[0].find(b.qin);
+        if (!t || t->qout != b.qout || t->rows != b.rows || t->cols != b.cols)
+            throw InvalidArg(std::string("heff: ") + what + " has a block outside the bond structure of AL and AR");
+        std::copy(src.begin() + b.off, src.begin() + b.off + b.size(), out.begin() + t->off);
+    }
+    return out;
+}
+
 long entry_start(const Layout& L, int e) {
     for (int k = e; k < (int)L.l.size(); ++k)
         if (!L.l[k].b.empty()) return L.l[k].b.front().off;
@@ -354,8 +391,7 @@ XObj HeffEngine::redop_right_mixed(const XObj& Aket, const XObj& Bbra) {
     return comb(rows);
 }
 
-XObj HeffEngine::invert_e(const std::vector<XObj>& As, const std::vector<XObj>& Bs, const XObj& x, int dir, int pow, const XObj* L,
-                          const XObj* R) {
+XObj HeffEngine::invert_e(const std::vector<XObj>& As, const std::vector<XObj>& Bs, const XObj& x, int dir, int pow, bool proj) {
     auto g = std::make_unique<GeoOp>();
     g->dir = dir;
     g->pow = pow;
@@ -390,11 +426,8 @@ XObj HeffEngine::invert_e(const std::vector<XObj>& As, const std::vector<XObj>& 
         g->chain_ops.push_back({A, B});
         in = out;
     }
-    if (L && R) {
-        // left: rho_in = R, rho_out = L; right: rho_in = L, rho_out = R (InvEBlockFunctions.hpp:243, :252)
+    if (proj) {
         g->proj = true;
-        g->rin = dir == IMPS_LEFT ? *R : *L;
-        g->rout_src = dir == IMPS_LEFT ? *L : *R;
         g->w = make(x.lay, false);
         g->rout = make(x.lay, false);
     }
@@ -506,6 +539,10 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
             Xout_v.push_back(view_entry(Xout, n));
         }
     }
+    bL = bond_layout(ALl[N - 1], true);
+    bR = bond_layout(ARl[0], false);
+    lmL = cross_layout(bR, bL);
+    rmL = cross_layout(bL, bR);
     coef(1.0, 0);
     rec = &setup_tape;
     record_setup();
@@ -540,30 +577,31 @@ void HeffEngine::record_setup() {
         HR[N - 1] = opr(AR[N - 1]);
         for (int n = N - 2; n >= 0; --n) HR[n] = add(opr(AR[n]), diag_part(tmr(AR[n], AR[n], HR[n + 1])));
     }
-    const Layout& b0 = Cl[N - 1];
-    const XObj Rr = prod(PROD_MPS_MATT, C[N - 1], C[N - 1]);  // C C^T
-    const XObj Ll = prod(PROD_MATT_MPS, C[N - 1], C[N - 1]);  // C^T C
-    one = make(b0, false);
-    LMo = make(b0, false);
-    RMo = C[N - 1];
-    const XObj xl = onto(HL[N - 1], b0), rl = onto(Rr, b0);
-    const XObj xr = onto(HR[0], b0), lr = onto(Ll, b0);
-    const XObj yl = make(b0, false), yr = make(b0, false);
+    // bond objects (filled by set_state): identities, R = C C^T on AL's side, L on AR's side, and the
+    // dominant eigenmatrices of the mixed TM
+    oneL = make(bL, false);
+    oneR = make(bR, false);
+    Rb = make(bL, false);
+    Lb = make(bR, false);
+    LMo = make(lmL, false);
+    RMo = make(rmL, false);
+    const XObj xl = onto(HL[N - 1], bL), xr = onto(HR[0], bR);
+    const XObj yl = make(bL, false), yr = make(bR, false);
     envL = std::make_unique<EnvSolver>();
-    envL->build(ctx, IMPS_LEFT, ALl, b0, IMPS_FP64);
+    envL->build(ctx, IMPS_LEFT, ALl, bL, IMPS_FP64);
     envR = std::make_unique<EnvSolver>();
-    envR->build(ctx, IMPS_RIGHT, ARl, b0, IMPS_FP64);
+    envR->build(ctx, IMPS_RIGHT, ARl, bR, IMPS_FP64);
     checkHL = HL[N - 1];
     checkHR = HR[0];
-    checkR = Rr;
-    checkL = Ll;
-    calls.push_back([this, xl, rl, yl]() {
-        KrylovStats st = envL->solve(A() + xl.off, A() + one.off, A() + rl.off, nullptr, A() + yl.off, inv_tol, 20000, 100);
+    checkR = Rb;
+    checkL = Lb;
+    calls.push_back([this, xl, yl]() {
+        KrylovStats st = envL->solve(A() + xl.off, A() + oneL.off, A() + Rb.off, nullptr, A() + yl.off, inv_tol, 20000, 100);
         env_iters += st.iters;
     });
     rec->push_back(Op{3, (int)calls.size() - 1});
-    calls.push_back([this, xr, lr, yr]() {
-        KrylovStats st = envR->solve(A() + xr.off, A() + lr.off, A() + one.off, nullptr, A() + yr.off, inv_tol, 20000, 100);
+    calls.push_back([this, xr, yr]() {
+        KrylovStats st = envR->solve(A() + xr.off, A() + Lb.off, A() + oneR.off, nullptr, A() + yr.off, inv_tol, 20000, 100);
         env_iters += st.iters;
     });
     rec->push_back(Op{3, (int)calls.size() - 1});
@@ -616,7 +654,7 @@ void HeffEngine::record_apply_observable() {
     std::vector<XObj> ABR(N);
     ABR[N - 1] = prod(PROD_TM_RIGHT, Bin[N - 1], AR[N - 1]);
     for (int n = N - 1; n > 0; --n) ABR[n - 1] = prod(PROD_TM_RIGHT, add(Bin[n - 1], prod(PROD_MPS_MAT, AL[n - 1], ABR[n])), AR[n - 1]);
-    XObj EBR = invert_e(AL, AR, ABR[0], IMPS_RIGHT, +1, &LMo, &RMo);
+    XObj EBR = invert_e(AL, AR, ABR[0], IMPS_RIGHT, +1, true);
     std::vector<XObj> ABRtot(N);
     ABRtot[0] = scale(EBR, 1.0, +1);
     for (int n = N - 1; n > 0; --n) {
@@ -629,7 +667,7 @@ void HeffEngine::record_apply_observable() {
     for (int n = 1; n < N; ++n)
         OPBL[n] = add(prod(PROD_TM_LEFT, add(prod(PROD_MAT_MPS, OPBL[n - 1], AR[n]), prod(PROD_MAT_MPS, HLtot[n - 1], Bin[n])), AL[n]),
                       prod(PROD_TM_LEFT, apply_site_operator(Bin[n]), AL[n]));
-    XObj EOPBL = invert_e(AR, AL, OPBL[N - 1], IMPS_LEFT, -1, &RMo, &LMo);
+    XObj EOPBL = invert_e(AR, AL, OPBL[N - 1], IMPS_LEFT, -1, true);
     // Bin and Bout on the same site
     std::vector<XObj> Bout(N);
     for (int n = 0; n < N; ++n) {
@@ -660,7 +698,7 @@ void HeffEngine::record_apply() {
     std::vector<XObj> ABR(N);
     ABR[N - 1] = prod(PROD_TM_RIGHT, Bin[N - 1], AR[N - 1]);
     for (int n = N - 1; n > 0; --n) ABR[n - 1] = prod(PROD_TM_RIGHT, add(Bin[n - 1], prod(PROD_MPS_MAT, AL[n - 1], ABR[n])), AR[n - 1]);
-    XObj EBR = invert_e(AL, AR, ABR[0], IMPS_RIGHT, +1, &LMo, &RMo);
+    XObj EBR = invert_e(AL, AR, ABR[0], IMPS_RIGHT, +1, true);
     std::vector<XObj> ABRtot(N);
     ABRtot[0] = scale(EBR, 1.0, +1);
     for (int n = N - 1; n > 0; --n) {
@@ -675,7 +713,7 @@ void HeffEngine::record_apply() {
     for (int n = 1; n < N; ++n)
         HBL[n] = add(prod(PROD_TM_LEFT, add(prod(PROD_MAT_MPS, HBL[n - 1], AR[n]), prod(PROD_MAT_MPS, HLtot[n - 1], Bin[n])), AL[n]),
                      prod(PROD_TM_LEFT, apply_operator(add(prod(PROD_MPS_MPS, AL[n - 1], Bin[n]), prod(PROD_MPS_MPS, Bin[n - 1], AR[n]))), AAL[n]));
-    XObj EHBL = invert_e(AR, AL, HBL[N - 1], IMPS_LEFT, -1, &RMo, &LMo);
+    XObj EHBL = invert_e(AR, AL, HBL[N - 1], IMPS_LEFT, -1, true);
     std::vector<XObj> HBLtot(N);
     HBLtot[N - 1] = scale(EHBL, 1.0, -1);
     for (int n = 0; n < N - 1; ++n) {
@@ -811,7 +849,7 @@ void HeffEngine::run_other(const Op& op) {
 }
 
 void HeffEngine::set_state(const std::vector<const double*>& al, const std::vector<const double*>& ar, const std::vector<const double*>& nl,
-                           const std::vector<const double*>& cl, double tol) {
+                           const std::vector<const double*>& cl, double tol, const BondInput* bond) {
     if ((int)al.size() != N || (int)ar.size() != N || (int)nl.size() != N || (int)cl.size() != N)
         throw InvalidArg("heff: one tensor per site is needed");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
@@ -826,19 +864,71 @@ void HeffEngine::set_state(const std::vector<const double*>& al, const std::vect
                           "copy state");
         }
     }
-    // host-side bond-0 constants: identity, LM = C^T blockwise (with RM = C the dominant eigenmatrices
-    // of the mixed TM in K = 0, tr(LM RM) = 1), and the projection weights of the geometric sums
-    const Layout& b0 = Cl[N - 1];
-    std::vector<double> ch(b0.nelem), lm(b0.nelem), id(b0.nelem, 0.0);
-    d2h_sync(s, ch.data(), A() + C[N - 1].off, sizeof(double) * b0.nelem);
-    for (const Blk& b : b0.l[0].b) {
-        if (b.rows != b.cols) throw InvalidArg("heff: C blocks must be square");
+    // host-side bond objects: identities; R = C C^T (the right fixed point of AL's TM); in closed form
+    // (AL C = C AR) L = C^T C and the dominant eigenmatrices LM = C^T, RM = C of the mixed TM in K = 0
+    // (eigenvalue 1, tr(LM RM) = 1), else L, LM, RM as given; then the projection weights of the
+    // geometric sums
+    const Layout& cl0 = Cl[N - 1];
+    std::vector<double> ch(cl0.nelem);
+    d2h_sync(s, ch.data(), A() + C[N - 1].off, sizeof(double) * cl0.nelem);
+    auto eye = [](const Layout& b) {
+        std::vector<double> id(b.nelem, 0.0);
+        for (const Blk& x : b.l[0].b)
+            for (int i = 0; i < x.rows; ++i) id[x.off + i + (long)i * x.rows] = 1.0;
+        return id;
+    };
+    std::vector<double> rh(bL.nelem, 0.0), lh(bR.nelem, 0.0), lm(lmL.nelem, 0.0), rm(rmL.nelem, 0.0);
+    for (const Blk& b : cl0.l[0].b) {
+        if (b.qin != b.qout) throw InvalidArg("heff: C must be block diagonal");
+        const double* c = ch.data() + b.off;  // rows x cols, column-major
+        const Blk* tb = bL.l[0].find(b.qin);
+        if (!tb || tb->rows != b.rows) throw InvalidArg("heff: a block of C does not match the right bond of AL");
+        for (int j = 0; j < b.rows; ++j)
+            for (int i = 0; i < b.rows; ++i) {
+                double a = 0;
+                for (int k = 0; k < b.cols; ++k) a += c[i + (long)k * b.rows] * c[j + (long)k * b.rows];
+                rh[tb->off + i + (long)j * b.rows] = a;
+            }
+        if (bond) continue;
+        const Blk* ub = bR.l[0].find(b.qin);
+        const Blk* lb = lmL.l[0].find(b.qin);
+        const Blk* rb = rmL.l[0].find(b.qin);
+        if (!ub || ub->rows != b.cols || !lb || !rb) throw InvalidArg("heff: a block of C does not match the left bond of AR");
         for (int j = 0; j < b.cols; ++j)
-            for (int i = 0; i < b.rows; ++i) lm[b.off + j + (long)i * b.rows] = ch[b.off + i + (long)j * b.rows];
-        for (int i = 0; i < b.rows; ++i) id[b.off + i + (long)i * b.rows] = 1.0;
+            for (int i = 0; i < b.cols; ++i) {
+                double a = 0;
+                for (int k = 0; k < b.rows; ++k) a += c[k + (long)i * b.rows] * c[k + (long)j * b.rows];
+                lh[ub->off + i + (long)j * b.cols] = a;
+            }
+        for (int j = 0; j < b.cols; ++j)
+            for (int i = 0; i < b.rows; ++i) {
+                rm[rb->off + i + (long)j * b.rows] = c[i + (long)j * b.rows];
+                lm[lb->off + j + (long)i * b.cols] = c[i + (long)j * b.rows];
+            }
     }
-    h2d_sync(s, A() + one.off, id.data(), sizeof(double) * b0.nelem);
-    h2d_sync(s, A() + LMo.off, lm.data(), sizeof(double) * b0.nelem);
+    if (bond) {
+        auto fetch = [&](const Layout& l, const double* p, const Layout& dl, const char* what) {
+            std::vector<double> h(l.nelem);
+            if (l.nelem > 0 && !p) throw InvalidArg(std::string("heff: ") + what + " data pointer is NULL");
+            d2h_sync(s, h.data(), p, sizeof(double) * l.nelem);
+            return map_blocks(l, h, dl, what);
+        };
+        lh = fetch(bond->Llay, bond->L, bR, "L");
+        // LM and RM project the geometric sums in K = 0 only (Excitations_main.cpp:264-268); for
+        // another charge they are not used
+        bool zeroK = true;
+        for (int i = 0; i < nsym; ++i) zeroK = zeroK && K.q[i] == 0;
+        if (zeroK) {
+            lm = fetch(bond->LMlay, bond->LM, lmL, "LM");
+            rm = fetch(bond->RMlay, bond->RM, rmL, "RM");
+        }
+    }
+    h2d_sync(s, A() + oneL.off, eye(bL).data(), sizeof(double) * bL.nelem);
+    h2d_sync(s, A() + oneR.off, eye(bR).data(), sizeof(double) * bR.nelem);
+    h2d_sync(s, A() + Rb.off, rh.data(), sizeof(double) * rh.size());
+    h2d_sync(s, A() + Lb.off, lh.data(), sizeof(double) * lh.size());
+    h2d_sync(s, A() + LMo.off, lm.data(), sizeof(double) * lm.size());
+    h2d_sync(s, A() + RMo.off, rm.data(), sizeof(double) * rm.size());
     for (auto& gp : geos) {
         GeoOp& g = *gp;
         for (size_t k = 0; k < g.chain.size(); ++k) {
@@ -846,21 +936,24 @@ void HeffEngine::set_state(const std::vector<const double*>& al, const std::vect
             if (!g.chain_lo.empty()) set_operator(g.chain_lo[k].get(), {A() + g.chain_ops[k].first.off, A() + g.chain_ops[k].second.off});
         }
         if (!g.proj) continue;
-        const std::vector<double>& rin = (g.rin.off == LMo.off) ? lm : ch;
-        const std::vector<double>& ro = (g.rout_src.off == LMo.off) ? lm : ch;
-        std::vector<double> w(g.x.nelem(), 0.0), r(g.x.nelem(), 0.0);
+        // both sums act on objects between AL's bond (rows) and AR's (columns); on x's sectors,
+        // lmt = LM^T and rmx = RM.  The right sum is InvertE_proj_fac(AL, AR, x, LM, RM, r): trace
+        // tr(LM x) (weights LM^T), rho_out = RM; the left sum InvertE_proj_fac(AR, AL, x, LM^T, RM^T, l)
+        // (Excitations_helpers.hpp:83, :124): trace tr(x RM^T) (weights RM), rho_out = LM^T
+        // (InvEBlockFunctions.hpp:268-331)
+        std::vector<double> lmt(g.x.nelem(), 0.0), rmx(g.x.nelem(), 0.0);
         for (const Blk& xb : g.x.lay.l[0].b) {
-            // w: rho_in^T on x's sectors (trace partner); r: rho_out restricted to x's sectors
-            if (const Blk* rb = b0.l[0].find(xb.qout))
+            if (const Blk* rb = lmL.l[0].find(xb.qout))
                 if (rb->qout == xb.qin && rb->rows == xb.cols && rb->cols == xb.rows)
                     for (int j = 0; j < xb.cols; ++j)
-                        for (int i = 0; i < xb.rows; ++i) w[xb.off + i + (long)j * xb.rows] = rin[rb->off + j + (long)i * rb->rows];
-            if (const Blk* ob = b0.l[0].find(xb.qin))
+                        for (int i = 0; i < xb.rows; ++i) lmt[xb.off + i + (long)j * xb.rows] = lm[rb->off + j + (long)i * rb->rows];
+            if (const Blk* ob = rmL.l[0].find(xb.qin))
                 if (ob->qout == xb.qout && ob->rows == xb.rows && ob->cols == xb.cols)
-                    for (long e = 0; e < xb.size(); ++e) r[xb.off + e] = ro[ob->off + e];
+                    for (long e = 0; e < xb.size(); ++e) rmx[xb.off + e] = rm[ob->off + e];
         }
-        h2d_sync(s, A() + g.w.off, w.data(), sizeof(double) * w.size());
-        h2d_sync(s, A() + g.rout.off, r.data(), sizeof(double) * r.size());
+        const bool right = g.dir == IMPS_RIGHT;
+        h2d_sync(s, A() + g.w.off, (right ? lmt : rmx).data(), sizeof(double) * lmt.size());
+        h2d_sync(s, A() + g.rout.off, (right ? rmx : lmt).data(), sizeof(double) * lmt.size());
     }
     envL->set_mps([&] { std::vector<const double*> v; for (auto& o : AL) v.push_back(A() + o.off); return v; }());
     envR->set_mps([&] { std::vector<const double*> v; for (auto& o : AR) v.push_back(A() + o.off); return v; }());

@@ -144,8 +144,12 @@ struct HeffEngine : BlockTape {
     std::vector<XObj> HLtot, HRtot, HLr, HRr;
     XObj Xin, Xout;                   // contiguous complex objects in XDims order (views per site)
     std::vector<XObj> Xin_v, Xout_v;
-    XObj LMo, RMo;                    // closed-form dominant eigenmatrices of the mixed TM (bond 0)
-    XObj one;                         // identity on bond 0
+    // bond between the unit cells: bL on AL's side (right bond of AL[N-1]), bR on AR's side (left
+    // bond of AR[0]); lmL / rmL: the charge-conserving blocks between them, (bR x bL) / (bL x bR)
+    Layout bL, bR, lmL, rmL;
+    XObj LMo, RMo;                    // dominant left / right eigenmatrices of the mixed TM (AL, AR)
+    XObj oneL, oneR;                  // identities on bL and bR
+    XObj Rb, Lb;                      // right fixed point C C^T of AL's TM, left fixed point of AR's TM
     std::vector<XObj> AAL;            // AL[n-1] AL[n] two-site products
     XObj checkHL, checkHR, checkL, checkR;  // operands of HeffConstants' trace check
 
@@ -158,7 +162,6 @@ struct HeffEngine : BlockTape {
         std::vector<std::pair<XObj, XObj>> chain_ops;     // (A, B) of each chain plan
         bool proj = false;
         XObj w, rout;         // real, on x's sectors: trace weights (rho_in^T) and rho_out
-        XObj rin, rout_src;   // the bond-0 objects they are built from (LM or RM)
     };
     std::vector<std::unique_ptr<GeoOp>> geos;
     std::vector<std::function<void()>> calls;
@@ -181,8 +184,15 @@ struct HeffEngine : BlockTape {
     void build(imps_context c, int ncell, int dphys, const std::vector<Layout>& al, const std::vector<Layout>& ar,
                const std::vector<Layout>& nl, const std::vector<Layout>& cl, const double* H, double E0, const int* Kq, int nmomenta,
                int precision = IMPS_FP64, int sites = 2);
+    // Bond objects given by the caller in place of the closed forms L = C^T C, LM = C^T, RM = C (an AR
+    // that is not C^-1 AL C, e.g. transformed or shifted): each with its own layout, mapped onto the
+    // engine's by charge
+    struct BondInput {
+        Layout Llay, LMlay, RMlay;
+        const double *L = nullptr, *LM = nullptr, *RM = nullptr;
+    };
     void set_state(const std::vector<const double*>& al, const std::vector<const double*>& ar, const std::vector<const double*>& nl,
-                   const std::vector<const double*>& cl, double tol);
+                   const std::vector<const double*>& cl, double tol, const BondInput* bond = nullptr);
     // y = H_eff(p_t) x for every momentum t (interleaved complex [nmom][mtot] device vectors)
     void apply(const double* p, const double* x, double* y);
     // lowest nev eigenpairs per momentum (Hermitian thick-restart Lanczos, batched; evecs
@@ -193,8 +203,7 @@ struct HeffEngine : BlockTape {
 
     XObj redop_left_mixed(const XObj& Abra, const XObj& Bket);
     XObj redop_right_mixed(const XObj& Aket, const XObj& Bbra);
-    XObj invert_e(const std::vector<XObj>& As, const std::vector<XObj>& Bs, const XObj& x, int dir, int pow, const XObj* L,
-                  const XObj* R);
+    XObj invert_e(const std::vector<XObj>& As, const std::vector<XObj>& Bs, const XObj& x, int dir, int pow, bool proj);
     void record_setup();
     void record_apply();
     void record_apply_observable();

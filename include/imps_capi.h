@@ -282,6 +282,19 @@ imps_status imps_opeff_plan(imps_context ctx, imps_plan* plan, int ncell, int d,
                             const imps_layout* NL, const imps_layout* C, const double* O, double O0, const int* K, int nmom, int prec);
 imps_status imps_heff_set_state(imps_plan plan, const double* const* AL, const double* const* AR, const double* const* NL,
                                 const double* const* C, double inv_tol);
+/* imps_heff_set_state for an AR that is not C^-1 AL C: the state of ApplyTransformation
+ * (Excitations_helpers.hpp:680-771, spin or charge flips of AR and L) or of a relative unit-cell
+ * shift (Excitations_main.cpp:153-163).  C still gives R = C C^T on AL's side
+ * (Excitations_main.cpp:142-146); L (BlockDiagMat, the left fixed point of AR's transfer matrix on
+ * AR's bond 0, Lvec.back() after the transformation, Excitations_main.cpp:173), and the dominant
+ * left / right eigenmatrices LM (rows on AR's bond, columns on AL's) and RM (rows on AL's bond,
+ * columns on AR's) of the mixed transfer matrix in K = 0 are given with their layouts (nlist = 1,
+ * device FP64, Vectorize order; LRoverlaps, :915-946, normalised to tr(LM RM) = 1).  Blocks are
+ * matched by charge; a block outside the bond structure of AL and AR is an error.  LM and RM only
+ * project the sums in K = 0 (Excitations_main.cpp:264-268): for another K they are not read. */
+imps_status imps_heff_set_state_lr(imps_plan plan, const double* const* AL, const double* const* AR, const double* const* NL,
+                                   const double* const* C, const imps_layout* Llay, const double* L, const imps_layout* LMlay,
+                                   const double* LM, const imps_layout* RMlay, const double* RM, double inv_tol);
 imps_status imps_heff_check(imps_plan plan, double* trHL_R, double* trL_HR, int* env_iters);
 /* Replaces the raw callback ApplyHeff(in, out, xdims, mtot, kfac, AL, AR, NL, LM, RM, H, HLtot, HRtot,
  * InvETol, ...) (Excitations_helpers.hpp:230-257) for nmom momenta at once: y_t = H_eff(p_t) x_t.

@@ -18,6 +18,8 @@ reference code it restates:
   LRoverlaps              Excitations_helpers.hpp:915-946 (TMmixedEigs, EigsBlockFunctions.hpp:282-391)
   TM spectra (dense)      TMEigs / TMmixedEigs, EigsBlockFunctions.hpp:177-391
   ApplyHeff               Excitations_helpers.hpp:20-228
+  ApplyTransformation     Excitations_helpers.hpp:680-771 (FlipQN MPSBlockMat.hpp:762-787,
+                          BlockObj.hpp:845-855, 1740-1749; FlipK KeyTypes.hpp:156-170)
 
 Deviations: GMRES starts from y0 = 0 when no guess is given (the reference draws a random y0;
 only the iterate sequence differs); the dominant mixed-TM eigenvectors LM, RM are obtained by a
@@ -496,6 +498,55 @@ def lr_overlaps(AL, AR, K: Key):
     s = tr_prod(LM, RM)  # the pairing of the projectors: tr(LM RM) (InvEfun_proj_fac's trace(x*rho))
     RM = bm_scale(RM, 1.0 / s)
     return LM, RM, 0.5 * (np.real(wl) + np.real(wr))
+
+
+# ---- ApplyTransformation (Excitations_helpers.hpp:680-771) -------------------------------------
+def transformation_masks(model: str, op: str, N: int):
+    """(which charge components flip, per-site sign flags over the physical index) of
+    ApplyTransformation: XXZ "FS" flips the one U(1) charge, no signs (:693-701); FHUB in the (N, M)
+    representation (FHUB_NM_REP, the reference's build) "FS" flips M with a sign on |3> (:704-716),
+    "FC" flips N with the signs {2} on even and {0, 2, 3} on odd sites, N even (:717-737)."""
+    if model == "XXZ" and op == "FS":
+        return (True,), [(False, False)] * N
+    if model == "FHUB" and op == "FS":
+        return (False, True), [(False, False, False, True)] * N
+    if model == "FHUB" and op == "FC":
+        if N % 2:
+            raise ValueError("unit cell size needs to be even for charge flip")
+        return (True, False), [((False, False, True, False) if n % 2 == 0 else (True, False, True, True)) for n in range(N)]
+    raise ValueError(f"no transformation {op} for {model}")
+
+
+def flip_key(q: Key, which) -> Key:
+    """FlipK (KeyTypes.hpp:156-170): negate the masked components."""
+    return tuple(-x if w else x for x, w in zip(q, which))
+
+
+def flip_bm(bm: BM, which) -> BM:
+    """BlockMat::FlipQN / BlockDiagMat::FlipQN: same matrices under flipped charges."""
+    return {flip_key(q, which): (flip_key(qo, which), m) for q, (qo, m) in sorted(bm.items())}
+
+
+def apply_transformation(AR, L: BM, I2K, which, signs):
+    """AR -> FlipQN(I2K, which) then the sign flags per site; L -> FlipQN(which).  Entry i of an
+    MPS tensor (charge dK = Qout - Qin, I2K[i] when empty) moves to the position of its flipped
+    charge in I2K."""
+    order = {tuple(k): i for i, k in enumerate(I2K)}
+    out = []
+    for n, A in enumerate(AR):
+        B = [dict() for _ in A]
+        for i, bm in enumerate(A):
+            k = dK(bm)
+            k = tuple(I2K[i]) if k is None else k
+            f = flip_key(k, which)
+            if f not in order:
+                raise ValueError(f"key {f} not found in target I2K")
+            B[order[f]] = flip_bm(bm, which)
+        for s, neg in enumerate(signs[n]):
+            if neg:
+                B[s] = bm_scale(B[s], -1.0)
+        out.append(B)
+    return out, flip_bm(L, which)
 
 
 # ---- ApplyHeff (Excitations_helpers.hpp:20-228) -----------------------------------------------

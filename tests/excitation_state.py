@@ -89,3 +89,32 @@ def oracle_heff_matrix(K, p, s=None, proj_at_unit_phase=False):
         e[j] = 1.0
         M[:, j] = oracle_heff(K, p, e, s, proj_at_unit_phase)
     return M
+
+
+MODELS = {"xxz": (0, "XXZ", [0]), "hub": (1, "FHUB", [2, 0])}
+
+
+@functools.lru_cache(maxsize=None)
+def transformed_state(which: str, op: str, K: tuple, shift: int = 0):
+    """The oracle's right half of the excitation program (Excitations_main.cpp:142-177) for the state
+    of `which` ("xxz": state(), "hub": hubbard_state()): ApplyTransformation(AR, L, op) when op is
+    given, the relative shift of AR and L by `shift` sites (site n takes site n - shift), LRoverlaps
+    in K and AR[0] times the sign of the overlap; HLtot / HRtot of HeffConstants with the new L."""
+    s = state() if which == "xxz" else hubbard_state()
+    model, mname, QN = MODELS[which]
+    N = s["N"]
+    b = (N - 1 - shift) % N
+    L = ex.bm_mul(ex.bm_t(s["C"][b]), s["C"][b])
+    AR = list(s["AR"])
+    if op:
+        I2K = [tuple(int(v) for v in k) for k in np.asarray(orc.i2k(model, N, QN)).reshape(s["d"], -1)]
+        wh, signs = ex.transformation_masks(mname, op, N)
+        AR, L = ex.apply_transformation(AR, L, I2K, wh, signs)
+    AR = [AR[(n - shift) % N] for n in range(N)]
+    _, _, ol = ex.lr_overlaps(s["AL"], AR, K)
+    if ol < 0:
+        AR[0] = [ex.bm_scale(bm, -1.0) for bm in AR[0]]
+    HLtot, HRtot = ex.heff_constants(s["AL"], AR, L, s["R"], s["H0"])
+    out = dict(s)
+    out.update(AR=AR, L=L, HLtot=HLtot, HRtot=HRtot, OL=ol, model=model, QN=QN)
+    return out
