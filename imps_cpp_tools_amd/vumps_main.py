@@ -80,14 +80,12 @@ def _bool(s: str) -> bool:
     return s.strip().lower() in ("1", "true", "yes", "on")
 
 
-def settings(values: Dict[str, str]) -> Dict:
-    """The run settings with the reference's defaults (VUMPS_main.cpp:30-46)."""
+def model_settings(values: Dict[str, str]):
+    """(model name, local dimension or None, Hamiltonian parameters) of CreateModel
+    (Models.cpp:881-1114) from the parsed keys."""
     model = _get(values, "model", str, required=True)
     if model not in MODELS:
         raise ValueError(f"{model} not found (models: {', '.join(MODELS)})")
-    N = _get(values, "N", int, required=True)
-    if N < 2:
-        raise ValueError("unit cell size must be at least 2")
     dim = None
     if model == "XXZ":
         params = [_get(values, "J", float, 1.0), _get(values, "Delta", float, required=True), _get(values, "hz", float, 0.0)]
@@ -104,6 +102,15 @@ def settings(values: Dict[str, str]) -> Dict:
     else:
         params = [_get(values, "t", float, 1.0), _get(values, "U", float, required=True), _get(values, "V", float, 0.0),
                   _get(values, "mu", float, 0.0), _get(values, "hz", float, 0.0)]
+    return model, dim, params
+
+
+def settings(values: Dict[str, str]) -> Dict:
+    """The run settings with the reference's defaults (VUMPS_main.cpp:30-46)."""
+    model, dim, params = model_settings(values)
+    N = _get(values, "N", int, required=True)
+    if N < 2:
+        raise ValueError("unit cell size must be at least 2")
     mmax = _get(values, "mmax", lambda s: _vector(s, int), required=True)
     if len(mmax) == 1:
         mmax = mmax * N
