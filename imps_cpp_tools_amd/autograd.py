@@ -16,7 +16,8 @@ the backward (nothing but the inputs is kept between the passes).
 """
 from __future__ import annotations
 
-from typing import Optional
+import math
+from typing import Optional, Sequence
 
 import torch
 
@@ -85,6 +86,70 @@ class _TMRegFn(torch.autograd.Function):
         x, A = ctx.saved_tensors
         xbar, Abar, _ = ctx.tm.vjp(x, A, None, ybar)
         return None, xbar, Abar
+
+
+class TransferChain:
+    """y = T_{L-1}(... T_1(T_0(x))) for a sequence of transfer-matrix applies (TransferMatrix objects,
+    one per step, each with its own MPS tensors): a unit cell's chain (TMBlockFunctions.hpp:542-788)
+    or several cells of it, as an environment is built.  The forward keeps only the input and every
+    `every`-th environment (default ceil(sqrt(L))); the backward recomputes each segment's
+    environments from its checkpoint and runs the steps' adjoints in reverse, so the memory is
+    O(L / every + every) environments instead of L, for one extra forward."""
+
+    def __init__(self, tms: Sequence[TransferMatrix], every: Optional[int] = None):
+        self.tms = list(tms)
+        if not self.tms:
+            raise ValueError("TransferChain: no steps")
+        for a, b in zip(self.tms, self.tms[1:]):
+            if not a.yl == b.xl:
+                raise ValueError("TransferChain: a step's output structure is not the next step's input")
+        if any(t.mixed != self.tms[0].mixed for t in self.tms):
+            raise ValueError("TransferChain: all steps regular or all mixed")
+        self.every = int(every) if every else max(1, math.ceil(math.sqrt(len(self.tms))))
+        self.last_checkpoints = 0  # environments the last forward kept (input included)
+
+    def __call__(self, x, As: Sequence, Bs: Optional[Sequence] = None):
+        L = len(self.tms)
+        if len(As) != L or (Bs is not None and len(Bs) != L) or (Bs is not None) != self.tms[0].mixed:
+            raise ValueError("TransferChain: one tensor (A, and B when mixed) per step")
+        return _ChainFn.apply(self, x, *As, *(Bs if Bs is not None else ()))
+
+
+class _ChainFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, chain: TransferChain, x, *ts):
+        L = len(chain.tms)
+        As, Bs = ts[:L], (ts[L:] if chain.tms[0].mixed else (None,) * L)
+        keep, idx = [x], [0]
+        y = x
+        for i, tm in enumerate(chain.tms):
+            y = tm.forward_raw(y, As[i], Bs[i])
+            if (i + 1) % chain.every == 0 and i + 1 < L:
+                keep.append(y)
+                idx.append(i + 1)
+        ctx.chain, ctx.idx, ctx.nkeep = chain, idx, len(keep)
+        chain.last_checkpoints = len(keep)
+        ctx.save_for_backward(*keep, *ts)
+        return y
+
+    @staticmethod
+    def backward(ctx, ybar):
+        chain = ctx.chain
+        L = len(chain.tms)
+        saved = ctx.saved_tensors
+        keep, ts = saved[:ctx.nkeep], saved[ctx.nkeep:]
+        As, Bs = ts[:L], (ts[L:] if chain.tms[0].mixed else (None,) * L)
+        gA, gB = [None] * L, [None] * L
+        g = ybar.contiguous()
+        for c in range(len(ctx.idx) - 1, -1, -1):
+            s = ctx.idx[c]
+            e = ctx.idx[c + 1] if c + 1 < len(ctx.idx) else L
+            xs = [keep[c]]
+            for i in range(s, e - 1):  # the segment's environments, recomputed
+                xs.append(chain.tms[i].forward_raw(xs[-1], As[i], Bs[i]))
+            for i in range(e - 1, s - 1, -1):
+                g, gA[i], gB[i] = chain.tms[i].vjp(xs[i - s], As[i], Bs[i], g)
+        return (None, g, *gA, *(gB if chain.tms[0].mixed else ()))
 
 
 class TwoSiteHAC:

@@ -240,3 +240,67 @@ def test_vjp_entries_match_autograd_of_dense_contractions():
     ref_a = extract(A, gA.numpy(), l0, l1, per_entry=True)
     assert np.abs(xbar.cpu().numpy() - ref_x).max() < 1e-13
     assert np.abs(abar.cpu().numpy() - ref_a).max() < 1e-13
+
+
+def _chain(c, direction, mixed, cells, every=None, prec=im.FP64):
+    """TransferChain over `cells` unit cells of c (left: sites 0, 1, ... from bond 0; right: sites
+    N-1, ..., 0 from bond 0), with the per-step MPS tensors."""
+    from imps_cpp_tools_amd.autograd import TransferChain, TransferMatrix
+    N = c.N
+    order = list(range(N)) * cells if direction == 0 else list(range(N - 1, -1, -1)) * cells
+    tms, As, Bs = [], [], []
+    for n in order:
+        A, B = c.AL[n], c.AR[n]
+        if direction == 0:
+            xl, yl = c.diag(n), c.diag((n + 1) % N)
+        else:
+            xl, yl = c.diag((n + 1) % N), c.diag(n)
+        tms.append(TransferMatrix(ctx(), direction, _lay(A), _lay(B) if mixed else None, _lay(xl), _lay(yl), prec))
+        As.append(A)
+        Bs.append(B)
+    return TransferChain(tms, every), As, Bs
+
+
+@pytest.mark.parametrize("direction", [0, 1], ids=["left", "right"])
+@pytest.mark.parametrize("mixed", [False, True], ids=["regular", "mixed"])
+def test_chain_gradcheck(direction, mixed):
+    """The checkpointed chain (recomputed environments) under central finite differences: two cells,
+    a checkpoint every other step; the same tensor at two steps gets both contributions."""
+    c = small_cell()
+    ch, As, Bs = _chain(c, direction, mixed, 2, every=2)
+    rng = np.random.default_rng(7)
+    x = _t(rng.standard_normal(ch.tms[0].xl.nelem), True)
+    a = [_t(A.data, True) for A in c.AL]
+    b = [_t(B.data, True) for B in c.AR]
+    order = [0, 1, 0, 1] if direction == 0 else [1, 0, 1, 0]
+    if mixed:
+        f = lambda x, a0, a1, b0, b1: ch(x, [[a0, a1][n] for n in order], [[b0, b1][n] for n in order])  # noqa: E731
+        assert torch.autograd.gradcheck(f, (x, a[0], a[1], b[0], b[1]), eps=1e-6, atol=1e-8, rtol=1e-6)
+    else:
+        f = lambda x, a0, a1: ch(x, [[a0, a1][n] for n in order])  # noqa: E731
+        assert torch.autograd.gradcheck(f, (x, a[0], a[1]), eps=1e-6, atol=1e-8, rtol=1e-6)
+    assert ch.last_checkpoints == 2  # the input and the environment after two steps
+
+
+def test_config4_chain_recompute_equals_stored():
+    """D = 256: three cells of left TMs.  The checkpointed chain keeps 2 of the 6 environments and
+    gives the gradients of the step-by-step graph that stores all of them (bitwise for x)."""
+    from imps_cpp_tools_amd.autograd import TransferMatrix
+    c = _config4_cell()
+    ch, As, _ = _chain(c, 0, False, 3)
+    rng = np.random.default_rng(8)
+    x0 = rng.standard_normal(ch.tms[0].xl.nelem)
+    w = _t(rng.standard_normal(ch.tms[-1].yl.nelem))
+    x1, a1 = _t(x0, True), [_t(A.data, True) for A in c.AL]
+    y = ch(x1, [a1[n] for n in [0, 1] * 3])
+    g1 = torch.autograd.grad(torch.dot(w, y), [x1] + a1)
+    assert ch.last_checkpoints == 2 and ch.every == 3
+    x2, a2 = _t(x0, True), [_t(A.data, True) for A in c.AL]
+    z = x2
+    for i, n in enumerate([0, 1] * 3):
+        z = ch.tms[i](z, a2[n])
+    assert torch.equal(z, y)
+    g2 = torch.autograd.grad(torch.dot(w, z), [x2] + a2)
+    assert torch.equal(g1[0], g2[0])
+    for p, q in zip(g1[1:], g2[1:]):  # three contributions per tensor, summed in autograd's order
+        assert (torch.linalg.vector_norm(p - q) / torch.linalg.vector_norm(q)).item() < 1e-14
