@@ -17,7 +17,9 @@ globally, L = C^T C and R = C C^T on the last bond, ApplyTransformation and the 
 and L, LRoverlaps in K with AR[0] times the overlap's sign, E0 from AL with R and from AR with L
 (they must agree to tol), H - E0, the null spaces of AL, XDims, HeffConstants, then all momenta at
 once: one batched eigensolve of H_eff(p) (the reference loops over the momenta with one eigs_n
-each).  Not built: saveX (the excitation vectors' file) and the ALRN file of saveE.
+each).  saveE / saveX write, under datafolder/<run tag>/, the momenta and the state actually
+excited (saveALRN: C, L, R, AL, AR, NL with the shifts and transformation), then the energies
+(np x nbands) or, per momentum, the nbands excitation vectors (Excitations_main.cpp:221-239, 285-309).
 """
 from __future__ import annotations
 
@@ -50,8 +52,6 @@ def settings(values: Dict[str, str]) -> Dict:
         raise ValueError("np must be at least 1")
     if cfg["prec"] not in ("fp64", "fp32"):
         raise ValueError("prec: fp64 or fp32")
-    if cfg["saveX"]:
-        raise ValueError("saveX is not built here")
     return cfg
 
 
@@ -212,7 +212,8 @@ def run(cfg: Dict, ctx=None, log=print):
     if cfg["verbose"]:
         log(f"HeffConstants check: tr(HL R) = {trl:.3e}, tr(L HR) = {trr:.3e}; mtot = {P.mtot}")
     t1 = time.perf_counter()
-    ev, _, res, napp = P.eigs(pvec, nev=cfg["nbands"], tol=cfg["tol"], maxit=cfg["maxit"], krylov=cfg["krylov"], check=False)
+    ev, vecs, res, napp = P.eigs(pvec, nev=cfg["nbands"], tol=cfg["tol"], maxit=cfg["maxit"], krylov=cfg["krylov"],
+                                 vectors=cfg["saveX"], check=False)
     torch.cuda.synchronize(dev)
     t_eig = time.perf_counter() - t1
     for t, p in enumerate(pvec):
@@ -223,14 +224,27 @@ def run(cfg: Dict, ctx=None, log=print):
     tel = time.perf_counter() - t0
     log(f"elapsed time: {tel / 60:.3f} min." if tel > 60 else f"elapsed time: {tel:.3f} s.")
     out.update(dE=ev, residuals=res, applies=napp, converged=conv, seconds_eigs=t_eig, seconds=tel, mtot=P.mtot)
-    if cfg["saveE"]:
+    if cfg["saveX"]:
+        out["X"] = vecs  # in the basis of the null spaces NL (saved in the ALRN file)
+        out["NL"] = list(zip(NLl, NLd))
+    if cfg["saveE"] or cfg["saveX"]:
+        from .imps_io import write_alrn, write_excitations
         tag = f"{cfg['model']}_QN{''.join(str(q) for q in cfg['QN'])}_N{N}_p{cfg['pmin']:g}_{cfg['pmax']:g}_{cfg['np']}_nb{cfg['nbands']}"
         tag += f"_shift{cfg['shift']}" + (f"_op{cfg['op']}" if cfg["op"] else "") + f"_K{''.join(str(k) for k in K)}"
         folder = os.path.join(cfg["datafolder"], tag)
         os.makedirs(folder, exist_ok=True)
         _save_arma(os.path.join(folder, tag + "_pvec.bin"), pvec.reshape(-1, 1))
-        _save_arma(os.path.join(folder, tag + "_dE.bin"), ev)
-        log(f"saved pvec and dE under {folder}")
+        host = lambda lt: (lt[0], lt[1].cpu().numpy())  # noqa: E731
+        write_alrn(os.path.join(folder, tag + "_ALRN.bin"), [host(x) for x in zip(Cl, Cd)], [host(x) for x in st["Lvec"]],
+                   [host((r.lay, r.data)) for r in Rs], [host(x) for x in zip(ALl, ALd)], [host(x) for x in zip(st["ARl"], st["AR"])],
+                   [host(x) for x in zip(NLl, NLd)])
+        if cfg["saveE"]:
+            _save_arma(os.path.join(folder, tag + "_dE.bin"), ev)
+        if cfg["saveX"]:
+            xv = vecs.cpu().numpy()
+            for t in range(cfg["np"]):
+                write_excitations(os.path.join(folder, tag + f"_X{t}.bin"), xv[t], P.xdims)
+        log(f"saved pvec, ALRN{', dE' if cfg['saveE'] else ''}{', X' if cfg['saveX'] else ''} under {folder}")
     return out
 
 

@@ -21,6 +21,13 @@ in the reference (Armadillo's version is not pinned there): the line "ARMA_MAT_B
 symmetry group is not stored (the reader's model supplies it, KeyTypes.hpp:101-103).  Real states
 only (the "R" tags; VUMPS saves real states).
 
+The excitation program's output files use the same pieces: saveALRN (Excitations_helpers.hpp:948-989)
+is C, L, R as BlockDiagMatArrays and AL, AR, NL as MPSBlockMatArrays, one after the other; its
+excitation vectors (Excitations_main.cpp:285-292, the generic save of BlockObj.hpp:3196-3222) are
+"CBLOCKMATARRAYVEC\n" u32 nev, nev x ("CBLOCKMATARRAY\n" u32 N, N x ("CBLOCKMAT\n" u32 nblk,
+nblk x (key in, key out, complex matrix))) with complex matrices "ARMA_MAT_BIN_FC016", "<rows> <cols>",
+then the column-major (real, imaginary) pairs.
+
 Cell convention (as randMPS_LR builds it and VUMPS_main.cpp:367-372 restores it before saving):
 AL[n] and AR[n] map bond n to bond n + 1, C[n] lives on bond n + 1 (AL[n] C[n] = C[n-1] AR[n]) and
 lam[n] are C[n]'s singular values per sector.
@@ -36,6 +43,7 @@ from . import BlockLayout
 
 Key = Tuple[int, ...]
 ARMA_REAL = b"ARMA_MAT_BIN_FN008"
+ARMA_CPLX = b"ARMA_MAT_BIN_FC016"
 
 
 # ---------------------------------------------------------------------------------------------
@@ -59,6 +67,12 @@ def _w_mat(f: BinaryIO, m: np.ndarray):
         m = m.reshape(-1, 1)
     f.write(ARMA_REAL + b"\n" + ("%d %d\n" % m.shape).encode("ascii"))
     f.write(np.asfortranarray(m).astype("<f8").tobytes(order="F"))
+
+
+def _w_cmat(f: BinaryIO, m: np.ndarray):
+    m = np.asarray(m, np.complex128)
+    f.write(ARMA_CPLX + b"\n" + ("%d %d\n" % m.shape).encode("ascii"))
+    f.write(np.asfortranarray(m).astype("<c16").tobytes(order="F"))
 
 
 def _r_word(f: BinaryIO) -> bytes:
@@ -155,6 +169,34 @@ def mps_from_items(nsym: int, items: List[List[Tuple[Key, Key, np.ndarray]]], pe
 
 
 # ---------------------------------------------------------------------------------------------
+def _w_diag_array(f: BinaryIO, arr: List[Tuple[BlockLayout, np.ndarray]]):
+    _w_tag(f, "RBLOCKDIAGMATARRAY")
+    _w_u32(f, len(arr))
+    for lay, data in arr:
+        items = _diag_items(lay, data)
+        _w_tag(f, "RBLOCKDIAGMAT")
+        _w_u32(f, len(items))
+        for k, m in items:
+            _w_key(f, k)
+            _w_mat(f, m)
+
+
+def _w_mps_array(f: BinaryIO, arr: List[Tuple[BlockLayout, np.ndarray]]):
+    _w_tag(f, "RBLOCKMPSARRAY")
+    _w_u32(f, len(arr))
+    for lay, data in arr:
+        _w_tag(f, "RBLOCKMPS")
+        _w_u32(f, lay.nlist)
+        _w_u32(f, 1)  # one-site tensors
+        for ent in _mps_items(lay, data):
+            _w_tag(f, "RBLOCKMAT")
+            _w_u32(f, len(ent))
+            for ki, ko, m in ent:
+                _w_key(f, ki)
+                _w_key(f, ko)
+                _w_mat(f, m)
+
+
 def write_imps(path: str, lam: List[Dict[Key, np.ndarray]], C: List[Tuple[BlockLayout, np.ndarray]],
                AL: List[Tuple[BlockLayout, np.ndarray]], AR: List[Tuple[BlockLayout, np.ndarray]]):
     """saveIMPS(Lamvec, Cvec, ALvec, ARvec, file): lam[n] = {charge: Schmidt values}, C[n] a
@@ -168,29 +210,41 @@ def write_imps(path: str, lam: List[Dict[Key, np.ndarray]], C: List[Tuple[BlockL
             for k in sorted(site):
                 _w_key(f, k)
                 _w_mat(f, np.asarray(site[k], np.float64).reshape(-1, 1))
-        _w_tag(f, "RBLOCKDIAGMATARRAY")
-        _w_u32(f, len(C))
-        for lay, data in C:
-            items = _diag_items(lay, data)
-            _w_tag(f, "RBLOCKDIAGMAT")
-            _w_u32(f, len(items))
-            for k, m in items:
-                _w_key(f, k)
-                _w_mat(f, m)
-        for arr in (AL, AR):
-            _w_tag(f, "RBLOCKMPSARRAY")
-            _w_u32(f, len(arr))
-            for lay, data in arr:
-                _w_tag(f, "RBLOCKMPS")
-                _w_u32(f, lay.nlist)
-                _w_u32(f, 1)  # one-site tensors
-                for ent in _mps_items(lay, data):
-                    _w_tag(f, "RBLOCKMAT")
-                    _w_u32(f, len(ent))
-                    for ki, ko, m in ent:
-                        _w_key(f, ki)
-                        _w_key(f, ko)
-                        _w_mat(f, m)
+        _w_diag_array(f, C)
+        _w_mps_array(f, AL)
+        _w_mps_array(f, AR)
+
+
+def write_alrn(path: str, C, L, R, AL, AR, NL):
+    """saveALRN (Excitations_helpers.hpp:948-989): C, L, R (per bond, (layout, data)) and AL, AR, NL
+    (per site), as the excitation program saves the state it excites (with its shifts and
+    transformations)."""
+    with open(path, "wb") as f:
+        for arr in (C, L, R):
+            _w_diag_array(f, arr)
+        for arr in (AL, AR, NL):
+            _w_mps_array(f, arr)
+
+
+def write_excitations(path: str, X: np.ndarray, xdims: BlockLayout):
+    """The excitation vectors of one momentum (Excitations_main.cpp:285-292): X (nev, mtot) complex in
+    XDims order; xdims the layout with one entry per site (X[n]'s blocks)."""
+    X = np.atleast_2d(np.asarray(X, np.complex128))
+    if X.shape[1] != xdims.nelem:
+        raise ValueError("excitation vectors do not match XDims")
+    with open(path, "wb") as f:
+        _w_tag(f, "CBLOCKMATARRAYVEC")
+        _w_u32(f, X.shape[0])
+        for x in X:
+            _w_tag(f, "CBLOCKMATARRAY")
+            _w_u32(f, xdims.nlist)
+            for ent in _mps_items(xdims, x):
+                _w_tag(f, "CBLOCKMAT")
+                _w_u32(f, len(ent))
+                for ki, ko, m in ent:
+                    _w_key(f, ki)
+                    _w_key(f, ko)
+                    _w_cmat(f, m)
 
 
 def read_imps(path: str, nsym: int, periods=None):

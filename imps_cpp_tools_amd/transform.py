@@ -212,22 +212,32 @@ def transformed_state(ctx: Context, model: int, N: int, QN: Sequence[int], AL: S
     rel_shift sites (MPSBlockUtilities.hpp:1153-1168: site n takes site n - rel_shift), L = the
     last bond's, LRoverlaps in charge K, and AR[0] times the sign of the overlap.  Returns a dict
     with AR, ARl (the new right tensors and layouts), L, LM, RM ((layout, device tensor) pairs for
-    HeffPlan.set_state) and OL."""
-    import torch
+    HeffPlan.set_state), Lvec (every bond's L, (layout, tensor), as saveALRN stores it) and OL."""
     from . import MATT_MPS, ProdPlan
-    s = rel_shift % N
-    b = (N - 1 - s) % N  # the bond whose L sits at the cell boundary after the shift
-    P = ProdPlan(ctx, MATT_MPS, Cl[b], Cl[b])
-    Ll = P.out_layout
-    L = torch.zeros(Ll.nelem, dtype=torch.float64, device=C[b].device)
-    P.apply(C[b], C[b], L)
+    import torch
+    Lvec = []
+    for b in range(N):
+        P = ProdPlan(ctx, MATT_MPS, Cl[b], Cl[b])
+        Lb = torch.zeros(P.out_layout.nelem, dtype=torch.float64, device=C[b].device)
+        P.apply(C[b], C[b], Lb)
+        Lvec.append((P.out_layout, Lb))
     ARn, ARln = list(AR), list(ARl)
     if op:
-        ARn, ARln, L, Ll = apply_transformation(model, N, QN, op, ARn, ARln, L, Ll)
+        tL = []
+        for n, (lay, t) in enumerate(Lvec):
+            if n == 0:
+                ARn, ARln, t2, lay2 = apply_transformation(model, N, QN, op, ARn, ARln, t, lay)
+            else:
+                lay2, idx, sg = bond_transform_plan(model, N, op, lay)
+                t2 = _apply_gather(t, idx, sg)
+            tL.append((lay2, t2))
+        Lvec = tL
+    s = rel_shift % N
     if s:
         ARn = [ARn[(n - s) % N] for n in range(N)]
         ARln = [ARln[(n - s) % N] for n in range(N)]
+        Lvec = [Lvec[(n - s) % N] for n in range(N)]
     LM, RM, OL = lr_overlaps(ctx, AL, ALl, ARn, ARln, K, nev=nev, tol=tol)
     if OL < 0:
         ARn[0] = -ARn[0]
-    return dict(AR=ARn, ARl=ARln, L=(Ll, L), LM=LM, RM=RM, OL=OL)
+    return dict(AR=ARn, ARl=ARln, L=Lvec[N - 1], LM=LM, RM=RM, OL=OL, Lvec=Lvec)

@@ -89,11 +89,79 @@ def test_measure_excitations_match_oracle(tmp_path):
     assert abs(got.sum() + 1.0) < 1e-10  # a normalised K = 2 (2 S^z) excitation changes S^z by -1 in this charge convention
 
 
-def test_saved_energies_file(tmp_path):
+def _read_xvec(path):
+    """The excitation-vector file (CBLOCKMATARRAYVEC of CBLOCKMATARRAY of CBLOCKMAT) as a list of
+    per-vector lists of per-site {qin: (qout, complex matrix)}."""
+    import struct
+    with open(path, "rb") as f:
+        buf = f.read()
+    pos = 0
+
+    def line():
+        nonlocal pos
+        e = buf.index(b"\n", pos)
+        out, pos = buf[pos:e], e + 1
+        return out
+
+    def u32():
+        nonlocal pos
+        v = struct.unpack_from("<I", buf, pos)[0]
+        pos += 4
+        return v
+
+    def key():
+        nonlocal pos
+        n = u32()
+        k = struct.unpack_from("<%di" % n, buf, pos)
+        pos += 4 * n
+        return tuple(k)
+    assert line() == b"CBLOCKMATARRAYVEC"
+    vecs = []
+    for _ in range(u32()):
+        assert line() == b"CBLOCKMATARRAY"
+        sites = []
+        for _ in range(u32()):
+            assert line() == b"CBLOCKMAT"
+            bm = {}
+            for _ in range(u32()):
+                qi, qo = key(), key()
+                assert line() == b"ARMA_MAT_BIN_FC016"
+                r, c = (int(v) for v in line().split())
+                m = np.frombuffer(buf, "<c16", r * c, pos).reshape(r, c, order="F")
+                pos += 16 * r * c
+                bm[qi] = (qo, m)
+            sites.append(bm)
+        vecs.append(sites)
+    assert pos == len(buf)
+    return vecs
+
+
+def test_saved_files(tmp_path):
+    """saveE / saveX: the energies as an arma matrix, per momentum the excitation vectors (each an
+    eigenvector of the oracle's H_eff with its energy), and the ALRN state file."""
     from imps_cpp_tools_amd.imps_io import _r_mat
-    r, _ = run(tmp_path, "xxz", (2,), extra=["--saveE=1", f"--datafolder={tmp_path}"])
+    r, s = run(tmp_path, "xxz", (2,), extra=["--saveE=1", "--saveX=1", f"--datafolder={tmp_path}"])
     files = sorted(p for p in tmp_path.rglob("*_dE.bin"))
     assert len(files) == 1
     with open(files[0], "rb") as f:
         dE = _r_mat(f)
     assert np.array_equal(dE, r["dE"])
+    assert len(list(tmp_path.rglob("*_ALRN.bin"))) == 1
+    ref = es.transformed_state("xxz", "", (2,), 0)
+    xd = ex.xdims(ref["NL"], ref["AR"], (2,))
+    # X is stored in the basis of the run's own null spaces (the ALRN file's NL), a rotation per
+    # sector of the oracle's: X_oracle[n] = (sum_s NL_oracle[n][s]^T NL_run[n][s]) X_run[n]
+    from oracle import pyoracle as orc
+    rot = [ex.tml_unity(ex.from_ob(orc.OB.like(lay, t.cpu().numpy())), ref["NL"][n]) for n, (lay, t) in enumerate(r["NL"])]
+    for t, p in enumerate(r["p"]):
+        (xf,) = tmp_path.rglob(f"*_X{t}.bin")
+        vecs = _read_xvec(xf)
+        assert len(vecs) == 2
+        M = es.oracle_heff_matrix((2,), p, ref, proj_at_unit_phase=True)
+        for j, sites in enumerate(vecs):
+            assert np.array_equal(ex.vectorize_list(sites), r["X"][t, j].cpu().numpy())
+            z = ex.vectorize_list([ex.bm_mul(rot[n], x) for n, x in enumerate(sites)])
+            assert z.size == sum(rr * c for dv in xd for *_, rr, c in dv)
+            assert abs(np.linalg.norm(z) - 1.0) < 1e-12
+            res = np.linalg.norm(M @ z - r["dE"][t, j] * z)
+            assert res < 1e-8

@@ -120,3 +120,47 @@ def test_reader_rejects_bad_files():
                 f.write(bad)
             with pytest.raises(ValueError):
                 imps_io.read_imps(path, 1)
+
+
+def _cmat(m):
+    m = np.asarray(m, np.complex128)
+    # arma_binary complex: header line, dims line, column-major (re, im) pairs
+    pairs = np.stack([m.real.T.reshape(-1), m.imag.T.reshape(-1)], axis=1)
+    return b"ARMA_MAT_BIN_FC016\n" + ("%d %d\n" % m.shape).encode() + pairs.astype("<f8").tobytes()
+
+
+def test_excitation_vectors_bytes():
+    """Two vectors over a two-site XDims (site 0: one 1 x 2 block (0) -> (2); site 1: none), as the
+    generic vector save of BlockObj.hpp:3196-3222 writes std::vector<BlockMatArray<IKey, Complex>>."""
+    xd = BlockLayout(1, np.array([1, 0], np.int32), np.array([[0]], np.int32), np.array([[2]], np.int32), np.array([1], np.int32),
+                     np.array([2], np.int32))
+    X = np.array([[1 + 2j, 3 - 4j], [0.5j, -1.0]])
+    want = b"CBLOCKMATARRAYVEC\n" + _u32(2)
+    for x in X:
+        want += (b"CBLOCKMATARRAY\n" + _u32(2) + b"CBLOCKMAT\n" + _u32(1) + _key(0) + _key(2) + _cmat(x.reshape(1, 2))
+                 + b"CBLOCKMAT\n" + _u32(0))
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "x.bin")
+        imps_io.write_excitations(p, X, xd)
+        with open(p, "rb") as f:
+            assert f.read() == want
+        with pytest.raises(ValueError):
+            imps_io.write_excitations(p, X[:, :1], xd)
+
+
+def test_alrn_sections_in_order():
+    """saveALRN: C, L, R as BlockDiagMatArrays, then AL, AR, NL as MPSBlockMatArrays."""
+    data, objs = _fixture()
+    C, AL, AR = objs["C"], objs["AL"], objs["AR"]
+    L = [imps_io.diag_from_items(1, [((1,), np.array([[2.0, 0.0], [0.0, 3.0]]))])]
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "alrn.bin")
+        imps_io.write_alrn(p, C, L, C, AL, AR, AL)
+        with open(p, "rb") as f:
+            got = f.read()
+    diag = lambda m: b"RBLOCKDIAGMATARRAY\n" + _u32(1) + b"RBLOCKDIAGMAT\n" + _u32(1) + _key(1) + _mat(m)  # noqa: E731
+    i = data.index(b"RBLOCKMPSARRAY")
+    mps_al = data[i:data.index(b"RBLOCKMPSARRAY", i + 1)]
+    mps_ar = data[data.index(b"RBLOCKMPSARRAY", i + 1):]
+    Cm = np.array([[0.8, 0.0], [0.1, 0.6]])
+    assert got == diag(Cm) + diag(np.diag([2.0, 3.0])) + diag(Cm) + mps_al + mps_ar + mps_al

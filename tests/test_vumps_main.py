@@ -97,7 +97,7 @@ def test_excitation_settings_defaults():
 @pytest.mark.parametrize("bad", [["--model=XXZ", "--Delta=1", "--QN=[0]", "--K=[0]", "--pmin=0", "--pmax=1", "--np=2"],
                                  ["--model=XXZ", "--Delta=1", "--state=x", "--QN=[0]", "--K=[0]", "--pmin=0", "--pmax=1", "--np=0"],
                                  ["--model=XXZ", "--Delta=1", "--state=x", "--QN=[0]", "--K=[0]", "--pmin=0", "--pmax=1", "--np=2",
-                                  "--saveX=1"]])
+                                  "--prec=bf16x3"]])
 def test_excitation_settings_reject(bad):
     from imps_cpp_tools_amd import excitations_main as em
     with pytest.raises(ValueError):
