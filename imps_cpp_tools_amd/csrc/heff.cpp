@@ -449,6 +449,7 @@ void HeffEngine::build(imps_context c, int ncell, int dphys, const std::vector<L
     if (nmomenta < 1 || nmomenta > 4096) throw InvalidArg("heff: nmom must be 1..4096");
     if (!H) throw InvalidArg("heff: H is NULL");
     ctx = c;
+    if (const char* e = std::getenv("IMPS_HEFF_GMRES_M")) krylov = std::max(2, std::min(1000, std::atoi(e)));
     N = ncell;
     ncell_phase = ncell;
     d = dphys;

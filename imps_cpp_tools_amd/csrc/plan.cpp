@@ -4,8 +4,11 @@
 #include "adjoint.hpp"
 #include "heff.hpp"
 
+#include <cinttypes>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 imps_plan_s::~imps_plan_s() {
@@ -84,12 +87,69 @@ void launch_panel_schedule(imps_plan_s* P, int sched, const PanelLaunch& L, hipS
 // Autotuner: on the first apply of a plan with nvec vectors, every schedule runs the GEMM stages
 // once to warm up and three times under HIP events on the plan's stream; the fastest is kept for
 // the plan and that vector count.  Skipped while the stream is being captured into a graph.
+// Tuned schedules kept in a text file (IMPS_TUNE_CACHE=path): one "key schedule" line per tuned
+// (panel program, precision, vector count, input read), the key a 64-bit FNV-1a hash of the
+// program's task and chunk lists.  A plan whose key is in the file takes the recorded schedule
+// without timing, so a run (or a later one) repeats the first run's choices bitwise; a new key is
+// timed and appended.
+std::mutex g_tune_mu;
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// field by field: the structs' padding bytes are not part of the program
+uint64_t schedule_key(const imps_plan_s* P, int nvec, bool xt) {
+    uint64_t h = 1469598103934665603ull;
+    const int head[4] = {P->kind, P->prec, nvec, xt ? 1 : 0};
+    h = fnv(h, head, sizeof(head));
+    for (const PStage& st : P->pprog.stages) {
+        const long hs[3] = {st.out_slot, (long)st.tasks.size(), (long)st.chunks.size()};
+        h = fnv(h, hs, sizeof(hs));
+        for (const PTask& t : st.tasks) {
+            const long v[8] = {t.c_off, t.c_ld, t.m0, t.n0, t.M, t.N, t.ch0, t.nch};
+            h = fnv(h, v, sizeof(v));
+        }
+        for (const PChunk& c : st.chunks) {
+            const long v[8] = {c.raw_off, c.pan, c.pan_sub, c.s_idx, c.flags, c.krem, c.xoff, c.xks};
+            h = fnv(h, v, sizeof(v));
+        }
+    }
+    return h;
+}
+
+int cached_schedule(const char* path, uint64_t key) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return -1;
+    unsigned long long k = 0;
+    int sc = -1, found = -1;
+    while (std::fscanf(f, "%llx %d", &k, &sc) == 2)
+        if (k == key && sc >= 0 && sc < 4) found = sc;
+    std::fclose(f);
+    return found;
+}
+
 int choose_schedule(imps_plan_s* P, int nvec, const PanelLaunch& L, hipStream_t s) {
     if (forced_schedule() >= 0) return forced_schedule();
     auto it = P->sched.find(nvec);
     if (it != P->sched.end()) return it->second;
     const int def = default_schedule(P->prec, nvec);
     if (!autotune_enabled()) return def;
+    const char* cache = std::getenv("IMPS_TUNE_CACHE");
+    const uint64_t key = cache && cache[0] ? schedule_key(P, nvec, L.xt_direct) : 0;
+    if (key) {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        const int sc = cached_schedule(cache, key);
+        if (sc >= 0) {
+            P->sched[nvec] = sc;
+            return sc;
+        }
+    }
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     hip_check(hipStreamIsCapturing(s, &cap), "hipStreamIsCapturing");
     if (cap != hipStreamCaptureStatusNone) return def;
@@ -114,6 +174,14 @@ int choose_schedule(imps_plan_s* P, int nvec, const PanelLaunch& L, hipStream_t 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     P->sched[nvec] = best;
+    if (key) {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        if (cached_schedule(cache, key) < 0)
+            if (FILE* f = std::fopen(cache, "a")) {
+                std::fprintf(f, "%016" PRIx64 " %d\n", key, best);
+                std::fclose(f);
+            }
+    }
     return best;
 }
 

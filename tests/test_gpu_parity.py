@@ -241,3 +241,45 @@ def test_every_schedule_every_precision(ctx, sched, monkeypatch):
             got = y.cpu().numpy().reshape(5, -1)
             for v in range(5):
                 assert relerr(got[v], refs[v]) < TOL[prec], (direction, prec, v)
+
+
+def test_tuned_schedules_cached(ctx, tmp_path, monkeypatch):
+    """IMPS_TUNE_CACHE: the first plan times the four schedules and records its choice under the
+    program's key; a second plan of the same program takes the recorded schedule (rewritten here to
+    another one) without timing or appending, and still matches the oracle."""
+    for v in ("IMPS_SCHEDULE", "IMPS_KSPLIT"):
+        monkeypatch.delenv(v, raising=False)
+    monkeypatch.setenv("IMPS_AUTOTUNE", "1")
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("IMPS_TUNE_CACHE", str(path))
+    table = {0: (1, False), 1: (1, True), 2: (2, False), 3: (4, False)}
+    c = cell("fhub")
+    H = ham(c)
+    rng = np.random.default_rng(19)
+    HL = orc.reduced_op(H, c.d, c.AL[1], 0)
+    HR = orc.reduced_op(H, c.d, c.AR[1], 1)
+    EHL = random_like(c.diag(0), rng, sym=True)
+    EHR = random_like(c.diag(1), rng, sym=True)
+    xs = [random_like(c.struct[0], rng) for _ in range(4)]
+    X = np.concatenate([x.data for x in xs])
+    refs = [orc.hac(EHL, EHR, HL, HR, x).data for x in xs]
+
+    def run():
+        p = im.HACPlan(ctx, lay(xs[0]), lay(EHL), lay(EHR), lay(HL), lay(HR), im.FP32)
+        p.set_operator(dev(EHL.data), dev(EHR.data), dev(HL.data), dev(HR.data))
+        y = torch.full((X.size,), float("nan"), dtype=torch.float64, device="cuda")
+        p.apply(dev(X), y, 4)
+        got = y.cpu().numpy().reshape(4, -1)
+        for v in range(4):
+            assert relerr(got[v], refs[v]) < TOL[im.FP32]
+        return p
+    p1 = run()
+    words = path.read_text().split()
+    assert len(words) == 2
+    key, sc = words[0], int(words[1])
+    assert p1.schedule(4) == table[sc]
+    other = (sc + 1) % 4
+    path.write_text(f"{key} {other}\n")
+    p2 = run()
+    assert p2.schedule(4) == table[other]
+    assert path.read_text() == f"{key} {other}\n"
