@@ -33,6 +33,7 @@ hipError_t launch_d2f(const double* in, float* out, long n, hipStream_t stream);
 struct DevPStage {
     const PTask* tasks = nullptr;
     const PChunk* chunks = nullptr;
+    const RawPack* packs = nullptr;
     int ntasks = 0;
     int out_slot = SLOT_OUT;
 };

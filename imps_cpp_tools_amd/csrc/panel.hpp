@@ -26,6 +26,8 @@
 #pragma once
 
 #include <algorithm>
+#include <array>
+#include <cstring>
 #include <map>
 #include <numeric>
 #include <tuple>
@@ -53,7 +55,22 @@ inline long split_bytes(int prec, long len) { return (prec == IMPS_BF16X3 ? 4 : 
 // read it directly, element (idx, k) at xoff + idx + k * xks: each lane loads its k one by one, and
 // the sixteen lanes of one k read sixteen consecutive elements; no split copy is made then.  FP32
 // and BF16X3 convert the FP64 elements as the split copy does, plane-N chunks included.
-enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_RAW_W = 4, PCF_RAW_XN = 8, PCF_RAW_XT = 16 };
+// PCF_PACKED: the chunk's k-slots hold up to four short products of the task (K <= the k-run of a
+// lane group, one product per lane group: the sectors of dimension 1..4, or 1..8 in BF16X3), whose
+// panel is packed the same way; raw_off indexes the stage's RawPack table, which gives each lane
+// group its own raw block.
+enum PChunkFlag : int { PCF_PANEL_IS_A = 1, PCF_RAW_W = 4, PCF_RAW_XN = 8, PCF_RAW_XT = 16, PCF_PACKED = 32 };
+
+// Raw side of a packed chunk, per lane group g: raw(idx0, 0) of product g (raw_off, s_idx as in
+// PChunk), its K (krem; 0: no product, the group reads zeros) and, PCF_RAW_XT, its input offset
+// and k stride.
+struct RawPack {      // 96 bytes
+    long raw_off[4];
+    int s_idx[4];
+    int krem[4];
+    int xoff[4];
+    int xks[4];
+};
 
 struct PChunk {       // 40 bytes
     long raw_off;     // element index, within the (hi) N|T planes, of raw(idx0, k0)
@@ -73,13 +90,20 @@ struct PTask {        // 40 bytes
     int ch0, nch;
 };
 
-struct PanelDesc {    // one packed operator block
+struct PanelDesc {    // one packed operator block, or (npk > 0) one packed chunk's operator blocks
     long src;         // element offset in the FP64 operator store
     long frag0;       // first fragment index
     int ld, tr;       // P(r, k) = alpha * (tr ? S[k + r*ld] : S[r + k*ld])
     int R, K;
     int nrg, nkc;
     double alpha;
+    // npk > 0: one chunk (nkc = 1) of the nrg (<= 2) row groups from row r0, whose lane group
+    // g < npk holds block g: P(r, RUN g + q) = palpha[g] * (ptr[g] ? S[q + r*pld[g]] :
+    // S[r + q*pld[g]]) at psrc[g], q < pK[g], r0 <= r < R
+    int npk, r0;
+    long psrc[4];
+    double palpha[4];
+    int pld[4], ptr[4], pK[4];
 };
 
 struct SplitSpan {    // one input-vector block read as a raw factor: offset and shape
@@ -101,6 +125,7 @@ struct PStage {
     int out_slot = SLOT_OUT;
     std::vector<PTask> tasks;
     std::vector<PChunk> chunks;
+    std::vector<RawPack> packs;
 };
 
 struct PanelProgram {
@@ -150,7 +175,9 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
     if (split_bytes(prec, P.x_elems) >= (1L << 31) || split_bytes(prec, P.w_elems) >= (1L << 31))
         throw InvalidArg("panel: vector too long for 32-bit offsets");
     const int KC = panel_kc(prec);
+    const int RUN = panel_run(prec);
     std::map<std::tuple<long, int, double>, int> index;  // (src, tr, alpha) -> panel
+    std::map<std::array<long, 22>, int> pindex;            // packed chunk panel -> panel
     std::map<long, SplitSpan> xb;
     for (auto& st : P.stages) {
         PStage ps;
@@ -159,6 +186,9 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
         std::vector<std::vector<PChunk>> tchunks;
         for (auto& t : st.tasks) {
             std::vector<PChunk> chs;
+            // short products (K <= RUN) of one kind (panel role, raw slot, raw orientation) wait
+            // here to be packed four to a chunk
+            std::map<int, std::vector<int>> pending;
             for (int si = t.seg0; si < t.seg0 + t.nseg; ++si) {
                 const Seg& s = st.segs[si];
                 const bool pa = s.a_slot == SLOT_OP;
@@ -168,20 +198,6 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                 const int ld = pa ? s.a_ld : s.b_ld;
                 const int ptr = pa ? s.a_tr : (s.b_tr ? 0 : 1);
                 const int R = pa ? t.M : t.N;
-                auto key = std::make_tuple(src, ptr, s.alpha);
-                auto it = index.find(key);
-                if (it == index.end()) {
-                    PanelDesc pd;
-                    pd.src = src; pd.ld = ld; pd.tr = ptr; pd.R = R; pd.K = s.K; pd.alpha = s.alpha;
-                    pd.nrg = (R + 15) / 16;
-                    pd.nkc = (s.K + KC - 1) / KC;
-                    pd.frag0 = Q.nfrag;
-                    Q.nfrag += (long)pd.nrg * pd.nkc;
-                    it = index.emplace(key, (int)Q.panels.size()).first;
-                    Q.panels.push_back(pd);
-                }
-                const PanelDesc& pd = Q.panels[it->second];
-                if (pd.R != R || pd.K != s.K) throw InvalidArg("panel: operator block used with two shapes");
                 // raw side.  The stored raw block S (rows Rb, cols Cb, at boff) is used as
                 //   raw = A: op(A)(i, k) = a_tr ? S(k, i) : S(i, k)     (i runs over the output rows)
                 //   raw = B: op(B)(k, j) = b_tr ? S(j, k) : S(k, j)     (j runs over the output columns)
@@ -195,9 +211,27 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                 const int Rb = pa ? (rtr ? t.N : s.K) : (rtr ? s.K : t.M);
                 const int Cb = pa ? (rtr ? s.K : t.N) : (rtr ? t.M : s.K);
                 const bool kmajor = pa ? (rtr == 0) : (rtr != 0);
+                if (rslot == SLOT_X) xb[boff] = SplitSpan{boff, Rb, Cb};
+                if (s.K <= RUN) {
+                    pending[(pa ? 1 : 0) | (rslot == SLOT_W ? 2 : 0) | (kmajor ? 4 : 0)].push_back(si);
+                    continue;
+                }
+                auto key = std::make_tuple(src, ptr, s.alpha);
+                auto it = index.find(key);
+                if (it == index.end()) {
+                    PanelDesc pd{};
+                    pd.src = src; pd.ld = ld; pd.tr = ptr; pd.R = R; pd.K = s.K; pd.alpha = s.alpha;
+                    pd.nrg = (R + 15) / 16;
+                    pd.nkc = (s.K + KC - 1) / KC;
+                    pd.frag0 = Q.nfrag;
+                    Q.nfrag += (long)pd.nrg * pd.nkc;
+                    it = index.emplace(key, (int)Q.panels.size()).first;
+                    Q.panels.push_back(pd);
+                }
+                const PanelDesc& pd = Q.panels[it->second];
+                if (pd.R != R || pd.K != s.K) throw InvalidArg("panel: operator block used with two shapes");
                 const int idx0 = pa ? t.n0 : t.m0;
                 const long pl = rslot == SLOT_X ? Q.plx : Q.plw;
-                if (rslot == SLOT_X) xb[boff] = SplitSpan{boff, Rb, Cb};
                 const int s_idx = kmajor ? Rb : Cb;
                 const long base = kmajor ? boff : pl + boff;
                 const int rg0 = (pa ? t.m0 : t.n0) / 16;
@@ -215,6 +249,74 @@ inline PanelProgram build_panel_program(const Program& P, int prec) {
                     c.xoff = fx && !kmajor ? (int)(boff + idx0 + (long)kc * KC * Rb) : 0;
                     c.xks = fx && !kmajor ? Rb : 0;
                     c.pad = 0;
+                    chs.push_back(c);
+                }
+            }
+            for (auto& kv : pending) {
+                const bool pa = (kv.first & 1) != 0, fw = (kv.first & 2) != 0, kmajor = (kv.first & 4) != 0;
+                const bool fx = !fw;
+                const std::vector<int>& sis = kv.second;
+                for (size_t g0 = 0; g0 < sis.size(); g0 += 4) {
+                    const int ng = (int)std::min<size_t>(4, sis.size() - g0);
+                    const int rg0 = (pa ? t.m0 : t.n0) / 16;
+                    PanelDesc pd{};
+                    pd.R = pa ? t.M : t.N;
+                    pd.K = KC;
+                    pd.r0 = 16 * rg0;
+                    pd.nrg = std::min(2, (pd.R + 15) / 16 - rg0);  // the task's row groups only
+                    pd.nkc = 1;
+                    pd.alpha = 1.0;
+                    pd.npk = ng;
+                    RawPack rp{};
+                    const int idx0 = pa ? t.n0 : t.m0;
+                    for (int g = 0; g < ng; ++g) {
+                        const Seg& s = st.segs[sis[g0 + g]];
+                        pd.psrc[g] = pa ? s.a_off : s.b_off;
+                        pd.pld[g] = pa ? s.a_ld : s.b_ld;
+                        pd.ptr[g] = pa ? s.a_tr : (s.b_tr ? 0 : 1);
+                        pd.pK[g] = s.K;
+                        pd.palpha[g] = s.alpha;
+                        const long boff = pa ? s.b_off : s.a_off;
+                        const int rtr = pa ? s.b_tr : s.a_tr;
+                        const int Rb = pa ? (rtr ? t.N : s.K) : (rtr ? s.K : t.M);
+                        const int Cb = pa ? (rtr ? s.K : t.N) : (rtr ? t.M : s.K);
+                        const long pl = fx ? Q.plx : Q.plw;
+                        const int s_idx = kmajor ? Rb : Cb;
+                        rp.raw_off[g] = (kmajor ? boff : pl + boff) + (long)idx0 * s_idx;
+                        rp.s_idx[g] = s_idx;
+                        rp.krem[g] = s.K;
+                        rp.xoff[g] = fx && !kmajor ? (int)(boff + idx0) : 0;
+                        rp.xks[g] = fx && !kmajor ? Rb : 0;
+                    }
+                    // tasks that pack the same operator blocks over the same rows share the panel
+                    std::array<long, 22> key{};
+                    for (int g = 0; g < 4; ++g) {
+                        long abits;
+                        std::memcpy(&abits, &pd.palpha[g], sizeof abits);
+                        key[g] = pd.psrc[g]; key[4 + g] = pd.pld[g]; key[8 + g] = pd.ptr[g]; key[12 + g] = pd.pK[g]; key[16 + g] = abits;
+                    }
+                    key[20] = pd.r0;
+                    key[21] = pd.R;
+                    auto pit = pindex.find(key);
+                    if (pit == pindex.end()) {
+                        pd.frag0 = Q.nfrag;
+                        Q.nfrag += pd.nrg;
+                        pit = pindex.emplace(key, (int)Q.panels.size()).first;
+                        Q.panels.push_back(pd);
+                    }
+                    const PanelDesc& pq = Q.panels[pit->second];
+                    PChunk c;
+                    c.raw_off = (long)ps.packs.size();
+                    c.pan = (int)pq.frag0;
+                    c.pan_sub = pq.nrg > 1 ? 1 : -1;
+                    c.s_idx = 0;
+                    c.flags = PCF_PACKED | (pa ? PCF_PANEL_IS_A : 0) | (fw ? PCF_RAW_W : 0) | (fx && kmajor ? PCF_RAW_XN : 0) |
+                              (fx && !kmajor ? PCF_RAW_XT : 0);
+                    c.krem = KC;
+                    c.xoff = 0;
+                    c.xks = 0;
+                    c.pad = 0;
+                    ps.packs.push_back(rp);
                     chs.push_back(c);
                 }
             }

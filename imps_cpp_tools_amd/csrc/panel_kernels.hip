@@ -100,20 +100,33 @@ template <int PREC>
 __global__ __launch_bounds__(256) void panelize_kernel(const PanelDesc* __restrict__ panels, const double* __restrict__ op,
                                                        void* __restrict__ store) {
     using TR = PTraits<PREC>;
-    const PanelDesc pd = panels[blockIdx.x];
+    // by reference: a local copy indexed by lane group would live in scratch memory
+    const PanelDesc& pd = panels[blockIdx.x];
     const int nf = pd.nrg * pd.nkc;
     for (int e = threadIdx.x; e < nf * 64; e += blockDim.x) {
         const int f = e >> 6, lane = e & 63;
         const int rg = f / pd.nkc, kc = f - rg * pd.nkc;
-        const int r = 16 * rg + (lane & 15);
+        const int r = 16 * rg + (lane & 15) + (pd.npk > 0 ? pd.r0 : 0);
         const int kb = TR::KC * kc + TR::RUN * (lane >> 4);
         double v[TR::RUN];
+        if (pd.npk > 0) {
+            // packed chunk: lane group g holds block g's k = 0 .. pK[g] - 1 of rows r0 ..
+            const int g = lane >> 4;
 #pragma unroll
-        for (int q = 0; q < TR::RUN; ++q) {
-            const int k = kb + q;
-            double x = 0.0;
-            if (r < pd.R && k < pd.K) x = pd.alpha * op[pd.src + (pd.tr ? (long)k + (long)r * pd.ld : (long)r + (long)k * pd.ld)];
-            v[q] = x;
+            for (int q = 0; q < TR::RUN; ++q) {
+                double x = 0.0;
+                if (g < pd.npk && r < pd.R && q < pd.pK[g])
+                    x = pd.palpha[g] * op[pd.psrc[g] + (pd.ptr[g] ? (long)q + (long)r * pd.pld[g] : (long)r + (long)q * pd.pld[g])];
+                v[q] = x;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < TR::RUN; ++q) {
+                const int k = kb + q;
+                double x = 0.0;
+                if (r < pd.R && k < pd.K) x = pd.alpha * op[pd.src + (pd.tr ? (long)k + (long)r * pd.ld : (long)r + (long)k * pd.ld)];
+                v[q] = x;
+            }
         }
         const long frag = pd.frag0 + f;
         if constexpr (PREC == IMPS_FP64) {
@@ -239,13 +252,55 @@ struct Frag {
     typename PTraits<PREC>::Frg p[2], r[2];
 };
 
+// The raw block a lane group reads in one chunk: raw(idx0 + idx, k) at ro + idx * si + k for
+// k < kr (k-major planes), or, PCF_RAW_XT read from the input, at xo + idx + k * xk.  A packed
+// chunk takes them from its RawPack entry (one product per lane group), any other chunk from its
+// descriptor, shifted to the group's k-run.
+struct RawRef {
+    long ro;
+    int si, kr, xo, xk;
+};
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int pick4(const int* v, int lk) {  // v[lk] from four uniform values
+    const int a = uni(v[0]), b = uni(v[1]), c = uni(v[2]), d = uni(v[3]);
+    return lk < 2 ? (lk == 0 ? a : b) : (lk == 2 ? c : d);
+}
+template <int RUN>
+__device__ __forceinline__ RawRef raw_ref(const PChunk& c, const RawPack* __restrict__ packs, int lk) {
+    RawRef r;
+    if (c.flags & PCF_PACKED) {
+        // the whole entry by uniform (scalar) loads, then the lane group's fields by selects: a
+        // per-lane load would put a vector-memory round trip between the descriptor and the data
+        const RawPack* p = packs + c.raw_off;
+        const int* ro = reinterpret_cast<const int*>(p->raw_off);
+        int lo[4], hi[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            lo[g] = ro[2 * g];
+            hi[g] = ro[2 * g + 1];
+        }
+        r.ro = (long)(((unsigned long)(unsigned)pick4(hi, lk) << 32) | (unsigned)pick4(lo, lk));
+        r.si = pick4(p->s_idx, lk);
+        r.kr = pick4(p->krem, lk);
+        r.xo = pick4(p->xoff, lk);
+        r.xk = pick4(p->xks, lk);
+    } else {
+        r.ro = c.raw_off + RUN * lk;
+        r.si = c.s_idx;
+        r.kr = c.krem - RUN * lk;
+        r.xo = c.xoff + RUN * lk * c.xks;
+        r.xk = c.xks;
+    }
+    return r;
+}
+
 // Raw fragments of an input-vector chunk read from the FP64 input itself (rxn): lane (li, lk) takes
 // raw(idx0 + li (+ 16), k0 + RUN lk + q) for q < RUN.  PCF_RAW_XT: eight-byte loads along the stored
 // columns, each zero (no access) outside the block; PCF_RAW_XN: the lane's k-run is contiguous (16-
 // byte loads; k past K inside a run reads finite neighbouring data that meets zero panel entries, a
 // run wholly past K reads zeros).  FP32 and BF16X3 convert as the split copy does.
 template <int PREC>
-__device__ __forceinline__ void load_raw_x(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rxn, int nidx, int li, int lk) {
+__device__ __forceinline__ void load_raw_x(Frag<PREC>& F, const PChunk& c, const RawRef& rr, __amdgpu_buffer_rsrc_t rxn, int nidx, int li) {
     constexpr int RUN = PTraits<PREC>::RUN;
     const bool xt = (c.flags & PCF_RAW_XT) != 0;
 #pragma unroll
@@ -255,13 +310,12 @@ __device__ __forceinline__ void load_raw_x(Frag<PREC>& F, const PChunk& c, __amd
         if (xt) {
 #pragma unroll
             for (int q = 0; q < RUN; ++q) {
-                const int k = RUN * lk + q;
-                const unsigned off = (idx < nidx && k < c.krem) ? (unsigned)(c.xoff + idx + k * c.xks) * 8u : kOOB;
+                const unsigned off = (idx < nidx && q < rr.kr) ? (unsigned)(rr.xo + idx + q * rr.xk) * 8u : kOOB;
                 d[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rxn, off, 0, 0));
             }
         } else {
-            const bool ok = idx < nidx && RUN * lk < c.krem;
-            const unsigned off = ok ? (unsigned)(c.raw_off + (long)idx * c.s_idx + RUN * lk) * 8u : kOOB;
+            const bool ok = idx < nidx && rr.kr > 0;
+            const unsigned off = ok ? (unsigned)(rr.ro + (long)idx * rr.si) * 8u : kOOB;
 #pragma unroll
             for (int q = 0; q < RUN; q += 2) {
                 const pd2_t v = __builtin_bit_cast(pd2_t, ld16(rxn, ok ? off + 8u * q : kOOB));
@@ -292,38 +346,40 @@ __device__ __forceinline__ void load_raw_x(Frag<PREC>& F, const PChunk& c, __amd
 // selected by value: a selected reference to one (a lambda capture) makes the compiler keep the
 // candidates in scratch memory and load the chosen one.
 template <int PREC, bool XD>
-__device__ __forceinline__ void load_raw_frags(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rxn,
-                                               __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx, int li, int lk) {
+__device__ __forceinline__ void load_raw_frags(Frag<PREC>& F, const PChunk& c, const RawPack* __restrict__ packs, __amdgpu_buffer_rsrc_t rx,
+                                               __amdgpu_buffer_rsrc_t rxn, __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx,
+                                               int li, int lk) {
     constexpr int RUN = PTraits<PREC>::RUN;
     constexpr unsigned EB = PREC == IMPS_FP64 ? 8 : PREC == IMPS_FP32 ? 4 : 2;
+    const RawRef rr = raw_ref<RUN>(c, packs, lk);
     if constexpr (XD) {
         if (c.flags & (PCF_RAW_XT | (PREC == IMPS_FP64 ? 0 : PCF_RAW_XN))) {
-            load_raw_x(F, c, rxn, nidx, li, lk);
+            load_raw_x(F, c, rr, rxn, nidx, li);
             return;
         }
     }
     const bool w = (c.flags & PCF_RAW_W) != 0;
     const __amdgpu_buffer_rsrc_t rs = w ? rw : ((PREC == IMPS_FP64 && (c.flags & PCF_RAW_XN)) ? rxn : rx);
     const unsigned lo = w ? wlo : xlo;
-    const bool kok = RUN * lk < c.krem;
-    const unsigned e0 = (unsigned)(c.raw_off + (long)li * c.s_idx + RUN * lk) * EB;
-    const unsigned es = (unsigned)(16 * c.s_idx) * EB;
+    const bool kok = rr.kr > 0;
+    const unsigned e0 = (unsigned)(rr.ro + (long)li * rr.si) * EB;
+    const unsigned es = (unsigned)(16 * rr.si) * EB;
     F.r[0] = load_frg<PREC>(rs, (kok && li < nidx) ? e0 : kOOB, lo);
     F.r[1] = load_frg<PREC>(rs, (kok && 16 + li < nidx) ? e0 + es : kOOB, lo);
 }
 
 // Issue every load of one chunk: panel fragments (row groups rg0, rg0 + 1) and the raw fragments.
 template <int PREC, bool XD>
-__device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, __amdgpu_buffer_rsrc_t rp, __amdgpu_buffer_rsrc_t rx,
-                                           __amdgpu_buffer_rsrc_t rxn, __amdgpu_buffer_rsrc_t rw, unsigned xlo, unsigned wlo, int nidx,
-                                           int li, int lk, int lane) {
+__device__ __forceinline__ void load_chunk(Frag<PREC>& F, const PChunk& c, const RawPack* __restrict__ packs, __amdgpu_buffer_rsrc_t rp,
+                                           __amdgpu_buffer_rsrc_t rx, __amdgpu_buffer_rsrc_t rxn, __amdgpu_buffer_rsrc_t rw, unsigned xlo,
+                                           unsigned wlo, int nidx, int li, int lk, int lane) {
     constexpr unsigned FB = PREC == IMPS_FP32 ? 64 * 16 : 64 * 32;  // bytes of one panel fragment
     constexpr unsigned LB = PREC == IMPS_FP64 ? 32 : 16;            // bytes of one lane's run (hi plane)
     const unsigned p0 = (unsigned)c.pan * FB + LB * lane;
     const unsigned p1 = c.pan_sub >= 0 ? (unsigned)(c.pan + c.pan_sub) * FB + LB * lane : kOOB;
     F.p[0] = load_frg<PREC>(rp, p0, 1024u);
     F.p[1] = load_frg<PREC>(rp, p1, 1024u);
-    load_raw_frags<PREC, XD>(F, c, rx, rxn, rw, xlo, wlo, nidx, li, lk);
+    load_raw_frags<PREC, XD>(F, c, packs, rx, rxn, rw, xlo, wlo, nidx, li, lk);
 }
 
 // acc[a][b] += (col-side fragment b)(row-side fragment a), the transposed sub-tile (a, b), for
@@ -474,6 +530,7 @@ __device__ __forceinline__ void store_sub(const typename PTraits<PREC>::Acc& v, 
 struct PArgs {
     const PTask* tasks;
     const PChunk* chunks;
+    const RawPack* packs;  // the stage's packed-chunk table
     const char* pstore;
     long pstore_bytes;
     const char* xs;  // split copy of the input vectors
@@ -523,18 +580,18 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_kernel(const PArgs A)
             int c = t.ch0 + part;
             Frag<PREC> F0, F1;
             PChunk d0 = A.chunks[c];
-            load_chunk<PREC, XD>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+            load_chunk<PREC, XD>(F0, d0, A.packs, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
             while (true) {
                 // prefetch the next chunk (the current one again at the end: same count of loads)
                 const int c1 = c + S < cend ? c + S : c;
                 const PChunk d1 = A.chunks[c1];
-                load_chunk<PREC, XD>(F1, d1, rp, rx, rxn, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                load_chunk<PREC, XD>(F1, d1, A.packs, rp, rx, rxn, rw, xlo, wlo, (d1.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
                 compute_chunk_n<PREC>(acc, F0, (d0.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c1;
                 const int c2 = c + S < cend ? c + S : c;
                 d0 = A.chunks[c2];
-                load_chunk<PREC, XD>(F0, d0, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
+                load_chunk<PREC, XD>(F0, d0, A.packs, rp, rx, rxn, rw, xlo, wlo, (d0.flags & PCF_PANEL_IS_A) ? ncol : nrow, li, lk, lane);
                 compute_chunk_n<PREC>(acc, F1, (d1.flags & PCF_PANEL_IS_A) != 0, na, nb);
                 if (c + S >= cend) break;
                 c = c2;
@@ -619,7 +676,7 @@ __global__ __launch_bounds__(64 * kPWaves) void panel_gemm_shared_kernel(const P
     };
     // raw fragments of output-side indices idx0 + [0, 16) and idx0 + [16, 32) (zeros when inactive)
     auto load_raw = [&](Frag<PREC>& F, const PChunk& c) {
-        load_raw_frags<PREC, XD>(F, c, rx, rxn, rw, xlo, wlo, active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0, li, lk);
+        load_raw_frags<PREC, XD>(F, c, A.packs, rx, rxn, rw, xlo, wlo, active ? ((c.flags & PCF_PANEL_IS_A) ? ncol : nrow) : 0, li, lk);
     };
     // panel fragments of the chunk in LDS buffer b, in the lane layout of load_frg
     auto read_panels = [&](Frag<PREC>& F, int b) {
@@ -699,6 +756,7 @@ hipError_t launch_ps(const DevPStage& st, PArgs a, bool shared, hipStream_t stre
     a.nvg = (a.nvec + V - 1) / V;
     a.tasks = st.tasks;
     a.chunks = st.chunks;
+    a.packs = st.packs;
     const long nblk = (long)st.ntasks * a.nvg;
     if (nblk > 0x7fffffffL) return hipErrorInvalidValue;
     if constexpr (S == 1) {

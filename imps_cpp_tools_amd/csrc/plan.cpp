@@ -196,13 +196,15 @@ size_t put(std::vector<char>& host, size_t pos, const std::vector<T>& v) {
 void upload_panel_program(imps_plan_s* P) {
     P->pprog = build_panel_program(P->prog, P->prec);
     std::vector<char> host;
-    std::vector<size_t> tpos, cpos;
+    std::vector<size_t> tpos, cpos, kpos;
     size_t pos = 0;
     for (auto& st : P->pprog.stages) {
         tpos.push_back(pos = put(host, pos, st.tasks));
         pos += st.tasks.size() * sizeof(PTask);
         cpos.push_back(pos = put(host, pos, st.chunks));
         pos += st.chunks.size() * sizeof(PChunk);
+        kpos.push_back(pos = put(host, pos, st.packs));
+        pos += st.packs.size() * sizeof(RawPack);
     }
     const size_t ppos = put(host, pos, P->pprog.panels);
     pos = ppos + P->pprog.panels.size() * sizeof(PanelDesc);
@@ -219,6 +221,7 @@ void upload_panel_program(imps_plan_s* P) {
         DevPStage ds;
         ds.tasks = (const PTask*)(base + tpos[i]);
         ds.chunks = (const PChunk*)(base + cpos[i]);
+        ds.packs = (const RawPack*)(base + kpos[i]);
         ds.ntasks = (int)P->pprog.stages[i].tasks.size();
         ds.out_slot = P->pprog.stages[i].out_slot;
         P->dpstages.push_back(ds);

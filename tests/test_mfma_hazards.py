@@ -1,6 +1,9 @@
 """The HIP kernels' gfx950 assembly has no accumulator read that follows an MFMA writing it, through
-a branch, closer than that MFMA's result latency (see mfma_hazards.py).  Builds device assembly
-with the library's own compiler flags; CPU only."""
+a branch, closer than that MFMA's result latency (see mfma_hazards.py), and no kernel of the GEMM
+sources uses per-thread scratch memory (a spill, or a local array indexed at run time: every
+thread's copy costs HBM writes and reads).  Builds device assembly with the library's own compiler
+flags; CPU only."""
+import re
 import os
 import shutil
 import subprocess
@@ -13,7 +16,16 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "imps_cpp_tools_amd", "csrc")
 
 
+_ASM = {}
+
+
 def _device_asm(src, tmp_path):
+    if src not in _ASM:
+        _ASM[src] = _compile(src, tmp_path)
+    return _ASM[src]
+
+
+def _compile(src, tmp_path):
     from imps_cpp_tools_amd import build
     exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     if not os.path.exists(exe):
@@ -47,3 +59,11 @@ def test_no_mfma_result_reads_across_branches(src, tmp_path):
     assert "v_mfma" in asm
     bad = mh.hazards(asm)
     assert not bad, bad[:5]
+
+
+@pytest.mark.parametrize("src", ["panel_kernels.hip", "gemm_kernels.hip"])
+def test_no_scratch_memory(src, tmp_path):
+    asm = _device_asm(os.path.join(CSRC, src), tmp_path)
+    sizes = re.findall(r"; ScratchSize: (\d+)", asm)
+    assert sizes and all(int(x) == 0 for x in sizes), sizes
+    assert "scratch_store" not in asm and "scratch_load" not in asm
