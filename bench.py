@@ -80,6 +80,7 @@ def main():
     ap.add_argument("--no-xxz", action="store_true", help="skip the D = 64 Heisenberg VUMPS runs (keep the D = 1024 Hubbard one)")
     ap.add_argument("--no-exc", action="store_true")
     ap.add_argument("--no-tm", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--exc-maxit", type=int, default=1600, help="bound on the batched H_eff applications of the ground-state eigensolve")
     ap.add_argument("--exc-krylov", type=int, default=160, help="Lanczos basis size of the ground-state excitation eigensolve")
     ap.add_argument("--exc-mixed-eigs", action="store_true",
@@ -396,6 +397,86 @@ def main():
             log(f"tm D=512 {dname}: " + ", ".join(f"{k} {v['batched']['applies_per_s']:.3g}/s (x{NV}), {v['single']['applies_per_s']:.3g}/s (x1)"
                                                   for k, v in tres[dname].items()))
         out["tm_xxz_d512"] = tres
+
+    # Adjoints (differentiation workload): anisotropic Heisenberg (reference parameters Jxy = 1,
+    # Delta = -1.5), U(1), d = 2, D = 256 over 17 sectors, seeded normal entries.  Per FP64 call: the
+    # left TM apply and its vector-Jacobian product (x-bar and A-bar), the H_AC apply and its VJP
+    # (x-bar, EHL-, EHR-, HL-, HR-bar); each VJP is checked against a central difference (h = 1e-6)
+    # of <y-bar, f(inputs + h d)> along a random direction d.
+    if not args.no_adjoint:
+        ak, asz = wl.xxz_profile(17, 28, 2, 256)
+        ab = im.make_dims(im.XXZ, 2, [0], ak, asz)
+        arng = np.random.default_rng(1004)
+        aA = im.mps_structure(im.XXZ, 2, [0], ab[1], ab[0])
+        aAC = im.mps_structure(im.XXZ, 2, [0], ab[0], ab[1])
+        ax0, ax1 = wl.diag_layout(1, ab[1]), wl.diag_layout(1, ab[0])
+        nsteps = min(args.steps, 100)
+
+        def timed(fn):
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(nsteps):
+                fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / nsteps * 1e3
+
+        def rnd(n):
+            return dev(arng.standard_normal(n))
+
+        ares = {}
+        # left TM: x on bond ab[1] -> y on bond ab[0] through A: ab[1] -> ab[0]
+        tpa = im.TMPlan(ctx, im.LEFT, aA, None, ax0, ax1, im.FP64)
+        A_, x_, yb = dev(wl.random_data(aA, arng, scale_by_rows=True)), rnd(ax0.nelem), rnd(ax1.nelem)
+        tpa.set_mps(A_)
+        y_ = torch.empty(ax1.nelem, dtype=torch.float64, device="cuda")
+        xb_, Ab_ = torch.empty_like(x_), torch.empty_like(A_)
+        fwd = timed(lambda: tpa.apply(x_, y_, 1))
+        bwd = timed(lambda: tpa.vjp(A_, None, x_, yb, xb_, Ab_))
+        dx, dA, hh = rnd(x_.numel()), rnd(A_.numel()), 1e-6
+
+        def tm_f(x, A):
+            tpa.set_mps(A)
+            tpa.apply(x, y_, 1)
+            return torch.dot(yb, y_).item()
+        fd = (tm_f(x_ + hh * dx, A_ + hh * dA) - tm_f(x_ - hh * dx, A_ - hh * dA)) / (2 * hh)
+        an = (torch.dot(xb_, dx) + torch.dot(Ab_, dA)).item()
+        ares["tm_left"] = {"forward_us": fwd, "vjp_us": bwd, "vjp_per_s": 1e6 / bwd, "vjp_over_forward": bwd / fwd,
+                           "fd_relerr": abs(fd - an) / abs(an)}
+        # H_AC at the same bonds, HL / HR by the GPU ReducedOp of a random A
+        aH = im.model_ham(im.XXZ, [1.0, -1.5, 0.0])
+        arl, arr = im.RedOpPlan(ctx, aH, 2, aA, im.LEFT), im.RedOpPlan(ctx, aH, 2, aA, im.RIGHT)
+        aHL = torch.empty(arl.out_layout.nelem, dtype=torch.float64, device="cuda")
+        aHR = torch.empty(arr.out_layout.nelem, dtype=torch.float64, device="cuda")
+        arl.apply(A_, aHL)
+        arr.apply(A_, aHR)
+        aEHL, aEHR = wl.diag_layout(1, ab[0]), wl.diag_layout(1, ab[1])
+        ops = [dev(wl.random_data(aEHL, arng, symmetric=True)), dev(wl.random_data(aEHR, arng, symmetric=True)), aHL, aHR]
+        hp = im.HACPlan(ctx, aAC, aEHL, aEHR, arl.out_layout, arr.out_layout, im.FP64)
+        hx, hyb = rnd(aAC.nelem), rnd(aAC.nelem)
+        hy, hxb = torch.empty_like(hx), torch.empty_like(hx)
+        obars = [torch.empty_like(o) for o in ops]
+        hp.set_operator(*ops)
+        fwd = timed(lambda: hp.apply(hx, hy, 1))
+        bwd = timed(lambda: hp.vjp(ops, hx, hyb, hxb, obars))
+        dirs = [rnd(t.numel()) for t in [hx] + ops]
+
+        def hac_f(sign):
+            p = [t + sign * hh * d for t, d in zip([hx] + ops, dirs)]
+            hp.set_operator(*p[1:])
+            hp.apply(p[0], hy, 1)
+            return torch.dot(hyb, hy).item()
+        fd = (hac_f(1.0) - hac_f(-1.0)) / (2 * hh)
+        an = sum(torch.dot(g, d).item() for g, d in zip([hxb] + obars, dirs))
+        ares["hac"] = {"forward_us": fwd, "vjp_us": bwd, "vjp_per_s": 1e6 / bwd, "vjp_over_forward": bwd / fwd,
+                       "fd_relerr": abs(fd - an) / abs(an)}
+        ares["config"] = "XXZ (Jxy 1, Delta -1.5), U(1), d = 2, D = 256 over 17 sectors, FP64, one vector per call"
+        log("adjoints D=256: " + ", ".join(f"{k} fwd {v['forward_us']:.1f} us, vjp {v['vjp_us']:.1f} us, fd relerr {v['fd_relerr']:.1e}"
+                                          for k, v in ares.items() if isinstance(v, dict)))
+        out["adjoints_xxz_d256"] = ares
 
     # Quasiparticle excitations (config 5): the Fermi-Hubbard ground state at D = 512, reached as the
     # reference does (bond growth from D = 64, then VUMPS iterations to F <= 1e-10), and the 32
