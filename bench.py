@@ -401,8 +401,10 @@ def main():
     # Adjoints (differentiation workload): anisotropic Heisenberg (reference parameters Jxy = 1,
     # Delta = -1.5), U(1), d = 2, D = 256 over 17 sectors, seeded normal entries.  Per FP64 call: the
     # left TM apply and its vector-Jacobian product (x-bar and A-bar), the H_AC apply and its VJP
-    # (x-bar, EHL-, EHR-, HL-, HR-bar); each VJP is checked against a central difference (h = 1e-6)
-    # of <y-bar, f(inputs + h d)> along a random direction d.
+    # (x-bar, EHL-, EHR-, HL-, HR-bar); each VJP is checked against a central difference of
+    # <y-bar, f(inputs + h d)> along a random direction d: h = 1e-6 for the TM (cubic along the
+    # line), h = 1e-2 for H_AC (bilinear in x and the operators, so quadratic along the line and
+    # its central difference exact up to rounding).
     if not args.no_adjoint:
         ak, asz = wl.xxz_profile(17, 28, 2, 256)
         ab = im.make_dims(im.XXZ, 2, [0], ak, asz)
@@ -463,6 +465,8 @@ def main():
         fwd = timed(lambda: hp.apply(hx, hy, 1))
         bwd = timed(lambda: hp.vjp(ops, hx, hyb, hxb, obars))
         dirs = [rnd(t.numel()) for t in [hx] + ops]
+
+        hh = 1e-2
 
         def hac_f(sign):
             p = [t + sign * hh * d for t, d in zip([hx] + ops, dirs)]
